@@ -1,0 +1,169 @@
+/*
+ * syndrome_ldpc.h -- C ABI of the MI355X (gfx950) layered offset min-sum
+ * QC-LDPC decoder.
+ *
+ * Part 1 is a drop-in for the reference's C ABI (ThomasGl/syndrome,
+ * capi/src/lib.rs): same symbol names, argument order, argument meaning,
+ * status codes and validation order.  Buffers are host memory owned by the
+ * caller, as in the reference; the decode itself runs on the GPU.
+ *
+ * Part 2 extends it with what the reference has in Rust but not in its C
+ * ABI (raw-edge / Wi-Fi codes, the non-5G decode, the int8 path) and with
+ * device-resident batch entry points (device pointers, many codewords per
+ * launch) for callers that keep data in HBM.
+ *
+ * Part 3 is the work-queue batch decoder (uneven codes in one launch) and
+ * the serving loop (continuous batching of code blocks).
+ *
+ * Every entry point returns an int32 status (below) and never lets a C++
+ * exception cross the boundary (capi/src/lib.rs:117-119 `guard`).
+ */
+#ifndef SYNDROME_LDPC_H
+#define SYNDROME_LDPC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* SyndromeStatus (capi/src/lib.rs:74-94), plus two device-side codes. */
+#define SYNDROME_OK 0
+#define SYNDROME_INVALID_PARAM (-1)
+#define SYNDROME_CRC_MISMATCH (-2)
+#define SYNDROME_DECODER_NOT_CONVERGED (-3) /* reserved; never returned, as in the reference */
+#define SYNDROME_BUFFER_TOO_SMALL (-4)
+#define SYNDROME_NULL_POINTER (-5)
+#define SYNDROME_PANIC (-6)
+#define SYNDROME_DEVICE_ERROR (-7)          /* HIP runtime failure (no reference counterpart) */
+#define SYNDROME_BUSY (-8)                  /* admission refused: HBM budget / slots exhausted */
+
+typedef struct SyndromeLdpcDecoder SyndromeLdpcDecoder;
+typedef struct SyndromeLdpcEncoder SyndromeLdpcEncoder;
+
+/* ======================================================================
+ * Part 1: the reference C ABI (capi/src/lib.rs)
+ * ====================================================================== */
+
+/* capi/src/lib.rs:148-178.  base_graph 0 = BG1, 1 = BG2; z a 3GPP lifting
+ * size (Table 5.3.2-1).  On failure *out_decoder is set to NULL.  The
+ * decoder binds to the HIP device current at creation. */
+int32_t syndrome_ldpc_decoder_create(int32_t base_graph, size_t z, float offset_beta,
+                                     SyndromeLdpcDecoder **out_decoder);
+/* capi/src/lib.rs:187-193.  NULL is a no-op. */
+void syndrome_ldpc_decoder_destroy(SyndromeLdpcDecoder *decoder);
+/* capi/src/lib.rs:202-250 (QcLdpcDecoder size queries, qc_ldpc.rs:569-588). */
+size_t syndrome_ldpc_decoder_variable_node_count(const SyndromeLdpcDecoder *decoder);
+size_t syndrome_ldpc_decoder_info_bit_count(const SyndromeLdpcDecoder *decoder);
+size_t syndrome_ldpc_decoder_required_edge_buffer(const SyndromeLdpcDecoder *decoder);
+size_t syndrome_ldpc_decoder_required_layer_buffer(const SyndromeLdpcDecoder *decoder);
+
+/* capi/src/lib.rs:290-343 -> QcLdpcDecoder::decode_5g (qc_ldpc.rs:644-655).
+ * llr [N] in/out (channel LLRs in, a-posteriori out), edge_r [>= E*Z] out
+ * (final check-to-variable messages), layer_scratch [>= maxdeg*Z] validated
+ * only (its contents are left unspecified), hard [N] out, iterations = max
+ * layered passes; *out_iterations_used (nullable) = passes run. */
+int32_t syndrome_ldpc_decode_5g(SyndromeLdpcDecoder *decoder, float *llr, size_t llr_len, size_t n_filler,
+                                float *edge_r, size_t edge_r_len, float *layer_scratch, size_t layer_scratch_len,
+                                uint8_t *hard_output, size_t hard_output_len, size_t iterations,
+                                size_t *out_iterations_used);
+
+/* capi/src/lib.rs:369-484 (QcLdpcEncoder, qc_ldpc.rs:2408-2614).  Host-side
+ * systematic encoder: codeword = [info | parity] with H * codeword = 0. */
+int32_t syndrome_ldpc_encoder_create(int32_t base_graph, size_t z, SyndromeLdpcEncoder **out_encoder);
+void syndrome_ldpc_encoder_destroy(SyndromeLdpcEncoder *encoder);
+size_t syndrome_ldpc_encoder_info_bit_count(const SyndromeLdpcEncoder *encoder);
+size_t syndrome_ldpc_encoder_codeword_bit_count(const SyndromeLdpcEncoder *encoder);
+int32_t syndrome_ldpc_encode_5g(const SyndromeLdpcEncoder *encoder, const uint8_t *info_bits, size_t info_len,
+                                size_t n_filler, uint8_t *codeword, size_t codeword_len);
+
+/* ======================================================================
+ * Part 2: extensions (host and device buffers)
+ * ====================================================================== */
+
+/* QcLdpcDecoder::from_raw_edges (qc_ldpc.rs:412-455, 555-565): entries are
+ * (row_block, col_block, shift) triples, shift already < z.  Duplicate
+ * (row, col) pairs are rejected with INVALID_PARAM (the GPU schedule relies
+ * on each variable appearing at most once per layer). */
+int32_t syndrome_ldpc_decoder_create_raw(const uint8_t *rows, const uint8_t *cols, const int16_t *shifts,
+                                         size_t n_entries, size_t num_row_blocks, size_t num_col_blocks, size_t z,
+                                         float offset_beta, SyndromeLdpcDecoder **out_decoder);
+/* wifi_ldpc_decoder (wifi_ldpc_tables.rs:521-529): z in {27,54,81}, rate
+ * rate_num/rate_den in {1/2, 2/3, 3/4, 5/6}. */
+int32_t syndrome_ldpc_decoder_create_wifi(size_t z, size_t rate_num, size_t rate_den, float offset_beta,
+                                          SyndromeLdpcDecoder **out_decoder);
+size_t syndrome_ldpc_decoder_check_node_count(const SyndromeLdpcDecoder *decoder);
+float syndrome_ldpc_decoder_offset_beta(const SyndromeLdpcDecoder *decoder);
+
+/* QcLdpcDecoder::decode_layered_offset_min_sum (qc_ldpc.rs:794-810): as
+ * decode_5g without the 5G prologue. */
+int32_t syndrome_ldpc_decode(SyndromeLdpcDecoder *decoder, float *llr, size_t llr_len, float *edge_r,
+                             size_t edge_r_len, float *layer_scratch, size_t layer_scratch_len,
+                             uint8_t *hard_output, size_t hard_output_len, size_t iterations,
+                             size_t *out_iterations_used);
+
+/* QcLdpcDecoder::decode_layered_offset_min_sum_i8 (qc_ldpc.rs:1345-1363):
+ * app [N] i16 in/out, edge_r [>= E*Z] i8 out; QuantParams{scale, app_clamp}
+ * (quantize.rs:217-305). */
+int32_t syndrome_ldpc_decode_i8(SyndromeLdpcDecoder *decoder, int16_t *app, size_t app_len, int8_t *edge_r,
+                                size_t edge_r_len, int8_t *layer_scratch, size_t layer_scratch_len,
+                                uint8_t *hard_output, size_t hard_output_len, size_t iterations, float scale,
+                                int16_t app_clamp, size_t *out_iterations_used);
+/* QcLdpcDecoder::decode_5g_i8 (qc_ldpc.rs:1486-1523). */
+int32_t syndrome_ldpc_decode_5g_i8(SyndromeLdpcDecoder *decoder, int16_t *app, size_t app_len, size_t n_filler,
+                                   int8_t *edge_r, size_t edge_r_len, int8_t *layer_scratch,
+                                   size_t layer_scratch_len, uint8_t *hard_output, size_t hard_output_len,
+                                   size_t iterations, float scale, int16_t app_clamp, size_t *out_iterations_used);
+
+/* Batch decode flags. */
+#define SYNDROME_BATCH_INIT_5G 1u      /* apply the decode_5g / decode_5g_i8 prologue */
+#define SYNDROME_BATCH_WRITE_POST 2u   /* write posteriors to d_post */
+#define SYNDROME_BATCH_WRITE_HARD 4u   /* write hard decisions to d_hard */
+#define SYNDROME_BATCH_WRITE_EDGE 8u   /* write final R messages to d_edge */
+#define SYNDROME_BATCH_QUANTIZE 16u    /* int8 only: d_in is f32 LLR, quantize_llr_i16 fused into the load */
+#define SYNDROME_BATCH_PLAIN 256u      /* force the runtime-structured kernel (R streamed through HBM) */
+
+/* Device-resident batch decode, f32: `batch` codewords stacked in HBM.
+ * d_in f32 [batch][N]; d_post f32 [batch][N] (may equal d_in); d_edge f32
+ * [batch][E*Z] (required with SYNDROME_BATCH_PLAIN or WRITE_EDGE, else may
+ * be NULL); d_hard u8 [batch][N]; d_iters i32 [batch] (required).  Each
+ * codeword gets exactly the result of QcLdpcDecoder::decode_5g (INIT_5G) or
+ * decode_layered_offset_min_sum on it alone.  Asynchronous on `stream`
+ * (a hipStream_t; NULL = default stream). */
+int32_t syndrome_ldpc_decode_batch_f32(const SyndromeLdpcDecoder *decoder, const float *d_in, float *d_post,
+                                       float *d_edge, uint8_t *d_hard, int32_t *d_iters, size_t batch,
+                                       size_t iterations, size_t n_filler, uint32_t flags, void *stream);
+/* Device-resident batch decode, int8 messages / i16 posterior.  d_in is i16
+ * app [batch][N], or f32 LLR with SYNDROME_BATCH_QUANTIZE; d_post i16;
+ * d_edge i8. */
+int32_t syndrome_ldpc_decode_batch_i8(const SyndromeLdpcDecoder *decoder, const void *d_in, int16_t *d_post,
+                                      int8_t *d_edge, uint8_t *d_hard, int32_t *d_iters, size_t batch,
+                                      size_t iterations, size_t n_filler, float scale, int16_t app_clamp,
+                                      uint32_t flags, void *stream);
+
+/* Launch geometry the batch entry points use for this decoder:
+ * codewords per workgroup, threads per workgroup, LDS bytes. */
+int32_t syndrome_ldpc_launch_shape(const SyndromeLdpcDecoder *decoder, int32_t int8, int32_t plain,
+                                   int32_t *out_codewords_per_group, int32_t *out_threads, int32_t *out_lds_bytes);
+
+/* quantize_llr_i16 (quantize.rs:331-341) on device: f32 [n] -> i16 [n]. */
+int32_t syndrome_ldpc_quantize_i16_device(const float *d_llr, int16_t *d_out, size_t n, float scale, void *stream);
+
+/* ======================================================================
+ * Part 3: host helpers for traffic generation
+ * ====================================================================== */
+
+/* AwgnChannel::new(ebno_db, code_rate, seed).transmit(bits) (channel_sim.rs:139-215). */
+int32_t syndrome_awgn_transmit(float ebno_db, float code_rate, uint64_t seed, const uint8_t *bits, size_t n,
+                               float *llr_out);
+/* Systematic encode of `count` info blocks (info [count][K] -> codeword
+ * [count][N]) for any decoder's code (3GPP, Wi-Fi or raw), `threads` host
+ * threads (0 = all). */
+int32_t syndrome_ldpc_encode_batch(const SyndromeLdpcDecoder *decoder, const uint8_t *info, uint8_t *codewords,
+                                   size_t count, int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

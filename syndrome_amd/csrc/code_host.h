@@ -1,0 +1,50 @@
+// Host-side code construction, encoder and channel model.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "ldpc_common.h"
+
+namespace syn {
+
+// QcLdpcParams::new (src/qc_ldpc.rs:335-366): 3GPP BG1 (bg = 0) or BG2 (bg = 1).
+int build_code_3gpp(int bg, long long z, SynCode *out);
+// QcLdpcParams::from_raw_edges (src/qc_ldpc.rs:412-455); duplicate
+// (row, col) pairs are rejected.
+int build_code_raw(const int *rows, const int *cols, const int *shifts, size_t n, long long m_b, long long n_b,
+                   long long z, SynCode *out);
+// wifi_ldpc_matrix (src/wifi_ldpc_tables.rs:455-466).
+int build_code_wifi(long long z, long long rate_num, long long rate_den, SynCode *out);
+
+inline size_t code_n(const SynCode &c) { return (size_t)c.n_b * c.z; }
+inline size_t code_k(const SynCode &c) { return (size_t)(c.n_b - c.m_b) * c.z; }
+inline size_t code_ez(const SynCode &c) { return (size_t)c.n_edges * c.z; }
+inline size_t code_layer_buf(const SynCode &c) { return (size_t)c.max_deg * c.z; }
+
+// Systematic encoder: codeword = [info (K bits) | parity], H * codeword = 0.
+// The parity part of every supported H is invertible, so the codeword is
+// unique and equals QcLdpcEncoder::encode (src/qc_ldpc.rs:2592-2614).
+// The parity solve splits H's parity columns into a jointly solved core
+// (the first `core` block rows, via a precomputed GF(2) inverse) and
+// extension rows that each close on an identity block in their own parity
+// column (the 3GPP structure); codes without that structure are solved as
+// one core.
+class Encoder {
+public:
+    explicit Encoder(const SynCode &c);
+    bool ok() const { return ok_; }
+    void encode(const uint8_t *info, uint8_t *codeword) const;
+
+private:
+    SynCode c_;
+    bool ok_ = false;
+    int core_ = 0, bits_ = 0, words_ = 0;
+    std::vector<uint64_t> inv_;  // bits_ x words_ rows of the core inverse
+};
+
+// AwgnChannel (src/channel_sim.rs:61-215): BPSK over AWGN, LLR = 2 r / sigma^2.
+void awgn_transmit(float ebno_db, float code_rate, uint64_t seed, const uint8_t *bits, size_t n, float *llr_out);
+
+}  // namespace syn

@@ -1,0 +1,488 @@
+// C ABI of the MI355X LOMS decoder (declarations: include/syndrome_ldpc.h).
+//
+// Part 1 mirrors capi/src/lib.rs symbol for symbol: null checks first, then
+// the Rust validation order (qc_ldpc.rs:711-732, 980-1004), status codes
+// from capi/src/lib.rs:74-105, and no exception ever crosses the boundary
+// (the `guard` of capi/src/lib.rs:117-119).
+#include <hip/hip_runtime.h>
+
+#include <limits.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/syndrome_ldpc.h"
+#include "code_host.h"
+#include "ldpc_common.h"
+
+namespace syn {
+int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, const SynLaunch &a, int block, size_t lds,
+                  hipStream_t stream);
+int launch_quantize(const float *in, int16_t *out, size_t n, float scale, hipStream_t stream);
+}
+
+struct SyndromeLdpcDecoder {
+    SynCode code;
+    float beta = 0.f;
+    int device = 0;
+    int lds_limit = 0;
+    SynCode *d_code = nullptr;
+    // host-buffer entry points: serialised per handle, one staging area
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    void *d_stage = nullptr;
+    size_t stage_bytes = 0;
+    // lazily built encoder for traffic generation
+    std::once_flag enc_once;
+    std::unique_ptr<syn::Encoder> enc;
+};
+
+struct SyndromeLdpcEncoder {
+    SynCode code;
+    std::unique_ptr<syn::Encoder> enc;
+};
+
+namespace {
+
+template <class F>
+int32_t guard(F &&body)
+{
+    try {
+        return body();
+    } catch (...) {
+        return SYNDROME_PANIC;
+    }
+}
+
+// Restores the caller's current HIP device on scope exit.
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceScope()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+int32_t make_decoder(const SynCode &code, float beta, SyndromeLdpcDecoder **out)
+{
+    std::unique_ptr<SyndromeLdpcDecoder> d(new SyndromeLdpcDecoder());
+    d->code = code;
+    d->beta = beta;
+    if (hipGetDevice(&d->device) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    if (hipDeviceGetAttribute(&d->lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, d->device) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    if (hipMalloc((void **)&d->d_code, sizeof(SynCode)) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    if (hipMemcpy(d->d_code, &d->code, sizeof(SynCode), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d->d_code);
+        return SYNDROME_DEVICE_ERROR;
+    }
+    *out = d.release();
+    return SYNDROME_OK;
+}
+
+struct Shape {
+    int cws = 0, block = 0;
+    size_t lds = 0;
+};
+
+bool use_fast(const SynCode &c, bool plain) { return !plain && (c.kind == SYN_KIND_BG1 || c.kind == SYN_KIND_BG2); }
+
+// Codewords per workgroup: as many whole codewords as fit the lane budget
+// (512 lanes for the register-resident kernels, whose state takes up to 256
+// VGPRs; 1024 for the plain ones) and the LDS of one workgroup.
+int32_t launch_shape(const SyndromeLdpcDecoder *d, bool i8, bool plain, Shape *s)
+{
+    const SynCode &c = d->code;
+    const size_t n = syn::code_n(c), elem = i8 ? 2 : 4;
+    const int max_lanes = use_fast(c, plain) ? 512 : 1024;
+    auto bytes = [&](int k) { return ((k * n * elem + 3) & ~(size_t)3) + 3 * sizeof(int) * (size_t)k; };
+    int k = std::max(1, max_lanes / c.z);
+    while (k > 1 && bytes(k) > (size_t)d->lds_limit) --k;
+    if (bytes(k) > (size_t)d->lds_limit) return SYNDROME_INVALID_PARAM;
+    s->cws = k;
+    s->block = ((k * c.z + 63) / 64) * 64;
+    s->lds = bytes(k);
+    return SYNDROME_OK;
+}
+
+int32_t run_batch(const SyndromeLdpcDecoder *d, bool i8, const void *in, void *post, void *edge, uint8_t *hard,
+                  int32_t *iters, size_t batch, size_t iterations, size_t n_filler, float scale, int16_t app_clamp,
+                  uint32_t flags, hipStream_t stream)
+{
+    if (batch == 0) return SYNDROME_OK;
+    if (batch > (size_t)INT_MAX / 2) return SYNDROME_INVALID_PARAM;
+    const bool plain = (flags & SYNDROME_BATCH_PLAIN) != 0;
+    if (!in || !iters) return SYNDROME_NULL_POINTER;
+    if ((flags & SYNDROME_BATCH_WRITE_POST) && !post) return SYNDROME_NULL_POINTER;
+    if ((flags & SYNDROME_BATCH_WRITE_HARD) && !hard) return SYNDROME_NULL_POINTER;
+    const bool fast = use_fast(d->code, plain);
+    if ((!fast || (flags & SYNDROME_BATCH_WRITE_EDGE)) && !edge) return SYNDROME_NULL_POINTER;
+    Shape s;
+    if (int32_t rc = launch_shape(d, i8, plain, &s)) return rc;
+    SynLaunch a{};
+    a.in = in;
+    a.post = post;
+    a.edge = edge;
+    a.hard = hard;
+    a.iters = iters;
+    a.batch = (int)batch;
+    a.iterations = (int)std::min(iterations, (size_t)INT_MAX);
+    a.cws = s.cws;
+    a.flags = flags & 0xffu;
+    a.n_filler = (long long)std::min(n_filler, (size_t)LLONG_MAX);
+    a.beta = d->beta;
+    a.scale = scale;
+    a.app_clamp = app_clamp < 0 ? 0 : app_clamp;
+    if (i8) {
+        // QuantParams::beta_q (quantize.rs:298-304): round half away from
+        // zero, clamp to [0, 127], non-finite -> 127.
+        const float b = d->beta * scale;
+        a.beta_q = !(b - b == 0.f) ? 127 : (int)std::min(127.f, std::max(0.f, roundf(b)));
+    }
+    DeviceScope ds(d->device);
+    return syn::launch_decode(d->code.kind, i8, fast, d->d_code, a, s.block, s.lds, stream);
+}
+
+int32_t ensure_stage(SyndromeLdpcDecoder *d, size_t bytes)
+{
+    if (!d->stream && hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    if (d->stage_bytes >= bytes) return SYNDROME_OK;
+    if (d->d_stage) (void)hipFree(d->d_stage);
+    d->d_stage = nullptr;
+    d->stage_bytes = 0;
+    if (hipMalloc(&d->d_stage, bytes) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    d->stage_bytes = bytes;
+    return SYNDROME_OK;
+}
+
+// One codeword through the GPU from host buffers: the path behind the
+// reference-shaped entry points.  `elem` = 4 (f32 posterior, f32 messages)
+// or 2 (i16 posterior, i8 messages).
+int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, size_t edge_elems, uint8_t *hard,
+                    size_t iterations, size_t n_filler, bool init5g, float scale, int16_t app_clamp,
+                    size_t *out_used)
+{
+    const SynCode &c = d->code;
+    const size_t n = syn::code_n(c), ez = syn::code_ez(c);
+    const size_t pe = i8 ? 2 : 4, me = i8 ? 1 : 4;
+    const size_t off_edge = (n * pe + 255) & ~(size_t)255;
+    const size_t off_hard = off_edge + ((ez * me + 255) & ~(size_t)255);
+    const size_t off_iter = off_hard + ((n + 255) & ~(size_t)255);
+    const size_t total = off_iter + 256;
+    std::lock_guard<std::mutex> lk(d->mu);
+    DeviceScope ds(d->device);
+    if (int32_t rc = ensure_stage(d, total)) return rc;
+    char *base = (char *)d->d_stage;
+    const uint32_t flags = (init5g ? SYNDROME_BATCH_INIT_5G : 0u) | SYNDROME_BATCH_WRITE_POST |
+                           SYNDROME_BATCH_WRITE_HARD | SYNDROME_BATCH_WRITE_EDGE;
+    if (hipMemcpyAsync(base, llr, n * pe, hipMemcpyHostToDevice, d->stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    int32_t rc = run_batch(d, i8, base, base, base + off_edge, (uint8_t *)(base + off_hard), (int32_t *)(base + off_iter),
+                           1, iterations, n_filler, scale, app_clamp, flags, d->stream);
+    if (rc) return rc;
+    int32_t used = 0;
+    if (hipMemcpyAsync(llr, base, n * pe, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+        hipMemcpyAsync(edge_r, base + off_edge, ez * me, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+        hipMemcpyAsync(hard, base + off_hard, n, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+        hipMemcpyAsync(&used, base + off_iter, 4, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+        hipStreamSynchronize(d->stream) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    (void)edge_elems;
+    if (out_used) *out_used = (size_t)used;
+    return SYNDROME_OK;
+}
+
+// init_5g_llr on host memory (src/qc_ldpc.rs:711-732): applied when a later
+// validation step fails, so the caller's buffer is left as the reference
+// leaves it (the prologue runs before the decode's own checks).
+template <class T>
+void init_5g_host(const SynCode &c, T *buf, size_t n_filler, T filler)
+{
+    const size_t k = syn::code_k(c);
+    const size_t kp = n_filler >= k ? 0 : k - n_filler;
+    for (size_t i = kp; i < k; ++i) buf[i] = filler;
+    for (size_t i = 0; i < (size_t)(2 * c.z); ++i) buf[i] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Part 1
+// ---------------------------------------------------------------------------
+
+int32_t syndrome_ldpc_decoder_create(int32_t base_graph, size_t z, float offset_beta, SyndromeLdpcDecoder **out)
+{
+    return guard([&]() -> int32_t {
+        if (!out) return SYNDROME_NULL_POINTER;
+        *out = nullptr;
+        if (base_graph != 0 && base_graph != 1) return SYNDROME_INVALID_PARAM;
+        SynCode c;
+        if (syn::build_code_3gpp(base_graph, z > (size_t)INT_MAX ? -1 : (long long)z, &c)) return SYNDROME_INVALID_PARAM;
+        return make_decoder(c, offset_beta, out);
+    });
+}
+
+void syndrome_ldpc_decoder_destroy(SyndromeLdpcDecoder *d)
+{
+    if (!d) return;
+    try {
+        DeviceScope ds(d->device);
+        if (d->d_code) (void)hipFree(d->d_code);
+        if (d->d_stage) (void)hipFree(d->d_stage);
+        if (d->stream) (void)hipStreamDestroy(d->stream);
+    } catch (...) {
+    }
+    delete d;
+}
+
+size_t syndrome_ldpc_decoder_variable_node_count(const SyndromeLdpcDecoder *d) { return syn::code_n(d->code); }
+size_t syndrome_ldpc_decoder_info_bit_count(const SyndromeLdpcDecoder *d) { return syn::code_k(d->code); }
+size_t syndrome_ldpc_decoder_required_edge_buffer(const SyndromeLdpcDecoder *d) { return syn::code_ez(d->code); }
+size_t syndrome_ldpc_decoder_required_layer_buffer(const SyndromeLdpcDecoder *d)
+{
+    return syn::code_layer_buf(d->code);
+}
+
+int32_t syndrome_ldpc_decode_5g(SyndromeLdpcDecoder *d, float *llr, size_t llr_len, size_t n_filler, float *edge_r,
+                                size_t edge_r_len, float *layer_scratch, size_t layer_scratch_len, uint8_t *hard,
+                                size_t hard_len, size_t iterations, size_t *out_used)
+{
+    return guard([&]() -> int32_t {
+        if (!d || !llr || !edge_r || !layer_scratch || !hard) return SYNDROME_NULL_POINTER;
+        const SynCode &c = d->code;
+        const size_t n = syn::code_n(c);
+        if (llr_len != n) return SYNDROME_INVALID_PARAM;  // init_5g_llr
+        int32_t rc = SYNDROME_OK;
+        if (hard_len != n) rc = SYNDROME_INVALID_PARAM;
+        else if (edge_r_len < syn::code_ez(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
+        else if (layer_scratch_len < syn::code_layer_buf(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
+        if (rc) {
+            init_5g_host<float>(c, llr, n_filler, 1000000.0f);
+            return rc;
+        }
+        return host_decode(d, false, llr, edge_r, edge_r_len, hard, iterations, n_filler, true, 0.f, 0, out_used);
+    });
+}
+
+int32_t syndrome_ldpc_encoder_create(int32_t base_graph, size_t z, SyndromeLdpcEncoder **out)
+{
+    return guard([&]() -> int32_t {
+        if (!out) return SYNDROME_NULL_POINTER;
+        *out = nullptr;
+        if (base_graph != 0 && base_graph != 1) return SYNDROME_INVALID_PARAM;
+        std::unique_ptr<SyndromeLdpcEncoder> e(new SyndromeLdpcEncoder());
+        if (syn::build_code_3gpp(base_graph, z > (size_t)INT_MAX ? -1 : (long long)z, &e->code))
+            return SYNDROME_INVALID_PARAM;
+        e->enc.reset(new syn::Encoder(e->code));
+        if (!e->enc->ok()) return SYNDROME_INVALID_PARAM;
+        *out = e.release();
+        return SYNDROME_OK;
+    });
+}
+
+void syndrome_ldpc_encoder_destroy(SyndromeLdpcEncoder *e) { delete e; }
+size_t syndrome_ldpc_encoder_info_bit_count(const SyndromeLdpcEncoder *e) { return syn::code_k(e->code); }
+size_t syndrome_ldpc_encoder_codeword_bit_count(const SyndromeLdpcEncoder *e) { return syn::code_n(e->code); }
+
+int32_t syndrome_ldpc_encode_5g(const SyndromeLdpcEncoder *e, const uint8_t *info, size_t info_len, size_t n_filler,
+                                uint8_t *codeword, size_t codeword_len)
+{
+    return guard([&]() -> int32_t {
+        if (!e || !info || !codeword) return SYNDROME_NULL_POINTER;
+        // QcLdpcEncoder::encode_5g / encode (qc_ldpc.rs:2547-2606)
+        const size_t k = syn::code_k(e->code), n = syn::code_n(e->code);
+        if (n_filler > k) return SYNDROME_INVALID_PARAM;
+        if (info_len != k - n_filler) return SYNDROME_BUFFER_TOO_SMALL;
+        if (codeword_len != n) return SYNDROME_BUFFER_TOO_SMALL;
+        std::vector<uint8_t> full(k, 0);
+        memcpy(full.data(), info, info_len);
+        e->enc->encode(full.data(), codeword);
+        return SYNDROME_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Part 2
+// ---------------------------------------------------------------------------
+
+int32_t syndrome_ldpc_decoder_create_raw(const uint8_t *rows, const uint8_t *cols, const int16_t *shifts, size_t n,
+                                         size_t m_b, size_t n_b, size_t z, float offset_beta,
+                                         SyndromeLdpcDecoder **out)
+{
+    return guard([&]() -> int32_t {
+        if (!out) return SYNDROME_NULL_POINTER;
+        *out = nullptr;
+        if (n > 0 && (!rows || !cols || !shifts)) return SYNDROME_NULL_POINTER;
+        if (n > SYN_MAX_EDGES || m_b > SYN_MAX_ROWS || n_b > SYN_MAX_COLS || z > SYN_Z_MAX)
+            return SYNDROME_INVALID_PARAM;
+        std::vector<int> r(n), c(n), s(n);
+        for (size_t i = 0; i < n; ++i) {
+            r[i] = rows[i];
+            c[i] = cols[i];
+            s[i] = shifts[i];
+        }
+        SynCode code;
+        if (syn::build_code_raw(r.data(), c.data(), s.data(), n, (long long)m_b, (long long)n_b, (long long)z, &code))
+            return SYNDROME_INVALID_PARAM;
+        return make_decoder(code, offset_beta, out);
+    });
+}
+
+int32_t syndrome_ldpc_decoder_create_wifi(size_t z, size_t rate_num, size_t rate_den, float offset_beta,
+                                          SyndromeLdpcDecoder **out)
+{
+    return guard([&]() -> int32_t {
+        if (!out) return SYNDROME_NULL_POINTER;
+        *out = nullptr;
+        SynCode code;
+        if (z > SYN_Z_MAX || rate_num > 16 || rate_den > 16 ||
+            syn::build_code_wifi((long long)z, (long long)rate_num, (long long)rate_den, &code))
+            return SYNDROME_INVALID_PARAM;
+        return make_decoder(code, offset_beta, out);
+    });
+}
+
+size_t syndrome_ldpc_decoder_check_node_count(const SyndromeLdpcDecoder *d) { return (size_t)d->code.m_b * d->code.z; }
+float syndrome_ldpc_decoder_offset_beta(const SyndromeLdpcDecoder *d) { return d->beta; }
+
+int32_t syndrome_ldpc_decode(SyndromeLdpcDecoder *d, float *llr, size_t llr_len, float *edge_r, size_t edge_r_len,
+                             float *layer_scratch, size_t layer_scratch_len, uint8_t *hard, size_t hard_len,
+                             size_t iterations, size_t *out_used)
+{
+    return guard([&]() -> int32_t {
+        if (!d || !llr || !edge_r || !layer_scratch || !hard) return SYNDROME_NULL_POINTER;
+        const SynCode &c = d->code;
+        const size_t n = syn::code_n(c);
+        if (llr_len != n || hard_len != n) return SYNDROME_INVALID_PARAM;  // qc_ldpc.rs:980-1004
+        if (edge_r_len < syn::code_ez(c) || layer_scratch_len < syn::code_layer_buf(c)) return SYNDROME_BUFFER_TOO_SMALL;
+        return host_decode(d, false, llr, edge_r, edge_r_len, hard, iterations, 0, false, 0.f, 0, out_used);
+    });
+}
+
+int32_t syndrome_ldpc_decode_i8(SyndromeLdpcDecoder *d, int16_t *app, size_t app_len, int8_t *edge_r, size_t edge_r_len,
+                                int8_t *layer_scratch, size_t layer_scratch_len, uint8_t *hard, size_t hard_len,
+                                size_t iterations, float scale, int16_t app_clamp, size_t *out_used)
+{
+    return guard([&]() -> int32_t {
+        if (!d || !app || !edge_r || !layer_scratch || !hard) return SYNDROME_NULL_POINTER;
+        const SynCode &c = d->code;
+        const size_t n = syn::code_n(c);
+        if (app_len != n || hard_len != n) return SYNDROME_INVALID_PARAM;  // qc_ldpc.rs:1544-1568
+        if (edge_r_len < syn::code_ez(c) || layer_scratch_len < syn::code_layer_buf(c)) return SYNDROME_BUFFER_TOO_SMALL;
+        return host_decode(d, true, app, edge_r, edge_r_len, hard, iterations, 0, false, scale, app_clamp, out_used);
+    });
+}
+
+int32_t syndrome_ldpc_decode_5g_i8(SyndromeLdpcDecoder *d, int16_t *app, size_t app_len, size_t n_filler, int8_t *edge_r,
+                                   size_t edge_r_len, int8_t *layer_scratch, size_t layer_scratch_len, uint8_t *hard,
+                                   size_t hard_len, size_t iterations, float scale, int16_t app_clamp,
+                                   size_t *out_used)
+{
+    return guard([&]() -> int32_t {
+        if (!d || !app || !edge_r || !layer_scratch || !hard) return SYNDROME_NULL_POINTER;
+        const SynCode &c = d->code;
+        const size_t n = syn::code_n(c);
+        if (app_len != n) return SYNDROME_INVALID_PARAM;  // qc_ldpc.rs:1499-1503
+        int32_t rc = SYNDROME_OK;
+        if (hard_len != n) rc = SYNDROME_INVALID_PARAM;
+        else if (edge_r_len < syn::code_ez(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
+        else if (layer_scratch_len < syn::code_layer_buf(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
+        if (rc) {
+            init_5g_host<int16_t>(c, app, n_filler, app_clamp < 0 ? (int16_t)0 : app_clamp);
+            return rc;
+        }
+        return host_decode(d, true, app, edge_r, edge_r_len, hard, iterations, n_filler, true, scale, app_clamp,
+                           out_used);
+    });
+}
+
+int32_t syndrome_ldpc_decode_batch_f32(const SyndromeLdpcDecoder *d, const float *in, float *post, float *edge,
+                                       uint8_t *hard, int32_t *iters, size_t batch, size_t iterations, size_t n_filler,
+                                       uint32_t flags, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if (!d) return SYNDROME_NULL_POINTER;
+        if (flags & SYNDROME_BATCH_QUANTIZE) return SYNDROME_INVALID_PARAM;
+        return run_batch(d, false, in, post, edge, hard, iters, batch, iterations, n_filler, 0.f, 0, flags,
+                         (hipStream_t)stream);
+    });
+}
+
+int32_t syndrome_ldpc_decode_batch_i8(const SyndromeLdpcDecoder *d, const void *in, int16_t *post, int8_t *edge,
+                                      uint8_t *hard, int32_t *iters, size_t batch, size_t iterations, size_t n_filler,
+                                      float scale, int16_t app_clamp, uint32_t flags, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if (!d) return SYNDROME_NULL_POINTER;
+        return run_batch(d, true, in, post, edge, hard, iters, batch, iterations, n_filler, scale, app_clamp, flags,
+                         (hipStream_t)stream);
+    });
+}
+
+int32_t syndrome_ldpc_launch_shape(const SyndromeLdpcDecoder *d, int32_t int8, int32_t plain, int32_t *cws,
+                                   int32_t *threads, int32_t *lds)
+{
+    return guard([&]() -> int32_t {
+        if (!d || !cws || !threads || !lds) return SYNDROME_NULL_POINTER;
+        Shape s;
+        if (int32_t rc = launch_shape(d, int8 != 0, plain != 0, &s)) return rc;
+        *cws = s.cws;
+        *threads = s.block;
+        *lds = (int32_t)s.lds;
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_ldpc_quantize_i16_device(const float *d_llr, int16_t *d_out, size_t n, float scale, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if ((!d_llr || !d_out) && n) return SYNDROME_NULL_POINTER;
+        return syn::launch_quantize(d_llr, d_out, n, scale, (hipStream_t)stream);
+    });
+}
+
+int32_t syndrome_awgn_transmit(float ebno_db, float code_rate, uint64_t seed, const uint8_t *bits, size_t n,
+                               float *llr_out)
+{
+    return guard([&]() -> int32_t {
+        if ((!bits || !llr_out) && n) return SYNDROME_NULL_POINTER;
+        syn::awgn_transmit(ebno_db, code_rate, seed, bits, n, llr_out);
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_ldpc_encode_batch(const SyndromeLdpcDecoder *d, const uint8_t *info, uint8_t *cws, size_t count,
+                                   int32_t threads)
+{
+    return guard([&]() -> int32_t {
+        if (!d || ((!info || !cws) && count)) return SYNDROME_NULL_POINTER;
+        SyndromeLdpcDecoder *md = const_cast<SyndromeLdpcDecoder *>(d);
+        std::call_once(md->enc_once, [&] { md->enc.reset(new syn::Encoder(md->code)); });
+        if (!md->enc || !md->enc->ok()) return SYNDROME_INVALID_PARAM;
+        const size_t k = syn::code_k(d->code), n = syn::code_n(d->code);
+        int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+        nt = (int)std::min<size_t>((size_t)nt, std::max<size_t>(count, 1));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                for (size_t i = (size_t)t; i < count; i += (size_t)nt) md->enc->encode(info + i * k, cws + i * n);
+            });
+        for (auto &th : pool) th.join();
+        return SYNDROME_OK;
+    });
+}
+
+}  // extern "C"

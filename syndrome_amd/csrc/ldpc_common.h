@@ -1,0 +1,72 @@
+// Shared host/device definitions for the MI355X LOMS decoder.
+//
+// Layout conventions (one codeword):
+//   posterior / channel LLR : f32 [N] or i16 [N], N = n_b * Z, index col*Z + z
+//   check-to-variable R     : f32 / i8 [E*Z], index (layer_begin + edge)*Z + z
+//                             (the reference's edge_r layout, src/qc_ldpc.rs:1070-1076)
+//   hard decisions          : u8 [N], 0/1
+// A batch stacks codewords contiguously: codeword b starts at b*N (b*E*Z for R).
+#pragma once
+#include <stdint.h>
+
+#define SYN_MAX_ROWS 46
+#define SYN_MAX_COLS 68
+#define SYN_MAX_EDGES 316
+#define SYN_MAX_DEG 19
+#define SYN_Z_MAX 384
+
+// Status codes: capi/src/lib.rs:74-94.
+#define SYN_OK 0
+#define SYN_INVALID_PARAM (-1)
+#define SYN_CRC_MISMATCH (-2)
+#define SYN_NOT_CONVERGED (-3)
+#define SYN_BUFFER_TOO_SMALL (-4)
+#define SYN_NULL_POINTER (-5)
+#define SYN_PANIC (-6)
+#define SYN_DEVICE_ERROR (-7)
+#define SYN_BUSY (-8)
+
+// Base-graph family a code was built from: selects the register-resident
+// kernel specialisation (kind 0/1) or the runtime-structured one (raw).
+#define SYN_KIND_BG1 0
+#define SYN_KIND_BG2 1
+#define SYN_KIND_RAW 2
+
+// Runtime description of one lifted code, QcLdpcParams (src/qc_ldpc.rs:233-253).
+// Lives in device memory; every field is read with wave-uniform (scalar) loads.
+struct SynCode {
+    int kind;
+    int z;
+    int m_b;
+    int n_b;
+    int n_edges;
+    int max_deg;
+    int layer_off[SYN_MAX_ROWS + 1];
+    int cols[SYN_MAX_EDGES];
+    int shifts[SYN_MAX_EDGES];  // reduced mod z
+};
+
+// Decode-call flags.
+#define SYN_F_INIT_5G 1u        // apply init_5g_llr / decode_5g_i8 prologue on load
+#define SYN_F_WRITE_POST 2u     // write the posterior back
+#define SYN_F_WRITE_HARD 4u     // write hard decisions
+#define SYN_F_WRITE_EDGE 8u     // write final R messages (edge_r)
+#define SYN_F_QUANTIZE_IN 16u   // i8 path: input is f32 LLR, quantize_llr_i16 on load
+
+// Per-launch arguments shared by every decode kernel.
+struct SynLaunch {
+    const void *in;        // f32 [B][N] (f32 path or SYN_F_QUANTIZE_IN) or i16 [B][N]
+    void *post;            // f32 / i16 [B][N]   (may equal `in`)
+    void *edge;            // f32 / i8 [B][E*Z]  (plain kernels: required working buffer)
+    uint8_t *hard;         // u8 [B][N]
+    int *iters;            // i32 [B]
+    int batch;
+    int iterations;
+    int cws;               // codewords per workgroup
+    unsigned flags;
+    long long n_filler;
+    float beta;            // f32 offset
+    float scale;           // i8 path: quantizer scale
+    int beta_q;            // i8 path: quantized offset
+    int app_clamp;         // i8 path: max(app_clamp, 0)
+};

@@ -1,0 +1,250 @@
+"""Python mirror of the reference's decoder API over the HIP C ABI.
+
+``QcLdpcDecoder`` follows ``syndrome::qc_ldpc::QcLdpcDecoder``
+(src/qc_ldpc.rs:481-1963): the same constructors, size queries, decode
+entry points, argument meaning and error behaviour (errors surface as
+:class:`SyndromeError` carrying the reference's status code).  Host arrays
+are numpy; the decode runs on the GPU.  ``decode_batch_*`` take torch
+tensors that already live in HBM.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SyndromeError, check, lib
+
+BG1, BG2 = 0, 1
+DEFAULT_SCALE = 8.0          # quantize.rs:182
+MSG_MAX = 127                # quantize.rs:190
+APP_CLAMP_I8 = 127           # quantize.rs:194
+APP_CLAMP_WIDE = 32767       # quantize.rs:205
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class QuantParams:
+    """QuantParams (src/quantize.rs:217-305)."""
+
+    def __init__(self, scale=DEFAULT_SCALE, app_clamp=APP_CLAMP_WIDE):
+        self.scale = float(scale)
+        self.app_clamp = int(app_clamp)
+
+
+class QcLdpcDecoder:
+    """GPU layered offset min-sum decoder behind the reference's API."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    # -- constructors (qc_ldpc.rs:506-565, wifi_ldpc_tables.rs:521-529) --
+    @classmethod
+    def with_lifting_size(cls, bg, z, offset_beta):
+        out = C.c_void_p()
+        check(lib().syndrome_ldpc_decoder_create(bg, z, offset_beta, C.byref(out)), "decoder_create")
+        return cls(out.value)
+
+    @classmethod
+    def new(cls, bg, offset_beta):
+        return cls.with_lifting_size(bg, 384 if bg == BG1 else 128, offset_beta)
+
+    @classmethod
+    def from_raw_edges(cls, entries, num_row_blocks, num_col_blocks, z, offset_beta):
+        e = np.asarray(entries, dtype=np.int64).reshape(-1, 3)
+        if (e < 0).any() or (e[:, :2] > 255).any() or (e[:, 2] > 32767).any():
+            raise SyndromeError(_lib.INVALID_PARAM, "from_raw_edges")
+        rows = np.ascontiguousarray(e[:, 0], np.uint8)
+        cols = np.ascontiguousarray(e[:, 1], np.uint8)
+        sh = np.ascontiguousarray(e[:, 2], np.int16)
+        out = C.c_void_p()
+        check(lib().syndrome_ldpc_decoder_create_raw(_ptr(rows), _ptr(cols), _ptr(sh), len(e), num_row_blocks,
+                                                     num_col_blocks, z, offset_beta, C.byref(out)),
+              "decoder_create_raw")
+        return cls(out.value)
+
+    @classmethod
+    def wifi(cls, z, rate_num, rate_den, offset_beta):
+        out = C.c_void_p()
+        check(lib().syndrome_ldpc_decoder_create_wifi(z, rate_num, rate_den, offset_beta, C.byref(out)),
+              "decoder_create_wifi")
+        return cls(out.value)
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().syndrome_ldpc_decoder_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- size queries (qc_ldpc.rs:569-588, 736-739) --
+    def variable_node_count(self):
+        return lib().syndrome_ldpc_decoder_variable_node_count(self._h)
+
+    def check_node_count(self):
+        return lib().syndrome_ldpc_decoder_check_node_count(self._h)
+
+    def info_bit_count_5g(self):
+        return lib().syndrome_ldpc_decoder_info_bit_count(self._h)
+
+    def required_edge_buffer(self):
+        return lib().syndrome_ldpc_decoder_required_edge_buffer(self._h)
+
+    def required_layer_buffer(self):
+        return lib().syndrome_ldpc_decoder_required_layer_buffer(self._h)
+
+    @property
+    def offset_beta(self):
+        return lib().syndrome_ldpc_decoder_offset_beta(self._h)
+
+    # -- host-buffer decodes (in place, like the reference) --
+    def decode_5g(self, llr, n_filler, edge_r, layer_scratch, hard_output, iterations):
+        """QcLdpcDecoder::decode_5g (qc_ldpc.rs:644-655).  Returns iterations used."""
+        used = C.c_size_t(0)
+        check(lib().syndrome_ldpc_decode_5g(self._h, _ptr(llr), llr.size, n_filler, _ptr(edge_r), edge_r.size,
+                                            _ptr(layer_scratch), layer_scratch.size, _ptr(hard_output),
+                                            hard_output.size, iterations, C.byref(used)), "decode_5g")
+        return used.value
+
+    def decode_layered_offset_min_sum(self, llr, edge_r, layer_scratch, hard_output, iterations):
+        """QcLdpcDecoder::decode_layered_offset_min_sum (qc_ldpc.rs:794-810)."""
+        used = C.c_size_t(0)
+        check(lib().syndrome_ldpc_decode(self._h, _ptr(llr), llr.size, _ptr(edge_r), edge_r.size,
+                                         _ptr(layer_scratch), layer_scratch.size, _ptr(hard_output),
+                                         hard_output.size, iterations, C.byref(used)), "decode")
+        return used.value
+
+    def decode_layered_offset_min_sum_i8(self, app, edge_r, layer_scratch, hard_output, iterations, quant=None):
+        """QcLdpcDecoder::decode_layered_offset_min_sum_i8 (qc_ldpc.rs:1345-1363)."""
+        q = quant or QuantParams()
+        used = C.c_size_t(0)
+        check(lib().syndrome_ldpc_decode_i8(self._h, _ptr(app), app.size, _ptr(edge_r), edge_r.size,
+                                            _ptr(layer_scratch), layer_scratch.size, _ptr(hard_output),
+                                            hard_output.size, iterations, q.scale, q.app_clamp, C.byref(used)),
+              "decode_i8")
+        return used.value
+
+    def decode_5g_i8(self, app, n_filler, edge_r, layer_scratch, hard_output, iterations, quant=None):
+        """QcLdpcDecoder::decode_5g_i8 (qc_ldpc.rs:1486-1523)."""
+        q = quant or QuantParams()
+        used = C.c_size_t(0)
+        check(lib().syndrome_ldpc_decode_5g_i8(self._h, _ptr(app), app.size, n_filler, _ptr(edge_r), edge_r.size,
+                                               _ptr(layer_scratch), layer_scratch.size, _ptr(hard_output),
+                                               hard_output.size, iterations, q.scale, q.app_clamp, C.byref(used)),
+              "decode_5g_i8")
+        return used.value
+
+    # -- workspace convenience (qc_ldpc.rs:1817-1963) --
+    def workspace(self):
+        return LdpcWorkspace(self)
+
+    def decode_with_workspace(self, llr, ws, iterations):
+        return self.decode_layered_offset_min_sum(llr, ws.edge_r, ws.layer_scratch, ws.hard, iterations)
+
+    def decode_5g_with_workspace(self, llr, n_filler, ws, iterations):
+        return self.decode_5g(llr, n_filler, ws.edge_r, ws.layer_scratch, ws.hard, iterations)
+
+    # -- device batch (torch tensors in HBM) --
+    def launch_shape(self, int8=False, plain=False):
+        cws, thr, lds = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().syndrome_ldpc_launch_shape(self._h, int(int8), int(plain), C.byref(cws), C.byref(thr),
+                                               C.byref(lds)), "launch_shape")
+        return cws.value, thr.value, lds.value
+
+    def decode_batch_f32(self, d_in, d_post, d_hard, d_iters, iterations, n_filler=0, flags=None, d_edge=None,
+                         stream=None):
+        import torch
+        if flags is None:
+            flags = _lib.WRITE_POST | _lib.WRITE_HARD
+        batch = d_iters.numel()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().syndrome_ldpc_decode_batch_f32(
+            self._h, d_in.data_ptr(), d_post.data_ptr() if d_post is not None else None,
+            d_edge.data_ptr() if d_edge is not None else None, d_hard.data_ptr() if d_hard is not None else None,
+            d_iters.data_ptr(), batch, iterations, n_filler, flags, s), "decode_batch_f32")
+
+    def decode_batch_i8(self, d_in, d_post, d_hard, d_iters, iterations, n_filler=0, flags=None, d_edge=None,
+                        quant=None, stream=None):
+        import torch
+        q = quant or QuantParams()
+        if flags is None:
+            flags = _lib.WRITE_POST | _lib.WRITE_HARD
+        batch = d_iters.numel()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().syndrome_ldpc_decode_batch_i8(
+            self._h, d_in.data_ptr(), d_post.data_ptr() if d_post is not None else None,
+            d_edge.data_ptr() if d_edge is not None else None, d_hard.data_ptr() if d_hard is not None else None,
+            d_iters.data_ptr(), batch, iterations, n_filler, q.scale, q.app_clamp, flags, s), "decode_batch_i8")
+
+    # -- traffic generation --
+    def encode_batch(self, info, threads=0):
+        """Encode info [count, K] (u8) to codewords [count, N]."""
+        info = np.ascontiguousarray(info, np.uint8)
+        count = info.shape[0] if info.ndim == 2 else 1
+        out = np.zeros((count, self.variable_node_count()), np.uint8)
+        check(lib().syndrome_ldpc_encode_batch(self._h, _ptr(info), _ptr(out), count, threads), "encode_batch")
+        return out
+
+
+class LdpcWorkspace:
+    """LdpcWorkspace (src/qc_ldpc.rs:1919-1963)."""
+
+    def __init__(self, dec):
+        n = dec.variable_node_count()
+        self.codeword_llr = np.zeros(n, np.float32)
+        self.edge_r = np.zeros(dec.required_edge_buffer(), np.float32)
+        self.layer_scratch = np.zeros(dec.required_layer_buffer(), np.float32)
+        self.hard = np.zeros(n, np.uint8)
+
+    def hard_output(self):
+        return self.hard
+
+
+class QcLdpcEncoder:
+    """QcLdpcEncoder (src/qc_ldpc.rs:2408-2614) through the C ABI."""
+
+    def __init__(self, bg, z):
+        out = C.c_void_p()
+        check(lib().syndrome_ldpc_encoder_create(bg, z, C.byref(out)), "encoder_create")
+        self._h = out
+
+    def __del__(self):
+        try:
+            if self._h and self._h.value:
+                lib().syndrome_ldpc_encoder_destroy(self._h)
+        except Exception:
+            pass
+
+    def info_bit_count(self):
+        return lib().syndrome_ldpc_encoder_info_bit_count(self._h)
+
+    def codeword_bit_count(self):
+        return lib().syndrome_ldpc_encoder_codeword_bit_count(self._h)
+
+    def encode_5g(self, info_bits, n_filler, codeword):
+        check(lib().syndrome_ldpc_encode_5g(self._h, _ptr(info_bits), info_bits.size, n_filler, _ptr(codeword),
+                                            codeword.size), "encode_5g")
+
+
+def quantize_llr_i16_device(d_llr, d_out, scale=DEFAULT_SCALE, stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().syndrome_ldpc_quantize_i16_device(d_llr.data_ptr(), d_out.data_ptr(), d_llr.numel(), scale, s),
+          "quantize")
+
+
+def awgn_transmit(ebno_db, code_rate, seed, bits):
+    """AwgnChannel::new(ebno_db, code_rate, seed).transmit(bits) (channel_sim.rs:139-215)."""
+    bits = np.ascontiguousarray(bits, np.uint8)
+    out = np.zeros(bits.size, np.float32)
+    check(lib().syndrome_awgn_transmit(ebno_db, code_rate, seed, _ptr(bits), bits.size, _ptr(out)), "awgn")
+    return out

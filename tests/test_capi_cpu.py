@@ -1,0 +1,102 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol
+include/syndrome_ldpc.h declares, and the host-side parts (argument
+validation, encoder, channel model) behave as the reference's
+(capi/src/lib.rs:494-621, qc_ldpc.rs:2547-2614, channel_sim.rs)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.helpers import f32_bits
+
+
+@pytest.fixture(scope="module")
+def L():
+    from syndrome_amd import _lib
+    return _lib.lib()
+
+
+def test_exports_every_declared_symbol(L):
+    from syndrome_amd._lib import SIGNATURES, declared_symbols
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(L, s), s
+        assert s in SIGNATURES, f"{s} has no ctypes signature"
+
+
+def test_decoder_create_rejects_bad_params_before_touching_the_device(L):
+    from syndrome_amd import _lib
+    out = C.c_void_p(1234)
+    assert L.syndrome_ldpc_decoder_create(2, 384, 0.5, C.byref(out)) == _lib.INVALID_PARAM
+    assert out.value is None  # *out = NULL on failure (capi/src/lib.rs:160)
+    for z in (0, 1, 383, 385, 2 ** 40):
+        out = C.c_void_p(99)
+        assert L.syndrome_ldpc_decoder_create(0, z, 0.5, C.byref(out)) == _lib.INVALID_PARAM
+        assert out.value is None
+    assert L.syndrome_ldpc_decoder_create(0, 384, 0.5, None) == _lib.NULL_POINTER
+    assert L.syndrome_ldpc_decoder_create_wifi(27, 1, 3, 0.5, C.byref(out)) == _lib.INVALID_PARAM
+    # raw edges: shift >= z, row out of range, duplicate (row, col)
+    for rows, cols, sh in (([0], [0], [4]), ([2], [0], [0]), ([0, 0], [1, 1], [0, 1])):
+        r = np.array(rows, np.uint8)
+        c = np.array(cols, np.uint8)
+        s = np.array(sh, np.int16)
+        rc = L.syndrome_ldpc_decoder_create_raw(r.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p),
+                                                s.ctypes.data_as(C.c_void_p), len(r), 2, 3, 4, 0.5, C.byref(out))
+        assert rc == _lib.INVALID_PARAM
+
+
+def test_null_decoder_is_reported_not_dereferenced(L):
+    from syndrome_amd import _lib
+    buf = np.zeros(16, np.float32)
+    p = buf.ctypes.data_as(C.c_void_p)
+    assert L.syndrome_ldpc_decode_5g(None, p, 16, 0, p, 16, p, 16, p, 16, 5, None) == _lib.NULL_POINTER
+    assert L.syndrome_ldpc_decode(None, p, 16, p, 16, p, 16, p, 16, 5, None) == _lib.NULL_POINTER
+    L.syndrome_ldpc_decoder_destroy(None)  # no-op
+    L.syndrome_ldpc_encoder_destroy(None)
+
+
+def test_encoder_matches_oracle_and_reference_conventions(L, oracle):
+    from syndrome_amd import QcLdpcEncoder, SyndromeError, _lib
+    for bg, z in ((0, 384), (1, 208), (0, 2), (1, 3), (0, 240)):
+        enc = QcLdpcEncoder(bg, z)
+        c = oracle.code_3gpp(bg, z)
+        assert (enc.info_bit_count(), enc.codeword_bit_count()) == (c.k, c.n)
+        for nf in (0, 5):
+            info = oracle.xorshift_bits(z + nf, c.k - nf)
+            cw = np.zeros(c.n, np.uint8)
+            enc.encode_5g(info, nf, cw)
+            full = np.concatenate([info, np.zeros(nf, np.uint8)])
+            assert np.array_equal(cw, oracle.encode(c, full))
+    enc = QcLdpcEncoder(0, 4)
+    cw = np.zeros(68 * 4, np.uint8)
+    with pytest.raises(SyndromeError) as e:
+        enc.encode_5g(np.zeros(88 + 1, np.uint8), 0, cw)  # wrong info length
+    assert e.value.status == _lib.BUFFER_TOO_SMALL
+    with pytest.raises(SyndromeError) as e:
+        enc.encode_5g(np.zeros(0, np.uint8), 89, cw)  # n_filler > k
+    assert e.value.status == _lib.INVALID_PARAM
+    with pytest.raises(SyndromeError) as e:
+        enc.encode_5g(np.zeros(88, np.uint8), 0, np.zeros(10, np.uint8))
+    assert e.value.status == _lib.BUFFER_TOO_SMALL
+
+
+def test_awgn_matches_oracle(oracle):
+    from syndrome_amd import awgn_transmit
+    bits = oracle.xorshift_bits(3, 5000)
+    for ebno, rate, seed in ((2.0, 0.5, 1), (-1.0, 1 / 3, 7), (4.5, 8448 / 26112, 42)):
+        assert np.array_equal(f32_bits(awgn_transmit(ebno, rate, seed, bits)), f32_bits(oracle.awgn(ebno, rate, seed, bits)))
+
+
+def test_product_has_no_oracle_dependency():
+    """The product package never imports, links or runs the oracle."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for d, _, files in os.walk(os.path.join(root, "syndrome_amd")):
+        if "_build" in d:
+            continue
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".h", ".inc")):
+                text = open(os.path.join(d, f)).read()
+                assert not re.search(r"\boracle\b|liborc|loms_oracle", text), f
