@@ -232,3 +232,43 @@ def ref_decode_bg1_z384_10it(json_text, llr):
     rc = L.ref_loms_bg1_z384_decode10(json_text.encode(), _p(llr), _p(er))
     assert rc == 0
     return llr, er
+
+
+# -- CPU baseline: AVX2 restatement + worker pipeline (libcpubase.so) ---------
+
+CPB_PATH = os.path.join(HERE, "libcpubase.so")
+_cpb = None
+
+
+def cpb():
+    global _cpb
+    if _cpb is None:
+        if not os.path.exists(CPB_PATH):
+            build()
+        L = C.CDLL(CPB_PATH)
+        L.cpb_decode_f32_avx2.argtypes = [C.POINTER(Code), C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
+        L.cpb_decode_f32_avx2.restype = C.c_long
+        L.cpb_pipeline_run.argtypes = [C.POINTER(Code), C.c_float, C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_int,
+                                       C.c_long, C.c_void_p, C.c_void_p]
+        L.cpb_pipeline_run.restype = C.c_longlong
+        _cpb = L
+    return _cpb
+
+
+def avx2_decode_f32(code, beta, llr, iterations):
+    llr = np.array(llr, dtype=np.float32, copy=True)
+    er = np.zeros(code.edge_buf, np.float32)
+    hard = np.zeros(code.n, np.uint8)
+    it = cpb().cpb_decode_f32_avx2(C.byref(code), beta, _p(llr), _p(er), _p(hard), iterations)
+    return it, llr, er, hard
+
+
+def pipeline_run(code, beta, llr_batch, iterations, workers, init5g=True, n_filler=0):
+    """Returns (seconds, iters[count], hard[count, n])."""
+    llr_batch = np.ascontiguousarray(llr_batch, np.float32)
+    count = llr_batch.shape[0]
+    iters = np.zeros(count, np.int32)
+    hard = np.zeros((count, code.n), np.uint8)
+    ns = cpb().cpb_pipeline_run(C.byref(code), beta, _p(llr_batch), count, iterations, workers, int(init5g),
+                                n_filler, _p(iters), _p(hard))
+    return ns * 1e-9, iters, hard
