@@ -103,7 +103,76 @@ def cpu_baseline(llr, iters, beta, budget_s=10.0):
                       f"{96 * code.k / t1 / 1e9:.6f} Gbit/s; mean iterations {float(its.mean()):.3f}"}
 
 
+def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_substr, cmd_trace, cmd_pmc):
+    """Turn rocprofv3 CSV output (kernel trace + two --pmc runs) into the
+    committed text summary and the traffic JSON the roofline reads.
+
+    HBM bytes per launch = FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of
+    the decode kernel, each scaled by what the same counter reads for the
+    device copy of the input batch (known bytes) in the same run."""
+    import csv
+    from collections import defaultdict
+
+    def rows(d, name):
+        return list(csv.DictReader(open(os.path.join(d, name))))
+
+    tr = rows(trace_dir, "run_kernel_trace.csv")
+    by_grid = defaultdict(list)
+    for r in tr:
+        if kernel_substr in r["Kernel_Name"]:
+            by_grid[int(r["Grid_Size_X"]) if "Grid_Size_X" in r else int(r.get("Grid_Size", 0))].append(
+                (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+    stats = rows(trace_dir, "run_kernel_stats.csv")
+    counters = {}
+    for nm, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        ker, cpy = [], []
+        for r in rows(d, "run_counter_collection.csv"):
+            v = float(r["Counter_Value"])
+            if kernel_substr in r["Kernel_Name"] and int(r["Grid_Size"]) == max(by_grid):
+                ker.append(v)
+            elif "copyBuffer" in r["Kernel_Name"] and v > 1000:
+                cpy.append(v)
+        counters[nm] = (float(np.median(ker)), float(np.median(cpy)))
+    calib = json.loads(open(os.path.join(fetch_dir, "..", "bench_copycal.json")).read())["copy_calibration_bytes"] \
+        if os.path.exists(os.path.join(fetch_dir, "..", "bench_copycal.json")) else None
+    lines = [f"# rocprofv3 summary: {kernel_substr}", "", "Kernel trace command:", "", f"    {cmd_trace}", "",
+             "Counter commands (one counter per run):", "", f"    {cmd_pmc}", "", "## kernel trace (stats CSV, all dispatches)", "",
+             "| kernel | calls | average ns | min ns | max ns |", "|---|---|---|---|---|"]
+    for r in stats[:6]:
+        lines.append(f"| {r['Name'][:70]} | {r['Calls']} | {float(r['AverageNs']):.0f} | {r['MinNs']} | {r['MaxNs']} |")
+    lines += ["", "## decode kernel by grid size (trace CSV)", "", "| grid (work-items) | launches | average ms |",
+              "|---|---|---|"]
+    for g, v in sorted(by_grid.items()):
+        lines.append(f"| {g} | {len(v)} | {np.mean(v):.4f} |")
+    out = {"kernel": kernel_substr, "grid_work_items": max(by_grid),
+           "average_ms_full_batch": float(np.mean(by_grid[max(by_grid)]))}
+    if calib:
+        fk, fc = counters["FETCH_SIZE"]
+        wk, wc = counters["WRITE_SIZE"]
+        f_scale = calib / (fc * 1024.0)
+        w_scale = calib / (wc * 1024.0)
+        fetch_b = fk * 1024.0 * f_scale
+        write_b = wk * 1024.0 * w_scale
+        out.update({"fetch_kib_raw": fk, "write_kib_raw": wk, "copy_fetch_kib_raw": fc, "copy_write_kib_raw": wc,
+                    "copy_bytes": calib, "fetch_scale": f_scale, "write_scale": w_scale,
+                    "hbm_fetch_bytes_per_launch": fetch_b, "hbm_write_bytes_per_launch": write_b,
+                    "hbm_bytes_per_launch": fetch_b + write_b})
+        lines += ["", "## HBM traffic per full-batch launch (--pmc)", "",
+                  f"FETCH_SIZE {fk:.0f} KiB raw; the input copy ({calib} B) reads {fc:.0f} KiB raw -> scale {f_scale:.4f}",
+                  f"WRITE_SIZE {wk:.0f} KiB raw; the input copy writes {wc:.0f} KiB raw -> scale {w_scale:.4f}",
+                  f"=> {fetch_b / 1e6:.1f} MB fetched + {write_b / 1e6:.1f} MB written = "
+                  f"{(fetch_b + write_b) / 1e6:.1f} MB per launch"]
+    os.makedirs(os.path.dirname(out_prefix), exist_ok=True)
+    open(out_prefix + ".md", "w").write("\n".join(lines) + "\n")
+    open(out_prefix + "_traffic.json", "w").write(json.dumps(out, indent=1) + "\n")
+    print("\n".join(lines))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-profiles":
+        # bench.py --summarize-profiles TRACE_DIR FETCH_DIR WRITE_DIR OUT_PREFIX KERNEL CMD_TRACE CMD_PMC
+        summarize_profiles(*sys.argv[2:9])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -181,6 +250,9 @@ def main():
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = int(d_in.numel() * 4)
         del d_copy
+        cal = os.environ.get("SYNDROME_COPYCAL_OUT")
+        if cal:
+            open(cal, "w").write(json.dumps({"copy_calibration_bytes": out["copy_calibration_bytes"]}))
     if not args.no_plain:
         log("[bench] timing the plain per-config decoder")
         d_edge = torch.empty((B, ez), dtype=torch.float32, device="cuda")
