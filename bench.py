@@ -185,6 +185,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--latency-samples", type=int, default=200,
+                    help="single-codeword launches timed for latency (0 = skip)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run device copies of the input batch (known bytes) to scale HBM counters")
     args = ap.parse_args()
@@ -273,7 +275,7 @@ def main():
 
     # single code block latency (one codeword per launch, inputs resident)
     one = lambda: dev.decode_batch_f32(d_in[:1], d_post[:1], d_hard[:1], d_it[:1], args.iters, flags=flags)
-    _, lat1 = time_launches(one, 200, 10, torch)
+    lat1 = time_launches(one, args.latency_samples, 10, torch)[1] if args.latency_samples > 0 else np.zeros(1)
 
     cpu = None
     if not args.no_cpu:

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsyndrome_amd.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["ldpc_kernels.hip", "ldpc_capi.hip", "code_host.cpp"]
+SOURCES = ["ldpc_fast_f32_bg1.hip", "ldpc_fast_i8_bg1.hip", "ldpc_fast_f32_bg2.hip", "ldpc_fast_i8_bg2.hip",
+           "ldpc_plain.hip", "ldpc_capi.hip", "code_host.cpp"]
 ARCH = os.environ.get("SYNDROME_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [

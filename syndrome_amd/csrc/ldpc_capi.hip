@@ -21,8 +21,8 @@
 #include "ldpc_common.h"
 
 namespace syn {
-int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, const SynLaunch &a, int block, size_t lds,
-                  hipStream_t stream);
+int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, const SynLaunch &a, int block,
+                  size_t lds, hipStream_t stream);
 int launch_quantize(const float *in, int16_t *out, size_t n, float scale, hipStream_t stream);
 }
 
@@ -106,7 +106,9 @@ int32_t launch_shape(const SyndromeLdpcDecoder *d, bool i8, bool plain, Shape *s
     const SynCode &c = d->code;
     const size_t n = syn::code_n(c), elem = i8 ? 2 : 4;
     const int max_lanes = use_fast(c, plain) ? 512 : 1024;
-    auto bytes = [&](int k) { return ((k * n * elem + 3) & ~(size_t)3) + 3 * sizeof(int) * (size_t)k; };
+    // fast kernels keep a Z*size pad in front of the posterior (ldpc_fast.h)
+    const size_t pad = use_fast(c, plain) ? (size_t)(((c.z * (int)elem) + 15) & ~15) : 0;
+    auto bytes = [&](int k) { return pad + ((k * n * elem + 3) & ~(size_t)3) + 3 * sizeof(int) * (size_t)k; };
     int k = std::max(1, max_lanes / c.z);
     while (k > 1 && bytes(k) > (size_t)d->lds_limit) --k;
     if (bytes(k) > (size_t)d->lds_limit) return SYNDROME_INVALID_PARAM;
@@ -151,7 +153,7 @@ int32_t run_batch(const SyndromeLdpcDecoder *d, bool i8, const void *in, void *p
         a.beta_q = !(b - b == 0.f) ? 127 : (int)std::min(127.f, std::max(0.f, roundf(b)));
     }
     DeviceScope ds(d->device);
-    return syn::launch_decode(d->code.kind, i8, fast, d->d_code, a, s.block, s.lds, stream);
+    return syn::launch_decode(d->code.kind, i8, fast, d->d_code, d->code.z, a, s.block, s.lds, stream);
 }
 
 int32_t ensure_stage(SyndromeLdpcDecoder *d, size_t bytes)

@@ -1,0 +1,469 @@
+// Register-resident LOMS kernels: compile-time base graph (BG1 / BG2) and,
+// optionally, compile-time lifting size.
+//
+// One workgroup decodes `cws` codewords of one (BG, Z).  Lane t owns
+// lifting index z = t % Z of codeword t / Z and processes check row
+// (layer, z) of every layer, in the reference's strict layer order
+// (src/qc_ldpc.rs:1055-1206).  The posterior lives in LDS; the row's
+// check-to-variable messages never leave the lane's registers, kept in the
+// compressed min-sum form (|R| of the argmin edge, |R| of every other
+// edge, the argmin and one sign bit per edge).
+//
+// ZC > 0 instantiates the kernel for one lifting size: every shift becomes
+// an immediate, so a posterior address is one compare, one select between
+// two per-lane bases, and the column offset folded into the LDS
+// instruction.  ZC == 0 reads the shifts of the decoder's Z at run time.
+#pragma once
+#include "ldpc_device.h"
+
+namespace syn {
+
+template <int KIND>
+struct Graph;
+
+template <>
+struct Graph<SYN_KIND_BG1> {
+    static constexpr int ROWS = SYN_BG1_ROWS, COLS = SYN_BG1_COLS, EDGES = SYN_BG1_EDGES;
+    static constexpr int deg(int l) { return SYN_BG1_ROW_DEG[l]; }
+    static constexpr int col(int e) { return SYN_BG1_EDGE_COL[e]; }
+    static constexpr int v(int e, int ils) { return SYN_BG1_EDGE_V[e][ils]; }
+};
+
+template <>
+struct Graph<SYN_KIND_BG2> {
+    static constexpr int ROWS = SYN_BG2_ROWS, COLS = SYN_BG2_COLS, EDGES = SYN_BG2_EDGES;
+    static constexpr int deg(int l) { return SYN_BG2_ROW_DEG[l]; }
+    static constexpr int col(int e) { return SYN_BG2_EDGE_COL[e]; }
+    static constexpr int v(int e, int ils) { return SYN_BG2_EDGE_V[e][ils]; }
+};
+
+// 3GPP TS 38.212 Table 5.3.2-1 (src/qc_ldpc.rs:103-125).
+constexpr int lifting_set(int z)
+{
+    int a = z;
+    while (a % 2 == 0) a /= 2;
+    switch (a) {
+    case 1: return 0;
+    case 3: return 1;
+    case 5: return 2;
+    case 7: return 3;
+    case 9: return 4;
+    case 11: return 5;
+    case 13: return 6;
+    case 15: return 7;
+    default: return -1;
+    }
+}
+
+template <class G>
+constexpr int layer_begin(int l)
+{
+    int s = 0;
+    for (int i = 0; i < l; ++i) s += G::deg(i);
+    return s;
+}
+
+// A barrier is needed after layer l unless layer l+1 touches none of the
+// columns touched since the last barrier: layers over disjoint variables
+// commute exactly, so running them without a barrier between them changes
+// nothing.  Greedy grouping from layer 0.
+template <class G>
+constexpr bool barrier_after(int l)
+{
+    if (l + 1 >= G::ROWS) return true;
+    bool used[SYN_MAX_COLS] = {};
+    for (int i = 0; i <= l + 1; ++i) {
+        const int b = layer_begin<G>(i);
+        bool clash = false;
+        for (int e = 0; e < G::deg(i); ++e) clash = clash || used[G::col(b + e)];
+        if (i == l + 1) return clash;
+        if (clash)
+            for (int c = 0; c < SYN_MAX_COLS; ++c) used[c] = false;
+        for (int e = 0; e < G::deg(i); ++e) used[G::col(b + e)] = true;
+    }
+    return true;
+}
+
+// Per-layer sign word: bits 0..deg-1 hold the sign of R on each edge, edge e
+// at bit deg-1-e (the order a shift-in accumulation produces), bits
+// deg..deg+4 the argmin edge (31 = none).  Low-degree layers share a
+// register: words are packed greedily, in layer order, into 32-bit words.
+template <class G>
+constexpr int sg_bits(int l)
+{
+    return G::deg(l) + 5;
+}
+
+template <class G>
+constexpr int sg_pos(int l, bool want_word)
+{
+    int w = 0, off = 0;
+    for (int i = 0; i <= l; ++i) {
+        if (off + sg_bits<G>(i) > 32) {
+            ++w;
+            off = 0;
+        }
+        if (i == l) return want_word ? w : off;
+        off += sg_bits<G>(i);
+    }
+    return 0;
+}
+
+template <class G>
+constexpr int sg_words()
+{
+    return sg_pos<G>(G::ROWS - 1, true) + 1;
+}
+
+template <class G>
+struct StateF32 {
+    float g1[G::ROWS];           // |R| on every edge but the argmin: max(min1 - beta, 0)
+    float g2[G::ROWS];           // |R| on the argmin edge:          max(min2 - beta, 0)
+    uint32_t sg[sg_words<G>()];  // packed sign words
+};
+
+template <class G>
+struct StateI8 {
+    uint32_t mg[(G::ROWS + 1) / 2];  // per layer 16 bits: |R| others (bits 0..7), argmin (8..15)
+    uint32_t sg[sg_words<G>()];
+};
+
+template <class G, int L>
+__device__ __forceinline__ uint32_t sg_get(const uint32_t *sg)
+{
+    constexpr int W = sg_pos<G>(L, true), O = sg_pos<G>(L, false);
+    return sg[W] >> O;
+}
+
+template <class G, int L>
+__device__ __forceinline__ void sg_put(uint32_t *sg, uint32_t v)
+{
+    constexpr int W = sg_pos<G>(L, true), O = sg_pos<G>(L, false);
+    constexpr uint32_t M = ((1u << sg_bits<G>(L)) - 1u) << O;
+    sg[W] = (sg[W] & ~M) | ((v << O) & M);
+}
+
+template <class G, int L>
+__device__ __forceinline__ uint32_t mg_get(const StateI8<G> &st)
+{
+    return (st.mg[L / 2] >> (16 * (L % 2))) & 0xffffu;
+}
+
+template <class G, int L>
+__device__ __forceinline__ void mg_put(StateI8<G> &st, uint32_t v)
+{
+    constexpr uint32_t M = 0xffffu << (16 * (L % 2));
+    st.mg[L / 2] = (st.mg[L / 2] & ~M) | (v << (16 * (L % 2)));
+}
+
+// Bytes of LDS reserved in front of the posterior so that "base - Z*size"
+// stays non-negative (lets the column offset ride in the LDS instruction).
+__host__ __device__ constexpr int fast_pad(int z, int esz) { return ((z * esz) + 15) & ~15; }
+
+// Per-lane addressing for one iteration: pb = LDS address of element
+// (codeword, column 0, z); pbw = pb - Z*size, used where (z + shift) wraps;
+// *h = the same plus 64 KiB, for column offsets beyond the 16-bit LDS
+// instruction offset.
+struct Lane {
+    unsigned char *pb, *pbw, *pbh, *pbwh;
+    int z;
+};
+
+template <class G, int ZC>
+struct Geo {
+    int Z;                         // run-time lifting size (ZC == 0)
+    const int *__restrict__ sh;    // run-time shifts (ZC == 0)
+
+    __device__ __forceinline__ int z_size() const
+    {
+        if constexpr (ZC > 0) return ZC;
+        else return Z;
+    }
+
+    // LDS address of variable col(E)*Z + (z + shift(E)) mod Z.
+    template <int E, int ESZ>
+    __device__ __forceinline__ unsigned char *addr(const Lane &ln) const
+    {
+        constexpr int col = G::col(E);
+        if constexpr (ZC > 0) {
+            constexpr int s = G::v(E, lifting_set(ZC)) % ZC;
+            constexpr int T = ZC - s;  // lanes z >= T wrap
+            constexpr int A = (col * ZC + s) * ESZ;
+            if constexpr (s == 0) {
+                return ((A >= 65536) ? ln.pbh : ln.pb) + (A & 0xffff);
+            } else if constexpr (A >= 65536) {
+                return ((ln.z >= T) ? ln.pbwh : ln.pbh) + (A & 0xffff);
+            } else {
+                return ((ln.z >= T) ? ln.pbw : ln.pb) + A;
+            }
+        } else {
+            const int s = sh[E];
+            return ((ln.z >= Z - s) ? ln.pbw : ln.pb) + (col * Z + s) * ESZ;
+        }
+    }
+};
+
+// R of edge e rebuilt from a layer's compressed state: s = sign word, am =
+// argmin edge.  The reference picks min2 for an edge whose |q| equals min1
+// and min1 otherwise; an edge other than the tracked argmin can equal min1
+// only when min1 is tied, and a tie forces min2 == min1 in pass 1, so "is
+// the argmin" selects the same value.
+template <int D>
+__device__ __forceinline__ float r_f32(float g1, float g2, uint32_t s, uint32_t am, int e)
+{
+    const float m = (am == (uint32_t)e) ? g2 : g1;
+    return __uint_as_float(__float_as_uint(m) | ((s << (31 - (D - 1 - e))) & 0x80000000u));
+}
+
+template <int D>
+__device__ __forceinline__ int r_i8(int g1, int g2, uint32_t s, uint32_t am, int e)
+{
+    const int m = (am == (uint32_t)e) ? g2 : g1;
+    const int neg = -(int)((s >> (D - 1 - e)) & 1u);
+    return (m ^ neg) - neg;
+}
+
+// One layer of the f32 decode (src/qc_ldpc.rs:1072-1204).
+template <class G, int ZC, int L>
+__device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo, StateF32<G> &st, float beta,
+                                          bool act)
+{
+    constexpr int D = G::deg(L), B = layer_begin<G>(L);
+    static_assert(D >= 2 && D <= SYN_MAX_DEG, "row degree");
+    float *adr[D];
+    float Lv[D], ro[D];
+    SFor<0, D>::run([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        adr[e] = (float *)geo.template addr<B + e, 4>(ln);
+        Lv[e] = *adr[e];
+    });
+    const float o1 = st.g1[L], o2 = st.g2[L];
+    const uint32_t os = sg_get<G, L>(st.sg);
+    const uint32_t oam = (os >> D) & 31u;
+    // Pass 1 on the bit patterns of |q|: for non-negative floats unsigned
+    // order is float order and a NaN sorts above +inf, so the integer
+    // min / clamp below keep exactly the reference's compare-based
+    // min1/min2 (a NaN is ignored there, and here it can never be selected).
+    uint32_t m1 = 0x7f7fffffu, m2 = 0x7f7fffffu;  // FLT_MAX
+    uint32_t qs = 0, am = 31;
+    SFor<0, D>::run([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        ro[e] = r_f32<D>(o1, o2, os, oam, e);
+        const uint32_t qb = __float_as_uint(Lv[e] - ro[e]);  // Q-build
+        qs = __builtin_amdgcn_alignbit(qs, qb, 31);           // shift in q's sign
+        const uint32_t a = qb & 0x7fffffffu;
+        am = (a <= m1) ? (uint32_t)e : am;
+        m2 = clampu(a, m1, m2);  // min(m2, max(m1, a)) as m1 <= m2
+        m1 = a < m1 ? a : m1;
+    });
+    float f1 = __uint_as_float(m1), f2 = __uint_as_float(m2);
+    if (m1 == 0x7f7fffffu) f1 = 0.f;
+    if (m2 == 0x7f7fffffu) f2 = f1;
+    float g1 = f1 - beta;
+    g1 = g1 < 0.f ? 0.f : g1;
+    float g2 = f2 - beta;
+    g2 = g2 < 0.f ? 0.f : g2;
+    // sign of R_e = XOR of all q signs but e's own
+    const uint32_t ns = qs ^ ((__popc(qs) & 1u) ? ((1u << D) - 1u) : 0u);
+    if (act) {
+        SFor<0, D>::run([&](auto ec) {
+            constexpr int e = decltype(ec)::value;
+            const float rn = r_f32<D>(g1, g2, ns, am, e);
+            *adr[e] = Lv[e] + (rn - ro[e]);  // pass 2: posterior += R_new - R_old
+        });
+        st.g1[L] = g1;
+        st.g2[L] = g2;
+        sg_put<G, L>(st.sg, ns | (am << D));
+    }
+}
+
+// One layer of the int8 decode (src/qc_ldpc.rs:1619-1745).
+template <class G, int ZC, int L>
+__device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC> &geo, StateI8<G> &st, int bq, int C,
+                                         bool act)
+{
+    constexpr int D = G::deg(L), B = layer_begin<G>(L);
+    static_assert(D >= 2 && D <= SYN_MAX_DEG, "row degree");
+    int16_t *adr[D];
+    int Av[D], ro[D];
+    SFor<0, D>::run([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        adr[e] = (int16_t *)geo.template addr<B + e, 2>(ln);
+        Av[e] = *adr[e];
+    });
+    const uint32_t om = mg_get<G, L>(st), os = sg_get<G, L>(st.sg);
+    const int o1 = (int)(om & 0xffu), o2 = (int)((om >> 8) & 0xffu);
+    const uint32_t oam = (os >> D) & 31u;
+    int m1 = 127, m2 = 127;
+    uint32_t qs = 0, am = 31;
+    SFor<0, D>::run([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        ro[e] = r_i8<D>(o1, o2, os, oam, e);
+        const int q = clampi(Av[e] - ro[e], -127, 127);  // Q-build, clamped to the message range
+        qs = __builtin_amdgcn_alignbit(qs, (uint32_t)q, 31);
+        const int a = q < 0 ? -q : q;
+        am = (a <= m1) ? (uint32_t)e : am;  // pass 1: sorted insertion
+        m2 = clampi(a, m1, m2);
+        m1 = a < m1 ? a : m1;
+    });
+    int g1 = m1 - bq;
+    g1 = g1 < 0 ? 0 : g1;
+    int g2 = m2 - bq;
+    g2 = g2 < 0 ? 0 : g2;
+    const uint32_t ns = qs ^ ((__popc(qs) & 1u) ? ((1u << D) - 1u) : 0u);
+    if (act) {
+        SFor<0, D>::run([&](auto ec) {
+            constexpr int e = decltype(ec)::value;
+            const int nv = Av[e] + (r_i8<D>(g1, g2, ns, am, e) - ro[e]);  // saturating add + clamp == clamp
+            *adr[e] = (int16_t)clampi(nv, -C, C);
+        });
+        mg_put<G, L>(st, (uint32_t)g1 | ((uint32_t)g2 << 8));
+        sg_put<G, L>(st.sg, ns | (am << D));
+    }
+}
+
+template <class G, int ZC, int L, class T>
+__device__ __forceinline__ bool row_parity(const Lane &ln, const Geo<G, ZC> &geo)
+{
+    constexpr int D = G::deg(L), B = layer_begin<G>(L);
+    bool p = false;
+    SFor<0, D>::run([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        p ^= *(const T *)geo.template addr<B + e, sizeof(T)>(ln) < (T)0;
+    });
+    return p;
+}
+
+template <class G, int ZC, int L0, int L1, class T>
+__device__ __forceinline__ bool rows_parity(const Lane &ln, const Geo<G, ZC> &geo)
+{
+    bool p = false;
+    SFor<L0, L1>::run([&](auto lc) { p = p || row_parity<G, ZC, decltype(lc)::value, T>(ln, geo); });
+    return p;
+}
+
+__device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
+
+// Syndrome of the lane's check rows (check_syndrome_f32 / _i16,
+// src/qc_ldpc.rs:1231-1252, 1766-1787) in layer segments.  A lane drops out
+// once it or another lane of its codeword has found a failed check; a wave
+// skips the remaining segments once none of its lanes is still checking
+// (wave-uniform branches, so no exec-mask bookkeeping).  Returns whether
+// this lane found a failed check.
+template <class G, int ZC, class T>
+__device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC> &geo, bool going, volatile int *cw_fail)
+{
+    constexpr int S0 = 1, S1 = 4, S2 = 10, S3 = 20, S4 = 32, S5 = G::ROWS;
+    bool fail = false;
+    auto segment = [&](auto lo, auto hi) {
+        constexpr int A = decltype(lo)::value, B = decltype(hi)::value;
+        if (!wave_any(going)) return;
+        const bool f = going && rows_parity<G, ZC, A, B, T>(ln, geo);
+        if (f) *cw_fail = 1;
+        fail = fail || f;
+        going = going && !f && !*cw_fail;
+    };
+    segment(std::integral_constant<int, 0>{}, std::integral_constant<int, S0>{});
+    segment(std::integral_constant<int, S0>{}, std::integral_constant<int, S1>{});
+    segment(std::integral_constant<int, S1>{}, std::integral_constant<int, S2>{});
+    segment(std::integral_constant<int, S2>{}, std::integral_constant<int, S3>{});
+    segment(std::integral_constant<int, S3>{}, std::integral_constant<int, S4>{});
+    segment(std::integral_constant<int, S4>{}, std::integral_constant<int, S5>{});
+    return fail;
+}
+
+// Shared kernel body.  I8 selects the int8 path.
+template <int KIND, int ZC, bool I8>
+__device__ __forceinline__ void fast_body(const SynCode *__restrict__ code, const SynLaunch &a)
+{
+    using G = Graph<KIND>;
+    using State = std::conditional_t<I8, StateI8<G>, StateF32<G>>;
+    constexpr int ESZ = I8 ? 2 : 4;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int Z = (ZC > 0) ? ZC : code->z;
+    const int N = G::COLS * Z, cws = a.cws, cw0 = blockIdx.x * cws;
+    const int EZ = G::EDGES * Z;
+    const int pad = fast_pad(Z, ESZ);
+    unsigned char *P0 = smem + pad;
+    CwFlags fl = cw_flags(P0 + ((cws * N * ESZ + 3) & ~3), cws);
+    const int t = threadIdx.x;
+    const int *__restrict__ shifts = code->shifts;
+    load_input<I8>(a, Z, G::COLS, G::ROWS, N, cw0, P0);
+    flags_init(fl, cws, cw0, a.batch);
+    __syncthreads();
+
+    const bool lane_ok = t < cws * Z;
+    const int cl = lane_ok ? t / Z : 0;
+    const int z = lane_ok ? t - cl * Z : 0;
+    const int zb0 = pad + (cl * N + z) * ESZ;
+
+    State st;  // R = 0 everywhere (qc_ldpc.rs:1006-1009 / 1579-1581)
+    if constexpr (I8) {
+        SFor<0, (G::ROWS + 1) / 2>::run([&](auto wc) { st.mg[decltype(wc)::value] = 0u; });
+    } else {
+        SFor<0, G::ROWS>::run([&](auto lc) {
+            st.g1[decltype(lc)::value] = 0.f;
+            st.g2[decltype(lc)::value] = 0.f;
+        });
+    }
+    SFor<0, sg_words<G>()>::run([&](auto wc) { st.sg[decltype(wc)::value] = 0u; });
+    SFor<0, G::ROWS>::run([&](auto lc) { sg_put<G, decltype(lc)::value>(st.sg, 31u << G::deg(decltype(lc)::value)); });
+
+    const float beta = a.beta;
+    const int bq = a.beta_q, C = a.app_clamp;
+    for (int it = 0; it < a.iterations; ++it) {
+        if (all_done(fl, cws)) break;
+        const bool act = lane_ok && !fl.done[cl];
+        if (t < cws && !fl.done[t]) fl.used[t] = it + 1;
+        // Opaque per-iteration copies keep the 316 address computations
+        // from being hoisted out of the loop into registers; the masks
+        // re-establish that the bases are small and non-negative.
+        unsigned char *pb = smem + (opaque_v(zb0) & 0x3ffff);
+        const Lane ln{pb, pb - Z * ESZ, pb + 65536, pb - Z * ESZ + 65536, opaque_v(z) & 0x3ff};
+        const Geo<G, ZC> geo{Z, shifts + opaque_s(0)};
+        SFor<0, G::ROWS>::run([&](auto lc) {
+            constexpr int L = decltype(lc)::value;
+            if constexpr (I8) layer_i8<G, ZC, L>(ln, geo, st, bq, C, act);
+            else layer_f32<G, ZC, L>(ln, geo, st, beta, act);
+            if constexpr (barrier_after<G>(L)) __syncthreads();
+        });
+        check_rows<G, ZC, std::conditional_t<I8, int16_t, float>>(ln, geo, act, fl.fail + cl);
+        __syncthreads();
+        flags_settle(fl, cws);
+        __syncthreads();
+    }
+    store_output<I8>(a, N, cw0, P0);
+    if (t < cws && cw0 + t < a.batch) a.iters[cw0 + t] = fl.used[t];
+    if ((a.flags & SYN_F_WRITE_EDGE) && lane_ok && cw0 + cl < a.batch) {
+        SFor<0, G::ROWS>::run([&](auto lc) {
+            constexpr int L = decltype(lc)::value;
+            constexpr int D = G::deg(L), B = layer_begin<G>(L);
+            const uint32_t sw = sg_get<G, L>(st.sg);
+            SFor<0, D>::run([&](auto ec) {
+                constexpr int e = decltype(ec)::value;
+                const size_t o = (size_t)(cw0 + cl) * EZ + (size_t)(B + e) * Z + z;
+                if constexpr (I8) {
+                    const uint32_t mw = mg_get<G, L>(st);
+                    ((int8_t *)a.edge)[o] =
+                        (int8_t)r_i8<D>((int)(mw & 0xffu), (int)((mw >> 8) & 0xffu), sw, (sw >> D) & 31u, e);
+                } else {
+                    ((float *)a.edge)[o] = r_f32<D>(st.g1[L], st.g2[L], sw, (sw >> D) & 31u, e);
+                }
+            });
+        });
+    }
+}
+
+template <int KIND, int ZC>
+__global__ void __launch_bounds__(512) fast_f32(const SynCode *__restrict__ code, SynLaunch a)
+{
+    fast_body<KIND, ZC, false>(code, a);
+}
+
+template <int KIND, int ZC>
+__global__ void __launch_bounds__(512) fast_i8(const SynCode *__restrict__ code, SynLaunch a)
+{
+    fast_body<KIND, ZC, true>(code, a);
+}
+
+}  // namespace syn
