@@ -163,6 +163,48 @@ int32_t syndrome_awgn_transmit(float ebno_db, float code_rate, uint64_t seed, co
 int32_t syndrome_ldpc_encode_batch(const SyndromeLdpcDecoder *decoder, const uint8_t *info, uint8_t *codewords,
                                    size_t count, int32_t threads);
 
+/* ======================================================================
+ * Part 4: work-queue batches of mixed codes (one persistent launch)
+ * ====================================================================== */
+
+typedef struct SyndromeLdpcPlan SyndromeLdpcPlan;
+
+/* A plan for `count` code blocks: block i is 3GPP base graph bg[i] (0 = BG1,
+ * 1 = BG2) at lifting size z[i] with n_filler[i] filler bits (NULL = all 0);
+ * its N_i = (bg ? 52 : 68) * z[i] decoder-input elements start at element
+ * offsets[i] of the caller's arenas (input, posterior and hard decisions use
+ * the same offsets; iterations are indexed by i).  Blocks are grouped by
+ * code into work items of as many codewords as fit a workgroup of `block`
+ * lanes (0 = default) and handed out longest first by a device queue.
+ * Decode semantics per block: decode_5g (SYNDROME_BATCH_INIT_5G) or the
+ * plain decode, exactly as if it were decoded alone.  No reference
+ * counterpart: the reference decodes one code block per call. */
+int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int32_t *n_filler,
+                                  const int64_t *offsets, size_t count, float offset_beta, int32_t int8,
+                                  int32_t block, SyndromeLdpcPlan **out_plan);
+void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *plan);
+/* One persistent launch over the plan.  int8 plans take i16 (or f32 with
+ * SYNDROME_BATCH_QUANTIZE) inputs and write i16 posteriors. */
+int32_t syndrome_ldpc_plan_run(SyndromeLdpcPlan *plan, const void *d_in, void *d_post, uint8_t *d_hard,
+                               int32_t *d_iters, size_t iterations, float scale, int16_t app_clamp, uint32_t flags,
+                               void *stream);
+int32_t syndrome_ldpc_plan_shape(const SyndromeLdpcPlan *plan, int32_t *out_block, int32_t *out_grid,
+                                 int32_t *out_items, int32_t *out_lds_bytes);
+/* Launch-shape tuner: times the plan at each workgroup size from the
+ * smallest that holds its largest Z up to 512 lanes, on the caller's data,
+ * and keeps the fastest (written to *out_block). */
+int32_t syndrome_ldpc_plan_tune(SyndromeLdpcPlan *plan, const void *d_in, void *d_post, uint8_t *d_hard,
+                                int32_t *d_iters, size_t iterations, float scale, int16_t app_clamp, uint32_t flags,
+                                int32_t reps, int32_t *out_block);
+/* Rate de-matching on device (rate_dematch_llr, rate_matching.rs:307-369, and
+ * the input placement of prepare_cb_llr, transport_block.rs:473-496): block
+ * i's e_lengths[i] received LLRs at element e_offsets[i] of d_e_llr, with
+ * modulation order qm[i] and redundancy version rv[i] (NULL = 1 / 0), become
+ * its N_i-element decoder input at offsets[i] of d_llr_out. */
+int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *plan, const int64_t *e_offsets, const int32_t *e_lengths,
+                                       const int32_t *qm, const int32_t *rv);
+int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *plan, const float *d_e_llr, float *d_llr_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -564,3 +564,65 @@ int orc_encode(const orc_code_t *c, const uint8_t *info, uint8_t *cw)
         }
     return orc_syndrome_ok_bits(c, cw) ? 0 : -1;
 }
+
+/* ------------------------------------------------------------------ */
+/* rate matching                                                       */
+/* ------------------------------------------------------------------ */
+
+/* RV_K0_SETS / rv_k0 (rate_matching.rs:72-94) */
+static size_t rv_k0(int bg, int rv, int z)
+{
+    static const int j[2][4] = {{0, 17, 33, 56}, {0, 13, 25, 43}};
+    return (size_t)j[bg][rv] * (size_t)z;
+}
+
+int orc_rate_match(const uint8_t *cw, size_t cw_len, uint8_t *e_out, size_t e_len, int rv, int qm, int bg, int z,
+                   size_t n_filler)
+{
+    if (rv < 0 || rv >= 4 || qm <= 0 || z <= 0 || e_len % (size_t)qm) return -1;
+    const size_t ncb = (size_t)(bg == 0 ? 66 : 50) * z, k = (size_t)(bg == 0 ? 22 : 10) * z;
+    const size_t kp = n_filler > k ? 0 : k - n_filler, two_z = 2 * (size_t)z;
+    const size_t fs = kp > two_z ? kp - two_z : 0, fe = k > two_z ? k - two_z : 0;
+    const size_t k0 = rv_k0(bg, rv, z);
+    uint8_t *e = (uint8_t *)malloc(e_len ? e_len : 1);
+    size_t ks = 0, j = 0;
+    while (ks < e_len) {
+        size_t pos = (k0 + j) % ncb;
+        if (pos >= fs && pos < fe) { ++j; continue; }
+        size_t cp = pos + two_z;
+        e[ks++] = cp < cw_len ? cw[cp] : 0;
+        ++j;
+        if (j > 2 * ncb) break;
+    }
+    /* interleave: write row-major, read column-major (rate_matching.rs:101-113) */
+    size_t rows = e_len / (size_t)qm;
+    for (int col = 0; col < qm; ++col)
+        for (size_t r = 0; r < rows; ++r) e_out[(size_t)col * rows + r] = e[r * (size_t)qm + (size_t)col];
+    free(e);
+    return 0;
+}
+
+int orc_rate_dematch(const float *e_llr, size_t e_len, float *cb, size_t cb_len, int rv, int qm, int bg, int z,
+                     size_t n_filler)
+{
+    if (rv < 0 || rv >= 4 || qm <= 0 || z <= 0 || e_len % (size_t)qm) return -1;
+    const size_t ncb = (size_t)(bg == 0 ? 66 : 50) * z, k = (size_t)(bg == 0 ? 22 : 10) * z;
+    const size_t kp = n_filler > k ? 0 : k - n_filler, two_z = 2 * (size_t)z;
+    const size_t fs = kp > two_z ? kp - two_z : 0, fe = k > two_z ? k - two_z : 0;
+    size_t rows = e_len / (size_t)qm;
+    float *d = (float *)malloc(sizeof(float) * (e_len ? e_len : 1));
+    for (int col = 0; col < qm; ++col) /* deinterleave */
+        for (size_t r = 0; r < rows; ++r) d[r * (size_t)qm + (size_t)col] = e_llr[(size_t)col * rows + r];
+    const size_t k0 = rv_k0(bg, rv, z);
+    size_t ks = 0, j = 0;
+    while (ks < e_len) {
+        size_t pos = (k0 + j) % ncb;
+        if (pos >= fs && pos < fe) { ++j; continue; }
+        if (pos < cb_len) cb[pos] += d[ks];
+        ++ks;
+        ++j;
+        if (j > 2 * ncb) break;
+    }
+    free(d);
+    return 0;
+}

@@ -99,6 +99,14 @@ void orc_awgn_transmit(float ebno_db, float code_rate, uint64_t seed, const uint
  * Returns 0, or -1 if the parity part could not be solved. */
 int orc_encode(const orc_code_t *c, const uint8_t *info, uint8_t *codeword);
 
+/* rate_match (rate_matching.rs:181-260): codeword = full N = n_b*Z (68Z or
+ * 52Z) bits; e_out gets E bits.  bg 0/1.  Returns 0 or -1. */
+int orc_rate_match(const uint8_t *codeword, size_t cw_len, uint8_t *e_out, size_t e_len, int rv, int qm, int bg,
+                   int z, size_t n_filler);
+/* rate_dematch_llr (rate_matching.rs:307-369): cb_llr [Ncb] accumulates. */
+int orc_rate_dematch(const float *e_llr, size_t e_len, float *cb_llr, size_t cb_len, int rv, int qm, int bg, int z,
+                     size_t n_filler);
+
 /* Knuth multiplicative-hash info bits used by the reference's exporters
  * (ldpc_convergence_export.rs:145-147, ldpc_pipeline_bench.rs:247-250). */
 void orc_knuth_info_bits(uint8_t *bits, size_t k);

@@ -79,6 +79,8 @@ def lib():
         L.orc_awgn_transmit.argtypes = [C.c_float, C.c_float, C.c_uint64, P, C.c_size_t, P]
         L.orc_encode.argtypes = [C.POINTER(Code), P, P]
         L.orc_knuth_info_bits.argtypes = [P, C.c_size_t]
+        L.orc_rate_match.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t]
+        L.orc_rate_dematch.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t]
         _lib = L
     return _lib
 
@@ -272,3 +274,23 @@ def pipeline_run(code, beta, llr_batch, iterations, workers, init5g=True, n_fill
     ns = cpb().cpb_pipeline_run(C.byref(code), beta, _p(llr_batch), count, iterations, workers, int(init5g),
                                 n_filler, _p(iters), _p(hard))
     return ns * 1e-9, iters, hard
+
+
+def rate_match(codeword, e_len, rv, qm, bg, z, n_filler):
+    cw = np.ascontiguousarray(codeword, np.uint8)
+    out = np.zeros(e_len, np.uint8)
+    assert lib().orc_rate_match(_p(cw), cw.size, _p(out), e_len, rv, qm, bg, z, n_filler) == 0
+    return out
+
+
+def rate_dematch(e_llr, rv, qm, bg, z, n_filler, cb=None):
+    e = np.ascontiguousarray(e_llr, np.float32)
+    ncb = (66 if bg == 0 else 50) * z
+    cb = np.zeros(ncb, np.float32) if cb is None else np.array(cb, np.float32, copy=True)
+    assert lib().orc_rate_dematch(_p(e), e.size, _p(cb), cb.size, rv, qm, bg, z, n_filler) == 0
+    return cb
+
+
+def decoder_input_from_cb(cb, z):
+    """prepare_cb_llr placement (transport_block.rs:473-496): [2Z zeros | cb]."""
+    return np.concatenate([np.zeros(2 * z, np.float32), np.asarray(cb, np.float32)])

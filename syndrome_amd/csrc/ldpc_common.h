@@ -70,3 +70,25 @@ struct SynLaunch {
     int beta_q;            // i8 path: quantized offset
     int app_clamp;         // i8 path: max(app_clamp, 0)
 };
+
+// One work item of a work-queue launch: `count` codewords (ids
+// ids[first .. first+count)) of code `code` (an index into the launch's code
+// table), whose base graph is `kind`.
+struct SynItem {
+    int code;
+    int kind;
+    int first;
+    int count;
+};
+
+// One code block for the device rate de-matcher.
+struct SynDematchCb {
+    long long e_off;    // offset of its E received LLRs
+    long long out_off;  // offset of its N-element decoder input
+    int e;              // E
+    int qm;             // modulation order (bit interleaver columns)
+    int z;
+    int bg;             // 0 = BG1, 1 = BG2
+    int rv;             // redundancy version 0..3
+    int n_filler;
+};

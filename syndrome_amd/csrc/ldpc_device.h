@@ -97,23 +97,52 @@ struct SFor {
     }
 };
 
+// Where a workgroup's codewords live: `count` codewords, ids base + j (or
+// ids[j]), element offsets id * N (or offs[id]), filler counts the launch's
+// (or nfill[id]).  The contiguous form serves the plain batch launches; the
+// indirect form serves work-queue items over a packed arena of mixed codes.
+struct GroupIO {
+    int count;
+    int base;
+    const int *ids;
+    const long long *offs;
+    const int *nfill;
+
+    __device__ __forceinline__ int id(int j) const { return ids ? ids[j] : base + j; }
+    __device__ __forceinline__ long long off(int j, int N) const
+    {
+        return offs ? offs[id(j)] : (long long)id(j) * N;
+    }
+    __device__ __forceinline__ long long n_filler(int j, long long dflt) const { return nfill ? nfill[id(j)] : dflt; }
+};
+
+__device__ __forceinline__ GroupIO contiguous_group(int cw0, int cws, int batch)
+{
+    const int n = batch - cw0;
+    return GroupIO{n < cws ? n : cws, cw0, nullptr, nullptr, nullptr};
+}
+
 // Load the channel input of the workgroup's codewords into LDS, applying the
 // 5G prologue (init_5g_llr, qc_ldpc.rs:711-732 / decode_5g_i8, 1505-1514),
 // quantisation (quantize_llr_i16, quantize.rs:331-341) and the int8 path's
-// entry clamp (qc_ldpc.rs:1582-1584) on the way.
+// entry clamp (qc_ldpc.rs:1582-1584) on the way.  `cws` codeword slots are
+// filled; slots past io.count get zeros.
 template <bool I8>
-__device__ __forceinline__ void load_input(const SynLaunch &a, int Z, int n_b, int m_b, int N, int cw0,
-                                           void *lds)
+__device__ __forceinline__ void load_input(const SynLaunch &a, int Z, int n_b, int m_b, int N, int cws,
+                                           const GroupIO &io, void *lds)
 {
-    const int total = a.cws * N;
+    const int total = cws * N;
     const int K = (n_b - m_b) * Z;
-    const long long kp = a.n_filler >= K ? 0 : K - a.n_filler;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
-        const int cl = i / N, pos = i - cl * N, cw = cw0 + cl;
+        const int cl = i / N, pos = i - cl * N;
+        const bool live = cl < io.count;
+        const long long g = live ? io.off(cl, N) + pos : 0;
+        const long long nf = live ? io.n_filler(cl, a.n_filler) : 0;
+        const long long kp = nf >= K ? 0 : K - nf;
         if constexpr (!I8) {
             float v = 0.f;
-            if (cw < a.batch) {
-                v = ((const float *)a.in)[(size_t)cw * N + pos];
+            if (live) {
+                v = ((const float *)a.in)[g];
                 if (a.flags & SYN_F_INIT_5G) {
                     if (pos < 2 * Z) v = 0.f;
                     else if (pos >= kp && pos < K) v = 1000000.0f;
@@ -122,9 +151,9 @@ __device__ __forceinline__ void load_input(const SynLaunch &a, int Z, int n_b, i
             ((float *)lds)[i] = v;
         } else {
             int v = 0;
-            if (cw < a.batch) {
+            if (live) {
                 if (a.flags & SYN_F_QUANTIZE_IN) {
-                    float f = ((const float *)a.in)[(size_t)cw * N + pos] * a.scale;
+                    float f = ((const float *)a.in)[g] * a.scale;
                     if (f != f) v = 0;
                     else {
                         f = roundf(f);  // ties away from zero, as f32::round
@@ -132,7 +161,7 @@ __device__ __forceinline__ void load_input(const SynLaunch &a, int Z, int n_b, i
                         v = (int)f;
                     }
                 } else {
-                    v = ((const int16_t *)a.in)[(size_t)cw * N + pos];
+                    v = ((const int16_t *)a.in)[g];
                 }
                 if (a.flags & SYN_F_INIT_5G) {
                     if (pos < 2 * Z) v = 0;
@@ -146,13 +175,12 @@ __device__ __forceinline__ void load_input(const SynLaunch &a, int Z, int n_b, i
 }
 
 template <bool I8>
-__device__ __forceinline__ void store_output(const SynLaunch &a, int N, int cw0, const void *lds)
+__device__ __forceinline__ void store_output(const SynLaunch &a, int N, const GroupIO &io, const void *lds)
 {
-    const int total = a.cws * N;
+    const int total = io.count * N;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
-        const int cl = i / N, pos = i - cl * N, cw = cw0 + cl;
-        if (cw >= a.batch) continue;
-        const size_t g = (size_t)cw * N + pos;
+        const int cl = i / N, pos = i - cl * N;
+        const long long g = io.off(cl, N) + pos;
         if constexpr (!I8) {
             const float v = ((const float *)lds)[i];
             if (a.flags & SYN_F_WRITE_POST) ((float *)a.post)[g] = v;
@@ -177,13 +205,18 @@ __device__ __forceinline__ CwFlags cw_flags(void *base, int cws)
     return {p, p + cws, p + 2 * cws};
 }
 
-__device__ __forceinline__ void flags_init(CwFlags f, int cws, int cw0, int batch)
+__device__ __forceinline__ void flags_init(CwFlags f, int cws, const GroupIO &io)
 {
     if ((int)threadIdx.x < cws) {
         f.fail[threadIdx.x] = 0;
-        f.done[threadIdx.x] = (cw0 + (int)threadIdx.x >= batch) ? 1 : 0;
+        f.done[threadIdx.x] = ((int)threadIdx.x >= io.count) ? 1 : 0;
         f.used[threadIdx.x] = 0;
     }
+}
+
+__device__ __forceinline__ void store_iters(const SynLaunch &a, CwFlags f, const GroupIO &io)
+{
+    if ((int)threadIdx.x < io.count) a.iters[io.id(threadIdx.x)] = f.used[threadIdx.x];
 }
 
 __device__ __forceinline__ bool all_done(CwFlags f, int cws)

@@ -372,27 +372,31 @@ __device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC> &geo
     return fail;
 }
 
-// Shared kernel body.  I8 selects the int8 path.
+// Decode one workgroup's codewords: `cws` codeword slots laid out in LDS
+// and over the lanes, `io` saying where the live ones come from and go to.
+// I8 selects the int8 path.  Waves without a live lane keep only the
+// barriers.
 template <int KIND, int ZC, bool I8>
-__device__ __forceinline__ void fast_body(const SynCode *__restrict__ code, const SynLaunch &a)
+__device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int Z, int cws, const GroupIO &io,
+                                           const SynLaunch &a)
 {
     using G = Graph<KIND>;
     using State = std::conditional_t<I8, StateI8<G>, StateF32<G>>;
     constexpr int ESZ = I8 ? 2 : 4;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int Z = (ZC > 0) ? ZC : code->z;
-    const int N = G::COLS * Z, cws = a.cws, cw0 = blockIdx.x * cws;
+    const int N = G::COLS * Z;
     const int EZ = G::EDGES * Z;
     const int pad = fast_pad(Z, ESZ);
     unsigned char *P0 = smem + pad;
     CwFlags fl = cw_flags(P0 + ((cws * N * ESZ + 3) & ~3), cws);
     const int t = threadIdx.x;
     const int *__restrict__ shifts = code->shifts;
-    load_input<I8>(a, Z, G::COLS, G::ROWS, N, cw0, P0);
-    flags_init(fl, cws, cw0, a.batch);
+    load_input<I8>(a, Z, G::COLS, G::ROWS, N, cws, io, P0);
+    flags_init(fl, cws, io);
     __syncthreads();
 
     const bool lane_ok = t < cws * Z;
+    const bool wave_live = wave_any(lane_ok && (t / Z) < io.count);
     const int cl = lane_ok ? t / Z : 0;
     const int z = lane_ok ? t - cl * Z : 0;
     const int zb0 = pad + (cl * N + z) * ESZ;
@@ -423,8 +427,10 @@ __device__ __forceinline__ void fast_body(const SynCode *__restrict__ code, cons
         const Geo<G, ZC> geo{Z, shifts + opaque_s(0)};
         SFor<0, G::ROWS>::run([&](auto lc) {
             constexpr int L = decltype(lc)::value;
-            if constexpr (I8) layer_i8<G, ZC, L>(ln, geo, st, bq, C, act);
-            else layer_f32<G, ZC, L>(ln, geo, st, beta, act);
+            if (wave_live) {
+                if constexpr (I8) layer_i8<G, ZC, L>(ln, geo, st, bq, C, act);
+                else layer_f32<G, ZC, L>(ln, geo, st, beta, act);
+            }
             if constexpr (barrier_after<G>(L)) __syncthreads();
         });
         check_rows<G, ZC, std::conditional_t<I8, int16_t, float>>(ln, geo, act, fl.fail + cl);
@@ -432,16 +438,17 @@ __device__ __forceinline__ void fast_body(const SynCode *__restrict__ code, cons
         flags_settle(fl, cws);
         __syncthreads();
     }
-    store_output<I8>(a, N, cw0, P0);
-    if (t < cws && cw0 + t < a.batch) a.iters[cw0 + t] = fl.used[t];
-    if ((a.flags & SYN_F_WRITE_EDGE) && lane_ok && cw0 + cl < a.batch) {
+    store_output<I8>(a, N, io, P0);
+    store_iters(a, fl, io);
+    if ((a.flags & SYN_F_WRITE_EDGE) && lane_ok && cl < io.count) {
+        const long long ebase = (long long)io.id(cl) * EZ;
         SFor<0, G::ROWS>::run([&](auto lc) {
             constexpr int L = decltype(lc)::value;
             constexpr int D = G::deg(L), B = layer_begin<G>(L);
             const uint32_t sw = sg_get<G, L>(st.sg);
             SFor<0, D>::run([&](auto ec) {
                 constexpr int e = decltype(ec)::value;
-                const size_t o = (size_t)(cw0 + cl) * EZ + (size_t)(B + e) * Z + z;
+                const size_t o = (size_t)ebase + (size_t)(B + e) * Z + z;
                 if constexpr (I8) {
                     const uint32_t mw = mg_get<G, L>(st);
                     ((int8_t *)a.edge)[o] =
@@ -454,6 +461,14 @@ __device__ __forceinline__ void fast_body(const SynCode *__restrict__ code, cons
     }
 }
 
+template <int KIND, int ZC, bool I8>
+__device__ __forceinline__ void fast_body(const SynCode *__restrict__ code, const SynLaunch &a)
+{
+    const int Z = (ZC > 0) ? ZC : code->z;
+    const GroupIO io = contiguous_group(blockIdx.x * a.cws, a.cws, a.batch);
+    fast_group<KIND, ZC, I8>(code, Z, a.cws, io, a);
+}
+
 template <int KIND, int ZC>
 __global__ void __launch_bounds__(512) fast_f32(const SynCode *__restrict__ code, SynLaunch a)
 {
@@ -464,6 +479,37 @@ template <int KIND, int ZC>
 __global__ void __launch_bounds__(512) fast_i8(const SynCode *__restrict__ code, SynLaunch a)
 {
     fast_body<KIND, ZC, true>(code, a);
+}
+
+}  // namespace syn
+
+namespace syn {
+
+// Persistent work-queue decoder: a grid sized to the resident workgroups
+// pulls items off a device counter until the queue is drained (every
+// workgroup reaches the exit test with the same counter, so all of them
+// leave), decoding each item with the register-resident body for its base
+// graph at its lifting size.  Items of different base graphs, lifting sizes
+// and filler counts share one launch.
+template <bool I8>
+__device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, const SynItem *__restrict__ items,
+                                             int n_items, int *head, const int *__restrict__ ids,
+                                             const long long *__restrict__ offs, const int *__restrict__ nfill,
+                                             const SynLaunch &a)
+{
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();  // the previous item is finished with LDS and s_item
+        if (threadIdx.x == 0) s_item = atomicAdd(head, 1);
+        __syncthreads();
+        const int w = s_item;
+        if (w >= n_items) return;
+        const SynItem it = items[w];
+        const SynCode *code = codes + it.code;
+        const GroupIO io{it.count, 0, ids + it.first, offs, nfill};
+        if (it.kind == SYN_KIND_BG1) fast_group<SYN_KIND_BG1, 0, I8>(code, code->z, it.count, io, a);
+        else fast_group<SYN_KIND_BG2, 0, I8>(code, code->z, it.count, io, a);
+    }
 }
 
 }  // namespace syn

@@ -16,11 +16,12 @@ __global__ void __launch_bounds__(1024) plain_f32(const SynCode *__restrict__ co
     float *P0 = (float *)smem;
     CwFlags fl = cw_flags(P0 + cws * N, cws);
     const int t = threadIdx.x;
-    load_input<false>(a, Z, n_b, m_b, N, cw0, P0);
+    const GroupIO io = contiguous_group(cw0, cws, a.batch);
+    load_input<false>(a, Z, n_b, m_b, N, cws, io, P0);
     float *edge = (float *)a.edge;
     for (int i = t; i < cws * EZ; i += blockDim.x)
         if (cw0 + i / EZ < a.batch) edge[(size_t)cw0 * EZ + i] = 0.f;  // qc_ldpc.rs:1006-1009
-    flags_init(fl, cws, cw0, a.batch);
+    flags_init(fl, cws, io);
     __syncthreads();
 
     const bool lane_ok = t < cws * Z;
@@ -85,8 +86,8 @@ __global__ void __launch_bounds__(1024) plain_f32(const SynCode *__restrict__ co
         flags_settle(fl, cws);
         __syncthreads();
     }
-    store_output<false>(a, N, cw0, P0);
-    if (t < cws && cw0 + t < a.batch) a.iters[cw0 + t] = fl.used[t];
+    store_output<false>(a, N, io, P0);
+    store_iters(a, fl, io);
 }
 
 __global__ void __launch_bounds__(1024) plain_i8(const SynCode *__restrict__ code, SynLaunch a)
@@ -97,11 +98,12 @@ __global__ void __launch_bounds__(1024) plain_i8(const SynCode *__restrict__ cod
     int16_t *P0 = (int16_t *)smem;
     CwFlags fl = cw_flags(P0 + ((cws * N + 1) & ~1), cws);
     const int t = threadIdx.x;
-    load_input<true>(a, Z, n_b, m_b, N, cw0, P0);
+    const GroupIO io = contiguous_group(cw0, cws, a.batch);
+    load_input<true>(a, Z, n_b, m_b, N, cws, io, P0);
     int8_t *edge = (int8_t *)a.edge;
     for (int i = t; i < cws * EZ; i += blockDim.x)
         if (cw0 + i / EZ < a.batch) edge[(size_t)cw0 * EZ + i] = 0;  // qc_ldpc.rs:1579-1581
-    flags_init(fl, cws, cw0, a.batch);
+    flags_init(fl, cws, io);
     __syncthreads();
 
     const bool lane_ok = t < cws * Z;
@@ -170,8 +172,8 @@ __global__ void __launch_bounds__(1024) plain_i8(const SynCode *__restrict__ cod
         flags_settle(fl, cws);
         __syncthreads();
     }
-    store_output<true>(a, N, cw0, P0);
-    if (t < cws && cw0 + t < a.batch) a.iters[cw0 + t] = fl.used[t];
+    store_output<true>(a, N, io, P0);
+    store_iters(a, fl, io);
 }
 
 // quantize_llr_i16 (quantize.rs:331-341) as a standalone pass.

@@ -1,0 +1,358 @@
+// Work-queue batch plans: many code blocks of different 3GPP base graphs,
+// lifting sizes and filler counts decoded in one persistent launch, plus the
+// device rate de-matcher that feeds them and the launch-shape tuner.
+#include <hip/hip_runtime.h>
+
+#include <limits.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/syndrome_ldpc.h"
+#include "code_host.h"
+#include "ldpc_common.h"
+
+namespace syn {
+const void *persist_kernel_f32();
+const void *persist_kernel_i8();
+int launch_reset(int *head, hipStream_t stream);
+int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, int count, int max_n, hipStream_t stream);
+}  // namespace syn
+
+struct SyndromeLdpcPlan {
+    int device = 0;
+    bool i8 = false;
+    float beta = 0.f;
+    int lds_limit = 0;
+    int cus = 0;
+    // per code block
+    std::vector<int> bg, z, nfill;
+    std::vector<long long> offs;
+    // unique codes
+    std::vector<SynCode> codes;
+    std::vector<int> cb_code;
+    // current launch shape
+    int block = 0, grid = 0, n_items = 0;
+    size_t lds = 0;
+    std::vector<int> cws_of_code;
+    // device buffers
+    SynCode *d_codes = nullptr;
+    SynItem *d_items = nullptr;
+    int *d_ids = nullptr, *d_nfill = nullptr, *d_head = nullptr;
+    long long *d_offs = nullptr;
+    SynDematchCb *d_dm = nullptr;
+    int dm_max_n = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+template <class F>
+int32_t guard(F &&body)
+{
+    try {
+        return body();
+    } catch (...) {
+        return SYNDROME_PANIC;
+    }
+}
+
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceScope()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+template <class T>
+int32_t upload(T **dst, const std::vector<T> &src)
+{
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    if (src.empty()) return SYNDROME_OK;
+    if (hipMalloc((void **)dst, sizeof(T) * src.size()) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    if (hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    return SYNDROME_OK;
+}
+
+size_t group_bytes(const SynCode &c, int cws, bool i8)
+{
+    const size_t esz = i8 ? 2 : 4, n = syn::code_n(c);
+    const size_t pad = (size_t)(((c.z * (int)esz) + 15) & ~15);
+    return pad + ((cws * n * esz + 3) & ~(size_t)3) + 3 * sizeof(int) * (size_t)cws;
+}
+
+// Items for a workgroup of `block` lanes: each code's blocks in groups of
+// as many codewords as fit the lanes and the LDS, longest work first.
+int32_t build_items(SyndromeLdpcPlan *p, int block)
+{
+    if (block <= 0 || block > 512 || block % 64) return SYNDROME_INVALID_PARAM;
+    const size_t nc = p->codes.size();
+    std::vector<int> cws(nc);
+    size_t lds = 0;
+    for (size_t c = 0; c < nc; ++c) {
+        const SynCode &code = p->codes[c];
+        if (code.z > block) return SYNDROME_INVALID_PARAM;
+        int k = std::max(1, block / code.z);
+        while (k > 1 && group_bytes(code, k, p->i8) > (size_t)p->lds_limit) --k;
+        if (group_bytes(code, k, p->i8) > (size_t)p->lds_limit) return SYNDROME_INVALID_PARAM;
+        cws[c] = k;
+        lds = std::max(lds, group_bytes(code, k, p->i8));
+    }
+    std::vector<std::vector<int>> members(nc);
+    for (size_t i = 0; i < p->cb_code.size(); ++i) members[p->cb_code[i]].push_back((int)i);
+    struct Item {
+        SynItem it;
+        long long cost;
+    };
+    std::vector<Item> items;
+    std::vector<int> ids;
+    for (size_t c = 0; c < nc; ++c)
+        for (size_t j = 0; j < members[c].size(); j += (size_t)cws[c]) {
+            const int cnt = (int)std::min(members[c].size() - j, (size_t)cws[c]);
+            Item x;
+            x.it = SynItem{(int)c, p->codes[c].kind, 0, cnt};
+            x.cost = (long long)p->codes[c].n_edges * p->codes[c].z * ((cnt + 0LL));
+            x.it.first = (int)ids.size();
+            for (int q = 0; q < cnt; ++q) ids.push_back(members[c][j + q]);
+            items.push_back(x);
+        }
+    // longest-processing-time first: the queue hands out big items early
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+    std::vector<SynItem> flat;
+    for (auto &x : items) flat.push_back(x.it);
+    if (int32_t rc = upload(&p->d_items, flat)) return rc;
+    if (int32_t rc = upload(&p->d_ids, ids)) return rc;
+    const void *fn = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32();
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    p->block = block;
+    p->lds = lds;
+    p->n_items = (int)flat.size();
+    p->grid = std::max(1, std::min(p->n_items, per_cu * p->cus));
+    p->cws_of_code = cws;
+    return SYNDROME_OK;
+}
+
+int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *hard, int32_t *iters, size_t iterations,
+                    float scale, int16_t app_clamp, uint32_t flags, hipStream_t stream)
+{
+    SynLaunch a{};
+    a.in = in;
+    a.post = post;
+    a.hard = hard;
+    a.iters = iters;
+    a.batch = (int)p->bg.size();
+    a.iterations = (int)std::min(iterations, (size_t)INT_MAX);
+    a.flags = flags & (SYN_F_INIT_5G | SYN_F_WRITE_POST | SYN_F_WRITE_HARD | SYN_F_QUANTIZE_IN);
+    a.beta = p->beta;
+    a.scale = scale;
+    a.app_clamp = app_clamp < 0 ? 0 : app_clamp;
+    if (p->i8) {
+        const float b = p->beta * scale;
+        a.beta_q = !(b - b == 0.f) ? 127 : (int)std::min(127.f, std::max(0.f, roundf(b)));
+    }
+    if (syn::launch_reset(p->d_head, stream)) return SYNDROME_DEVICE_ERROR;
+    const void *fn = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32();
+    const SynCode *codes = p->d_codes;
+    const SynItem *items = p->d_items;
+    int n_items = p->n_items;
+    int *head = p->d_head;
+    const int *ids = p->d_ids;
+    const long long *offs = p->d_offs;
+    const int *nf = p->d_nfill;
+    void *args[] = {&codes, &items, &n_items, &head, &ids, &offs, &nf, &a};
+    if (hipLaunchKernel(fn, dim3(p->grid), dim3(p->block), args, p->lds, stream) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    return SYNDROME_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int32_t *n_filler, const int64_t *offsets,
+                                  size_t count, float offset_beta, int32_t int8, int32_t block,
+                                  SyndromeLdpcPlan **out)
+{
+    return guard([&]() -> int32_t {
+        if (!out) return SYNDROME_NULL_POINTER;
+        *out = nullptr;
+        if (count && (!bg || !z || !offsets)) return SYNDROME_NULL_POINTER;
+        if (count == 0 || count > (size_t)INT_MAX / 2) return SYNDROME_INVALID_PARAM;
+        std::unique_ptr<SyndromeLdpcPlan> p(new SyndromeLdpcPlan());
+        p->i8 = int8 != 0;
+        p->beta = offset_beta;
+        if (hipGetDevice(&p->device) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        if (hipDeviceGetAttribute(&p->lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device) != hipSuccess ||
+            hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
+            return SYNDROME_DEVICE_ERROR;
+        std::map<std::pair<int, int>, int> index;
+        int max_z = 0;
+        for (size_t i = 0; i < count; ++i) {
+            if (bg[i] != 0 && bg[i] != 1) return SYNDROME_INVALID_PARAM;
+            if (offsets[i] < 0 || (n_filler && n_filler[i] < 0)) return SYNDROME_INVALID_PARAM;
+            auto key = std::make_pair((int)bg[i], (int)z[i]);
+            auto f = index.find(key);
+            if (f == index.end()) {
+                SynCode c;
+                if (syn::build_code_3gpp(bg[i], z[i], &c)) return SYNDROME_INVALID_PARAM;
+                f = index.emplace(key, (int)p->codes.size()).first;
+                p->codes.push_back(c);
+            }
+            p->cb_code.push_back(f->second);
+            p->bg.push_back(bg[i]);
+            p->z.push_back(z[i]);
+            p->nfill.push_back(n_filler ? n_filler[i] : 0);
+            p->offs.push_back(offsets[i]);
+            max_z = std::max(max_z, (int)z[i]);
+        }
+        DeviceScope ds(p->device);
+        if (int32_t rc = upload(&p->d_codes, p->codes)) return rc;
+        if (int32_t rc = upload(&p->d_offs, p->offs)) return rc;
+        if (int32_t rc = upload(&p->d_nfill, p->nfill)) return rc;
+        if (hipMalloc((void **)&p->d_head, 64) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        const int min_block = ((max_z + 63) / 64) * 64;
+        const int b = block > 0 ? block : std::max(min_block, 512);
+        if (int32_t rc = build_items(p.get(), std::max(b, min_block))) return rc;
+        *out = p.release();
+        return SYNDROME_OK;
+    });
+}
+
+void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *p)
+{
+    if (!p) return;
+    try {
+        DeviceScope ds(p->device);
+        for (void *q : {(void *)p->d_codes, (void *)p->d_items, (void *)p->d_ids, (void *)p->d_nfill,
+                        (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm})
+            if (q) (void)hipFree(q);
+    } catch (...) {
+    }
+    delete p;
+}
+
+int32_t syndrome_ldpc_plan_run(SyndromeLdpcPlan *p, const void *d_in, void *d_post, uint8_t *d_hard, int32_t *d_iters,
+                               size_t iterations, float scale, int16_t app_clamp, uint32_t flags, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if (!p || !d_in || !d_iters) return SYNDROME_NULL_POINTER;
+        if ((flags & SYNDROME_BATCH_WRITE_POST) && !d_post) return SYNDROME_NULL_POINTER;
+        if ((flags & SYNDROME_BATCH_WRITE_HARD) && !d_hard) return SYNDROME_NULL_POINTER;
+        if ((flags & SYNDROME_BATCH_QUANTIZE) && !p->i8) return SYNDROME_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(p->mu);
+        DeviceScope ds(p->device);
+        return plan_launch(p, d_in, d_post, d_hard, d_iters, iterations, scale, app_clamp, flags, (hipStream_t)stream);
+    });
+}
+
+int32_t syndrome_ldpc_plan_shape(const SyndromeLdpcPlan *p, int32_t *block, int32_t *grid, int32_t *items,
+                                 int32_t *lds_bytes)
+{
+    return guard([&]() -> int32_t {
+        if (!p) return SYNDROME_NULL_POINTER;
+        if (block) *block = p->block;
+        if (grid) *grid = p->grid;
+        if (items) *items = p->n_items;
+        if (lds_bytes) *lds_bytes = (int32_t)p->lds;
+        return SYNDROME_OK;
+    });
+}
+
+// Launch-shape tuner: time the plan at each candidate workgroup size (the
+// lanes that set how many codewords of each code share a workgroup and
+// the LDS of the launch) on the caller's data and keep the fastest.
+int32_t syndrome_ldpc_plan_tune(SyndromeLdpcPlan *p, const void *d_in, void *d_post, uint8_t *d_hard,
+                                int32_t *d_iters, size_t iterations, float scale, int16_t app_clamp, uint32_t flags,
+                                int32_t reps, int32_t *out_block)
+{
+    return guard([&]() -> int32_t {
+        if (!p || !d_in || !d_iters) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(p->mu);
+        DeviceScope ds(p->device);
+        int max_z = 0;
+        for (int zz : p->z) max_z = std::max(max_z, zz);
+        const int min_block = ((max_z + 63) / 64) * 64;
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        hipEvent_t e0, e1;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        float best_ms = 1e30f;
+        int best = p->block;
+        const int nrep = reps > 0 ? reps : 2;
+        for (int b = min_block; b <= 512; b += 64) {
+            if (build_items(p, b)) continue;
+            if (plan_launch(p, d_in, d_post, d_hard, d_iters, iterations, scale, app_clamp, flags, s)) continue;
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < nrep; ++r)
+                (void)plan_launch(p, d_in, d_post, d_hard, d_iters, iterations, scale, app_clamp, flags, s);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = b;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        if (int32_t rc = build_items(p, best)) return rc;
+        if (out_block) *out_block = best;
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *p, const int64_t *e_offsets, const int32_t *e_lengths,
+                                       const int32_t *qm, const int32_t *rv)
+{
+    return guard([&]() -> int32_t {
+        if (!p || !e_offsets || !e_lengths) return SYNDROME_NULL_POINTER;
+        std::vector<SynDematchCb> v(p->bg.size());
+        int max_n = 0;
+        for (size_t i = 0; i < v.size(); ++i) {
+            const int q = qm ? qm[i] : 1, r = rv ? rv[i] : 0;
+            if (q <= 0 || e_lengths[i] < 0 || e_lengths[i] % q || r < 0 || r > 3 || e_offsets[i] < 0)
+                return SYNDROME_INVALID_PARAM;
+            v[i] = SynDematchCb{e_offsets[i], p->offs[i], e_lengths[i], q, p->z[i], p->bg[i], r, p->nfill[i]};
+            max_n = std::max(max_n, (p->bg[i] == 0 ? 68 : 52) * p->z[i]);
+        }
+        std::lock_guard<std::mutex> lk(p->mu);
+        DeviceScope ds(p->device);
+        if (int32_t rc = upload(&p->d_dm, v)) return rc;
+        p->dm_max_n = max_n;
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *p, const float *d_e_llr, float *d_llr_out, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if (!p || !d_e_llr || !d_llr_out) return SYNDROME_NULL_POINTER;
+        if (!p->d_dm) return SYNDROME_INVALID_PARAM;
+        DeviceScope ds(p->device);
+        return syn::launch_dematch(d_e_llr, d_llr_out, p->d_dm, (int)p->bg.size(), p->dm_max_n, (hipStream_t)stream)
+                   ? SYNDROME_DEVICE_ERROR
+                   : SYNDROME_OK;
+    });
+}
+
+}  // extern "C"

@@ -248,3 +248,72 @@ def awgn_transmit(ebno_db, code_rate, seed, bits):
     out = np.zeros(bits.size, np.float32)
     check(lib().syndrome_awgn_transmit(ebno_db, code_rate, seed, _ptr(bits), bits.size, _ptr(out)), "awgn")
     return out
+
+
+class DecodePlan:
+    """Work-queue batch of mixed 3GPP code blocks decoded in one persistent
+    launch (include/syndrome_ldpc.h Part 4).  Block i: base graph bg[i]
+    (0/1), lifting size z[i], n_filler[i]; its N_i inputs start at element
+    offsets[i] of the arenas."""
+
+    def __init__(self, bg, z, n_filler, offsets, offset_beta, int8=False, block=0):
+        self.bg = np.ascontiguousarray(bg, np.int32)
+        self.z = np.ascontiguousarray(z, np.int32)
+        self.n_filler = np.ascontiguousarray(n_filler if n_filler is not None else np.zeros(len(self.bg)), np.int32)
+        self.offsets = np.ascontiguousarray(offsets, np.int64)
+        self.int8 = bool(int8)
+        out = C.c_void_p()
+        check(lib().syndrome_ldpc_plan_create(_ptr(self.bg), _ptr(self.z), _ptr(self.n_filler), _ptr(self.offsets),
+                                              len(self.bg), offset_beta, int(self.int8), block, C.byref(out)),
+              "plan_create")
+        self._h = out
+
+    def __del__(self):
+        try:
+            if self._h and self._h.value:
+                lib().syndrome_ldpc_plan_destroy(self._h)
+        except Exception:
+            pass
+
+    @property
+    def count(self):
+        return len(self.bg)
+
+    def n_of(self):
+        return np.where(self.bg == 0, 68, 52) * self.z
+
+    def shape(self):
+        b, g, it, lds = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().syndrome_ldpc_plan_shape(self._h, C.byref(b), C.byref(g), C.byref(it), C.byref(lds)), "plan_shape")
+        return {"block": b.value, "grid": g.value, "items": it.value, "lds": lds.value}
+
+    def run(self, d_in, d_post, d_hard, d_iters, iterations, flags, quant=None, stream=None):
+        import torch
+        q = quant or QuantParams()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().syndrome_ldpc_plan_run(self._h, d_in.data_ptr(), d_post.data_ptr() if d_post is not None else None,
+                                           d_hard.data_ptr() if d_hard is not None else None, d_iters.data_ptr(),
+                                           iterations, q.scale, q.app_clamp, flags, s), "plan_run")
+
+    def tune(self, d_in, d_post, d_hard, d_iters, iterations, flags, quant=None, reps=2):
+        q = quant or QuantParams()
+        best = C.c_int32()
+        check(lib().syndrome_ldpc_plan_tune(self._h, d_in.data_ptr(), d_post.data_ptr() if d_post is not None else None,
+                                            d_hard.data_ptr() if d_hard is not None else None, d_iters.data_ptr(),
+                                            iterations, q.scale, q.app_clamp, flags, reps, C.byref(best)),
+              "plan_tune")
+        return best.value
+
+    def set_dematch(self, e_offsets, e_lengths, qm=None, rv=None):
+        self._eo = np.ascontiguousarray(e_offsets, np.int64)
+        self._el = np.ascontiguousarray(e_lengths, np.int32)
+        self._qm = None if qm is None else np.ascontiguousarray(qm, np.int32)
+        self._rv = None if rv is None else np.ascontiguousarray(rv, np.int32)
+        check(lib().syndrome_ldpc_plan_set_dematch(self._h, _ptr(self._eo), _ptr(self._el),
+                                                   None if self._qm is None else _ptr(self._qm),
+                                                   None if self._rv is None else _ptr(self._rv)), "plan_set_dematch")
+
+    def dematch(self, d_e_llr, d_llr_out, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().syndrome_ldpc_plan_dematch(self._h, d_e_llr.data_ptr(), d_llr_out.data_ptr(), s), "plan_dematch")

@@ -1,0 +1,148 @@
+"""GPU parity of the work-queue (persistent) decoder over mixed codes, its
+launch-shape tuner, and the device rate de-matcher.
+
+Every code block in a mixed batch must come out exactly as the oracle
+decodes it alone (decode_5g with its own filler count): iterations,
+posterior and hard decisions bit for bit; and exactly as the per-config
+batch path decodes it alone.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import f32_bits, say
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+MIX = [(0, 384), (1, 208), (0, 2), (1, 3), (0, 44), (1, 52), (0, 104), (1, 384), (0, 15), (1, 120), (0, 256),
+       (1, 7), (0, 352), (1, 26)]
+
+
+def _mixed_inputs(oracle, count, seed, with_filler=True):
+    rng = np.random.default_rng(seed)
+    bg, z, nf, blocks = [], [], [], []
+    for i in range(count):
+        b, zz = MIX[i % len(MIX)]
+        code = oracle.code_3gpp(b, zz)
+        f = int(rng.integers(0, min(64, code.k // 4) + 1)) if with_filler else 0
+        info = oracle.xorshift_bits(seed * 1000 + i, code.k)
+        info[code.k - f:] = 0  # fillers are known zeros
+        cw = oracle.encode(code, info)
+        ebno = float(rng.uniform(-0.5, 3.0))
+        llr = oracle.awgn(ebno, code.k / code.n, seed + 77 * i, cw)
+        bg.append(b)
+        z.append(zz)
+        nf.append(f)
+        blocks.append(llr)
+    offs = np.concatenate([[0], np.cumsum([x.size for x in blocks])[:-1]]).astype(np.int64)
+    return np.array(bg), np.array(z), np.array(nf), offs, np.concatenate(blocks).astype(np.float32), blocks
+
+
+def test_plan_f32_mixed_matches_oracle_and_alone(dev_ok, oracle):
+    import syndrome_amd._lib as L
+    from syndrome_amd import DecodePlan, QcLdpcDecoder
+    say("[plan f32] 28 mixed code blocks")
+    bg, z, nf, offs, arena, blocks = _mixed_inputs(oracle, 28, 5)
+    plan = DecodePlan(bg, z, nf, offs, 0.5)
+    say("  shape", plan.shape())
+    d_in = torch.from_numpy(arena).cuda()
+    d_post = torch.empty_like(d_in)
+    d_hard = torch.empty(arena.size, dtype=torch.uint8, device="cuda")
+    d_it = torch.empty(len(bg), dtype=torch.int32, device="cuda")
+    flags = L.INIT_5G | L.WRITE_POST | L.WRITE_HARD
+    plan.run(d_in, d_post, d_hard, d_it, 8, flags)
+    torch.cuda.synchronize()
+    post, hard, its = d_post.cpu().numpy(), d_hard.cpu().numpy(), d_it.cpu().numpy()
+    for i in range(len(bg)):
+        code = oracle.code_3gpp(int(bg[i]), int(z[i]))
+        ri, rp, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, blocks[i], int(nf[i])), 8)
+        o, n = offs[i], code.n
+        assert its[i] == ri, f"block {i} (BG{bg[i] + 1} Z={z[i]}): iterations {its[i]} != {ri}"
+        assert np.array_equal(hard[o:o + n], rh), f"block {i}: hard differs"
+        assert np.array_equal(f32_bits(post[o:o + n]), f32_bits(rp)), f"block {i}: posterior differs"
+    # alone, through the per-config batch path: identical
+    for i in (0, 1, 5, 9):
+        dec = QcLdpcDecoder.with_lifting_size(int(bg[i]), int(z[i]), 0.5)
+        x = torch.from_numpy(blocks[i]).cuda()[None, :]
+        p1, h1, i1 = torch.empty_like(x), torch.empty(x.shape, dtype=torch.uint8, device="cuda"), \
+            torch.empty(1, dtype=torch.int32, device="cuda")
+        dec.decode_batch_f32(x, p1, h1, i1, 8, n_filler=int(nf[i]), flags=flags)
+        torch.cuda.synchronize()
+        o, n = offs[i], blocks[i].size
+        assert int(i1.item()) == its[i]
+        assert np.array_equal(f32_bits(p1.cpu().numpy()[0]), f32_bits(post[o:o + n]))
+    # the tuner may change the launch shape, never the results
+    best = plan.tune(d_in, d_post, d_hard, d_it, 8, flags)
+    say("  tuned block", best, plan.shape())
+    d_post2, d_hard2, d_it2 = torch.empty_like(d_post), torch.empty_like(d_hard), torch.empty_like(d_it)
+    plan.run(d_in, d_post2, d_hard2, d_it2, 8, flags)
+    torch.cuda.synchronize()
+    assert torch.equal(d_it2, d_it) and torch.equal(d_hard2, d_hard)
+    assert torch.equal(d_post2.view(torch.int32), d_post.view(torch.int32))
+
+
+def test_plan_i8_mixed_matches_oracle(dev_ok, oracle):
+    import syndrome_amd._lib as L
+    from syndrome_amd import DecodePlan, QuantParams
+    say("[plan i8] 28 mixed code blocks, fused quantizer")
+    bg, z, nf, offs, arena, blocks = _mixed_inputs(oracle, 28, 9)
+    plan = DecodePlan(bg, z, nf, offs, 0.5, int8=True)
+    d_in = torch.from_numpy(arena).cuda()
+    d_post = torch.empty(arena.size, dtype=torch.int16, device="cuda")
+    d_hard = torch.empty(arena.size, dtype=torch.uint8, device="cuda")
+    d_it = torch.empty(len(bg), dtype=torch.int32, device="cuda")
+    plan.run(d_in, d_post, d_hard, d_it, 8, L.INIT_5G | L.WRITE_POST | L.WRITE_HARD | L.QUANTIZE,
+             quant=QuantParams(8.0, 32767))
+    torch.cuda.synchronize()
+    post, hard, its = d_post.cpu().numpy(), d_hard.cpu().numpy(), d_it.cpu().numpy()
+    for i in range(len(bg)):
+        code = oracle.code_3gpp(int(bg[i]), int(z[i]))
+        app = oracle.init_5g_i8(code, oracle.quantize_i16(blocks[i], 8.0), int(nf[i]), 32767)
+        ri, ra, _, rh = oracle.decode_i8(code, 0.5, app, 8)
+        o, n = offs[i], code.n
+        assert its[i] == ri, f"block {i}: iterations"
+        assert np.array_equal(post[o:o + n], ra), f"block {i}: app differs"
+        assert np.array_equal(hard[o:o + n], rh), f"block {i}: hard differs"
+
+
+def test_device_dematch_matches_oracle(dev_ok, oracle):
+    from syndrome_amd import DecodePlan
+    say("[dematch] 20 code blocks, Qm in {1,2,4,6}, rv 0..3, repetition and puncturing")
+    rng = np.random.default_rng(3)
+    bg, z, nf, e_len, qm, rv = [], [], [], [], [], []
+    for i in range(20):
+        b, zz = MIX[i % len(MIX)]
+        k = (22 if b == 0 else 10) * zz
+        q = [1, 2, 4, 6][i % 4]
+        f = int(rng.integers(0, min(40, k // 4) + 1))
+        ncb = (66 if b == 0 else 50) * zz
+        e = int(rng.integers(max(q, ncb // 3), 2 * ncb)) // q * q  # punctured and repeated lengths
+        bg.append(b)
+        z.append(zz)
+        nf.append(f)
+        e_len.append(e)
+        qm.append(q)
+        rv.append(i % 4)
+    n = [(68 if b == 0 else 52) * zz for b, zz in zip(bg, z)]
+    offs = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+    e_offs = np.concatenate([[0], np.cumsum(e_len)[:-1]]).astype(np.int64)
+    e_llr = rng.normal(0, 3, int(sum(e_len))).astype(np.float32)
+    plan = DecodePlan(bg, z, nf, offs, 0.5)
+    plan.set_dematch(e_offs, e_len, qm, rv)
+    d_e = torch.from_numpy(e_llr).cuda()
+    d_out = torch.full((int(sum(n)),), 7.0, dtype=torch.float32, device="cuda")
+    plan.dematch(d_e, d_out)
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy()
+    for i in range(20):
+        cb = oracle.rate_dematch(e_llr[e_offs[i]:e_offs[i] + e_len[i]], rv[i], qm[i], bg[i], z[i], nf[i])
+        ref = oracle.decoder_input_from_cb(cb, z[i])
+        assert np.array_equal(f32_bits(out[offs[i]:offs[i] + n[i]]), f32_bits(ref)), f"block {i}"
+
+
+@pytest.fixture(scope="module")
+def dev_ok():
+    assert torch.cuda.is_available()
+    import syndrome_amd
+    syndrome_amd.lib()
+    return True
