@@ -205,6 +205,56 @@ int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *plan, const int64_t *e_
                                        const int32_t *qm, const int32_t *rv);
 int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *plan, const float *d_e_llr, float *d_llr_out, void *stream);
 
+/* ======================================================================
+ * Part 5: serving loop (continuous batching of code blocks on one GPU)
+ * ====================================================================== */
+
+/* The protocol is the reference pipeline's (LdpcPipeline::acquire / submit /
+ * try_recv / release, src/ldpc_pipeline.rs:499-576); the engine behind it is
+ * the GPU: admission by an HBM budget (fixed slots sized for 68 x 384),
+ * a dispatcher that launches everything submitted since the last batch as
+ * soon as one of `streams` launch lanes is free (so arrivals join the next
+ * batch while earlier ones run, and batches overlap on the device), and the
+ * work-queue decoder for mixed codes.  Each code block is decoded with
+ * decode_5g semantics (its n_filler) at `iterations`, exactly as alone. */
+typedef struct SyndromeServer SyndromeServer;
+
+typedef struct {
+    uint32_t slot;
+    uint64_t tag;
+    int32_t iterations;   /* passes used; -1 if its batch failed to launch */
+    const uint8_t *hard;  /* n hard decisions, valid until release */
+    size_t n;
+    double submit_ns;     /* steady-clock ns at submit */
+    double done_ns;       /* steady-clock ns when its batch was seen complete */
+} SyndromeServed;
+
+int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_budget_bytes, int32_t max_batch,
+                               int32_t streams, SyndromeServer **out_server);
+void syndrome_server_destroy(SyndromeServer *server);
+/* A slot for one code block of (base_graph, z): *out_llr points at its
+ * N = (bg ? 52 : 68) * z decoder-input floats (pinned host memory) to fill
+ * before submit.  SYNDROME_BUSY when the HBM budget is exhausted. */
+int32_t syndrome_server_acquire(SyndromeServer *server, int32_t base_graph, size_t z, size_t n_filler,
+                                float **out_llr, uint32_t *out_slot);
+int32_t syndrome_server_submit(SyndromeServer *server, uint32_t slot, uint64_t tag);
+/* One completed code block, or SYNDROME_BUSY when none is ready. */
+int32_t syndrome_server_try_recv(SyndromeServer *server, SyndromeServed *out);
+int32_t syndrome_server_release(SyndromeServer *server, uint32_t slot);
+int32_t syndrome_server_stats(const SyndromeServer *server, int64_t *out_batches, int64_t *out_decoded,
+                              int32_t *out_slots);
+
+/* Synthetic code-block traffic, as the reference's benchmarks build it
+ * (channel_sim.rs, QcLdpcEncoder, rate_match): for block i, K'_i = K_i -
+ * n_filler[i] seeded info bits (written at info + info_off[i] when info is
+ * not NULL), fillers zero, encoded for (bg[i], z[i]), rate matched to
+ * e_len[i] bits (rv 0, Qm 1), BPSK over AWGN at ebno_db[i] for code rate
+ * rate[i]; the e_len[i] LLRs are written at e_llr + e_off[i].  `threads`
+ * host threads (0 = all). */
+int32_t syndrome_traffic_blocks(const int32_t *bg, const int32_t *z, const int32_t *n_filler, const int32_t *e_len,
+                                const float *ebno_db, const float *rate, uint64_t seed, size_t count, float *e_llr,
+                                const int64_t *e_off, uint8_t *info, const int64_t *info_off, int32_t threads);
+
 #ifdef __cplusplus
 }
 #endif

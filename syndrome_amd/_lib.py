@@ -72,7 +72,21 @@ SIGNATURES = {
     "syndrome_ldpc_plan_tune": (I32, [P, P, P, P, P, SZ, C.c_float, C.c_int16, C.c_uint32, I32, C.POINTER(I32)]),
     "syndrome_ldpc_plan_set_dematch": (I32, [P, P, P, P, P]),
     "syndrome_ldpc_plan_dematch": (I32, [P, P, P, P]),
+    "syndrome_traffic_blocks": (I32, [P, P, P, P, P, P, C.c_uint64, SZ, P, P, P, P, I32]),
+    "syndrome_server_create": (I32, [C.c_float, SZ, SZ, I32, I32, C.POINTER(P)]),
+    "syndrome_server_destroy": (None, [P]),
+    "syndrome_server_acquire": (I32, [P, I32, SZ, SZ, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32)]),
+    "syndrome_server_submit": (I32, [P, C.c_uint32, C.c_uint64]),
+    "syndrome_server_try_recv": (I32, [P, P]),
+    "syndrome_server_release": (I32, [P, C.c_uint32]),
+    "syndrome_server_stats": (I32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(I32)]),
 }
+
+
+class Served(C.Structure):
+    _fields_ = [("slot", C.c_uint32), ("tag", C.c_uint64), ("iterations", C.c_int32),
+                ("hard", C.POINTER(C.c_uint8)), ("n", C.c_size_t), ("submit_ns", C.c_double),
+                ("done_ns", C.c_double)]
 
 _lib = None
 

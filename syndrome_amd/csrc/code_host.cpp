@@ -224,6 +224,31 @@ void Encoder::encode(const uint8_t *info, uint8_t *cw) const
 }
 
 // ---------------------------------------------------------------------------
+// rate matching
+// ---------------------------------------------------------------------------
+
+bool rate_match(const uint8_t *cw, size_t cw_len, uint8_t *e_out, size_t e_len, int rv, int qm, int bg, int z,
+                size_t n_filler)
+{
+    if (rv < 0 || rv > 3 || qm <= 0 || z <= 0 || (bg != 0 && bg != 1) || e_len % (size_t)qm) return false;
+    static const int k0_table[2][4] = {{0, 17, 33, 56}, {0, 13, 25, 43}};  // TS 38.212 Table 5.4.2.1-2
+    const size_t ncb = (size_t)(bg == 0 ? 66 : 50) * z, k = (size_t)(bg == 0 ? 22 : 10) * z, twoz = 2 * (size_t)z;
+    const size_t kp = n_filler > k ? 0 : k - n_filler;
+    const size_t fill_lo = kp > twoz ? kp - twoz : 0, fill_hi = k > twoz ? k - twoz : 0;
+    std::vector<uint8_t> sel(e_len);
+    size_t pos = (size_t)k0_table[bg][rv] * z, taken = 0, steps = 0;
+    while (taken < e_len && steps <= 2 * ncb) {
+        if (!(pos >= fill_lo && pos < fill_hi)) sel[taken++] = pos + twoz < cw_len ? cw[pos + twoz] : 0;
+        pos = pos + 1 == ncb ? 0 : pos + 1;
+        ++steps;
+    }
+    const size_t rows = e_len / (size_t)qm;  // write rows of Qm, read columns
+    for (size_t r = 0; r < rows; ++r)
+        for (int c = 0; c < qm; ++c) e_out[(size_t)c * rows + r] = sel[r * (size_t)qm + (size_t)c];
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // channel
 // ---------------------------------------------------------------------------
 

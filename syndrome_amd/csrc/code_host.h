@@ -44,6 +44,12 @@ private:
     std::vector<uint64_t> inv_;  // bits_ x words_ rows of the core inverse
 };
 
+// rate_match (src/rate_matching.rs:181-260): E bits from the circular buffer
+// of the full codeword (n_b*Z bits; the first 2Z are never sent) starting at
+// k0(rv), skipping fillers, then the Qm-column bit interleaver.  bg 0/1.
+bool rate_match(const uint8_t *codeword, size_t cw_len, uint8_t *e_out, size_t e_len, int rv, int qm, int bg, int z,
+                size_t n_filler);
+
 // AwgnChannel (src/channel_sim.rs:61-215): BPSK over AWGN, LLR = 2 r / sigma^2.
 void awgn_transmit(float ebno_db, float code_rate, uint64_t seed, const uint8_t *bits, size_t n, float *llr_out);
 

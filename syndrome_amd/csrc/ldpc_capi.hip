@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -481,6 +482,62 @@ int32_t syndrome_ldpc_encode_batch(const SyndromeLdpcDecoder *d, const uint8_t *
         for (int t = 0; t < nt; ++t)
             pool.emplace_back([&, t] {
                 for (size_t i = (size_t)t; i < count; i += (size_t)nt) md->enc->encode(info + i * k, cws + i * n);
+            });
+        for (auto &th : pool) th.join();
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_traffic_blocks(const int32_t *bg, const int32_t *z, const int32_t *n_filler, const int32_t *e_len,
+                                const float *ebno_db, const float *rate, uint64_t seed, size_t count, float *e_llr,
+                                const int64_t *e_off, uint8_t *info, const int64_t *info_off, int32_t threads)
+{
+    return guard([&]() -> int32_t {
+        if (count && (!bg || !z || !e_len || !ebno_db || !rate || !e_llr || !e_off)) return SYNDROME_NULL_POINTER;
+        if (info && !info_off) return SYNDROME_NULL_POINTER;
+        // one encoder per distinct code, built up front
+        std::map<std::pair<int, int>, std::unique_ptr<syn::Encoder>> enc;
+        std::map<std::pair<int, int>, SynCode> codes;
+        for (size_t i = 0; i < count; ++i) {
+            auto key = std::make_pair((int)bg[i], (int)z[i]);
+            if (codes.count(key)) continue;
+            SynCode c;
+            if (syn::build_code_3gpp(bg[i], z[i], &c)) return SYNDROME_INVALID_PARAM;
+            codes[key] = c;
+            enc[key].reset(new syn::Encoder(c));
+            if (!enc[key]->ok()) return SYNDROME_INVALID_PARAM;
+        }
+        for (size_t i = 0; i < count; ++i) {
+            const size_t k = syn::code_k(codes[std::make_pair((int)bg[i], (int)z[i])]);
+            if (e_len[i] < 0 || (n_filler && (n_filler[i] < 0 || (size_t)n_filler[i] > k))) return SYNDROME_INVALID_PARAM;
+        }
+        int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+        nt = (int)std::min<size_t>((size_t)nt, std::max<size_t>(count, 1));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                std::vector<uint8_t> inf, cw, e;
+                for (size_t i = (size_t)t; i < count; i += (size_t)nt) {
+                    auto key = std::make_pair((int)bg[i], (int)z[i]);
+                    const SynCode &c = codes[key];
+                    const size_t k = syn::code_k(c), n = syn::code_n(c);
+                    const size_t nf = n_filler ? (size_t)n_filler[i] : 0;
+                    inf.assign(k, 0);
+                    uint64_t s = (seed + 0x9E3779B97F4A7C15ULL * (i + 1)) | 1;
+                    for (size_t b = 0; b < k - nf; ++b) {  // xorshift64, a middle bit per info bit
+                        s ^= s << 13;
+                        s ^= s >> 7;
+                        s ^= s << 17;
+                        inf[b] = (uint8_t)((s >> 27) & 1);
+                    }
+                    if (info) memcpy(info + info_off[i], inf.data(), k - nf);
+                    cw.resize(n);
+                    enc[key]->encode(inf.data(), cw.data());
+                    e.resize((size_t)e_len[i]);
+                    syn::rate_match(cw.data(), n, e.data(), e.size(), 0, 1, bg[i], z[i], nf);
+                    syn::awgn_transmit(ebno_db[i], rate[i], seed ^ (0xD1B54A32D192ED03ULL * (i + 1)), e.data(),
+                                       e.size(), e_llr + e_off[i]);
+                }
             });
         for (auto &th : pool) th.join();
         return SYNDROME_OK;

@@ -1,0 +1,459 @@
+// Serving loop: continuous batching of code blocks on one GPU.
+//
+// The caller-facing protocol is the reference pipeline's
+// (src/ldpc_pipeline.rs:499-576): acquire a slot, write the code block's
+// decoder input into it, submit, poll completions, release.  Behind it:
+//
+//  * admission by HBM budget: the device arena (inputs, hard decisions,
+//    iteration counts) is cut into fixed slots sized for the largest code
+//    (68 x 384); `budget / slot_bytes` slots exist and acquire refuses with
+//    SYNDROME_BUSY when none is free;
+//  * a dispatcher thread forms a batch from everything submitted since the
+//    last one as soon as one of `streams` launch lanes is free, so requests
+//    that arrive while earlier batches run on the GPU are admitted into the
+//    next batch and overlap with them on the device;
+//  * a batch is three launches on its lane: gather (pinned host -> HBM),
+//    the persistent mixed-code decode, scatter (hard bits and iterations
+//    back to the pinned slots); completions are observed by event.
+// Each code block's result is exactly what decoding it alone gives.
+#include <hip/hip_runtime.h>
+
+#include <limits.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/syndrome_ldpc.h"
+#include "code_host.h"
+#include "ldpc_common.h"
+
+namespace syn {
+const void *persist_kernel_f32();
+int launch_reset(int *head, hipStream_t stream);
+}  // namespace syn
+
+namespace {
+
+constexpr int NMAX = 68 * 384;
+constexpr int VALID_Z[51] = {2,  3,  4,  5,  6,  7,  8,  9,  10,  11,  12,  13,  14,  15,  16,  18,  20,
+                             22, 24, 26, 28, 30, 32, 36, 40, 44,  48,  52,  56,  60,  64,  72,  80,  88,
+                             96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384};
+
+int z_index(int z)
+{
+    for (int i = 0; i < 51; ++i)
+        if (VALID_Z[i] == z) return i;
+    return -1;
+}
+
+// Copy each batch entry's decoder input from the mapped pinned slot to HBM.
+__global__ void __launch_bounds__(256) gather(const float *__restrict__ host, float *__restrict__ dev,
+                                              const int *__restrict__ slot, const int *__restrict__ n)
+{
+    const long long base = (long long)slot[blockIdx.x] * NMAX;
+    for (int i = threadIdx.x; i < n[blockIdx.x]; i += blockDim.x) dev[base + i] = host[base + i];
+}
+
+// Copy each batch entry's hard decisions and iteration count back.
+__global__ void __launch_bounds__(256) scatter(const uint8_t *__restrict__ dev_hard, uint8_t *__restrict__ host_hard,
+                                               const int *__restrict__ dev_it, int *__restrict__ host_it,
+                                               const int *__restrict__ slot, const int *__restrict__ n)
+{
+    const long long base = (long long)slot[blockIdx.x] * NMAX;
+    for (int i = threadIdx.x; i < n[blockIdx.x]; i += blockDim.x) host_hard[base + i] = dev_hard[base + i];
+    if (threadIdx.x == 0) host_it[slot[blockIdx.x]] = dev_it[blockIdx.x];
+}
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<int> slots;
+    // pinned descriptor block: items | ids | offs | nfill | slot | n
+    void *h_desc = nullptr, *d_desc = nullptr;
+    size_t desc_bytes = 0;
+    int *d_head = nullptr, *d_iters = nullptr;
+};
+
+double now_ns()
+{
+    return (double)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+struct SyndromeServer {
+    int device = 0;
+    float beta = 0.5f;
+    int iterations = 8;
+    int max_batch = 256;
+    int slots = 0;
+    int lds_limit = 0, cus = 0;
+    // arenas
+    float *h_llr = nullptr, *d_llr = nullptr, *h_llr_dev = nullptr;
+    uint8_t *h_hard = nullptr, *d_hard = nullptr, *h_hard_dev = nullptr;
+    int *h_it = nullptr, *h_it_dev = nullptr;
+    SynCode *d_codes = nullptr;  // 102 codes: bg * 51 + z index
+    std::vector<SynCode> codes;
+    // slot state
+    struct Slot {
+        int code = -1, n = 0, nfill = 0;
+        uint64_t tag = 0;
+        double t_submit = 0, t_done = 0;
+        int state = 0;  // 0 free, 1 acquired, 2 submitted, 3 on gpu, 4 done, 5 delivered
+    };
+    std::vector<Slot> st;
+    std::vector<int> free_list;
+    std::deque<int> submitted, done;
+    std::vector<Lane> lanes;
+    std::mutex mu;
+    std::atomic<bool> stop{false};
+    std::thread worker;
+    std::atomic<long long> batches{0}, decoded{0};
+    int error = 0;
+
+    int32_t launch_batch(Lane &ln, std::vector<int> &batch);
+    void run();
+};
+
+namespace {
+
+template <class F>
+int32_t guard(F &&body)
+{
+    try {
+        return body();
+    } catch (...) {
+        return SYNDROME_PANIC;
+    }
+}
+
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceScope()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+size_t group_bytes(const SynCode &c, int cws)
+{
+    const size_t n = syn::code_n(c);
+    return (size_t)(((c.z * 4) + 15) & ~15) + ((cws * n * 4 + 3) & ~(size_t)3) + 12 * (size_t)cws;
+}
+
+}  // namespace
+
+// Build the batch's work items (grouped by code, as many codewords per item
+// as fit 512 lanes and the LDS, longest first) and issue its three launches.
+int32_t SyndromeServer::launch_batch(Lane &ln, std::vector<int> &batch)
+{
+    const int nb = (int)batch.size();
+    std::vector<std::vector<int>> by_code(codes.size());
+    for (int j = 0; j < nb; ++j) by_code[st[batch[j]].code].push_back(j);
+    std::vector<SynItem> items;
+    std::vector<int> ids;
+    size_t lds = 0;
+    std::vector<std::pair<long long, SynItem>> tmp;
+    for (size_t c = 0; c < codes.size(); ++c) {
+        if (by_code[c].empty()) continue;
+        int k = std::max(1, 512 / codes[c].z);
+        while (k > 1 && group_bytes(codes[c], k) > (size_t)lds_limit) --k;
+        lds = std::max(lds, group_bytes(codes[c], k));
+        for (size_t j = 0; j < by_code[c].size(); j += (size_t)k) {
+            const int cnt = (int)std::min(by_code[c].size() - j, (size_t)k);
+            SynItem it{(int)c, codes[c].kind, (int)ids.size(), cnt};
+            for (int q = 0; q < cnt; ++q) ids.push_back(by_code[c][j + q]);
+            tmp.push_back({(long long)codes[c].n_edges * codes[c].z * cnt, it});
+        }
+    }
+    std::stable_sort(tmp.begin(), tmp.end(), [](auto &a, auto &b) { return a.first > b.first; });
+    for (auto &x : tmp) items.push_back(x.second);
+    const int ni = (int)items.size();
+    // descriptor layout (pinned, one H2D copy)
+    const size_t o_items = 0, o_ids = o_items + sizeof(SynItem) * ni, o_offs = (o_ids + 4 * nb + 7) & ~(size_t)7;
+    const size_t o_nf = o_offs + 8 * (size_t)nb, o_slot = o_nf + 4 * (size_t)nb, o_n = o_slot + 4 * (size_t)nb;
+    const size_t bytes = o_n + 4 * (size_t)nb;
+    if (bytes > ln.desc_bytes) return SYNDROME_INVALID_PARAM;
+    char *h = (char *)ln.h_desc;
+    memcpy(h + o_items, items.data(), sizeof(SynItem) * ni);
+    memcpy(h + o_ids, ids.data(), 4 * (size_t)nb);
+    for (int j = 0; j < nb; ++j) {
+        const Slot &s = st[batch[j]];
+        ((long long *)(h + o_offs))[j] = (long long)batch[j] * NMAX;
+        ((int *)(h + o_nf))[j] = s.nfill;
+        ((int *)(h + o_slot))[j] = batch[j];
+        ((int *)(h + o_n))[j] = s.n;
+    }
+    char *d = (char *)ln.d_desc;
+    if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ln.stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    hipLaunchKernelGGL(gather, dim3(nb), dim3(256), 0, ln.stream, h_llr_dev, d_llr, (const int *)(d + o_slot),
+                       (const int *)(d + o_n));
+    if (syn::launch_reset(ln.d_head, ln.stream)) return SYNDROME_DEVICE_ERROR;
+    SynLaunch a{};
+    a.in = d_llr;
+    a.post = nullptr;
+    a.hard = d_hard;
+    a.iters = ln.d_iters;
+    a.batch = nb;
+    a.iterations = iterations;
+    a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD;
+    a.beta = beta;
+    const void *fn = syn::persist_kernel_f32();
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    const SynCode *cp = d_codes;
+    const SynItem *ip = (const SynItem *)(d + o_items);
+    int n_items = ni;
+    int *head = ln.d_head;
+    const int *idp = (const int *)(d + o_ids);
+    const long long *op = (const long long *)(d + o_offs);
+    const int *nfp = (const int *)(d + o_nf);
+    void *args[] = {&cp, &ip, &n_items, &head, &idp, &op, &nfp, &a};
+    const int grid = std::max(1, std::min(ni, cus));
+    if (hipLaunchKernel(fn, dim3(grid), dim3(512), args, lds, ln.stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    hipLaunchKernelGGL(scatter, dim3(nb), dim3(256), 0, ln.stream, d_hard, h_hard_dev, ln.d_iters, h_it_dev,
+                       (const int *)(d + o_slot), (const int *)(d + o_n));
+    if (hipEventRecord(ln.done, ln.stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    return SYNDROME_OK;
+}
+
+void SyndromeServer::run()
+{
+    DeviceScope ds(device);
+    std::vector<int> batch;
+    while (!stop.load()) {
+        bool progressed = false;
+        for (Lane &ln : lanes) {
+            if (ln.busy && hipEventQuery(ln.done) == hipSuccess) {
+                const double t = now_ns();
+                std::lock_guard<std::mutex> lk(mu);
+                for (int s : ln.slots) {
+                    st[s].state = 4;
+                    st[s].t_done = t;
+                    done.push_back(s);
+                }
+                decoded += (long long)ln.slots.size();
+                ln.slots.clear();
+                ln.busy = false;
+                progressed = true;
+            }
+        }
+        for (Lane &ln : lanes) {
+            if (ln.busy) continue;
+            batch.clear();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                while (!submitted.empty() && (int)batch.size() < max_batch) {
+                    const int s = submitted.front();
+                    submitted.pop_front();
+                    st[s].state = 3;
+                    batch.push_back(s);
+                }
+            }
+            if (batch.empty()) break;
+            if (int32_t rc = launch_batch(ln, batch)) {
+                std::lock_guard<std::mutex> lk(mu);
+                error = rc;
+                for (int s : batch) {  // report the failure per code block
+                    st[s].state = 4;
+                    st[s].t_done = now_ns();
+                    h_it[s] = -1;
+                    done.push_back(s);
+                }
+                continue;
+            }
+            ln.slots = batch;
+            ln.busy = true;
+            batches++;
+            progressed = true;
+        }
+        if (!progressed) std::this_thread::yield();
+    }
+    for (Lane &ln : lanes)
+        if (ln.busy) (void)hipEventSynchronize(ln.done);
+}
+
+extern "C" {
+
+int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_budget_bytes, int32_t max_batch,
+                               int32_t streams, SyndromeServer **out)
+{
+    return guard([&]() -> int32_t {
+        if (!out) return SYNDROME_NULL_POINTER;
+        *out = nullptr;
+        const size_t slot_bytes = (size_t)NMAX * 5 + 64;
+        if (hbm_budget_bytes < slot_bytes || streams < 1 || streams > 8 || max_batch < 1 || max_batch > 65536)
+            return SYNDROME_INVALID_PARAM;
+        std::unique_ptr<SyndromeServer> s(new SyndromeServer());
+        s->beta = offset_beta;
+        s->iterations = (int)std::min(iterations, (size_t)INT_MAX);
+        s->max_batch = max_batch;
+        s->slots = (int)std::min(hbm_budget_bytes / slot_bytes, (size_t)1 << 20);
+        if (hipGetDevice(&s->device) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        if (hipDeviceGetAttribute(&s->lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) != hipSuccess ||
+            hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess)
+            return SYNDROME_DEVICE_ERROR;
+        for (int b = 0; b < 2; ++b)
+            for (int zi = 0; zi < 51; ++zi) {
+                SynCode c;
+                if (syn::build_code_3gpp(b, VALID_Z[zi], &c)) return SYNDROME_INVALID_PARAM;
+                s->codes.push_back(c);
+            }
+        const size_t n = (size_t)s->slots * NMAX;
+        if (hipMalloc((void **)&s->d_codes, sizeof(SynCode) * s->codes.size()) != hipSuccess ||
+            hipMemcpy(s->d_codes, s->codes.data(), sizeof(SynCode) * s->codes.size(), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMalloc((void **)&s->d_llr, n * 4) != hipSuccess || hipMalloc((void **)&s->d_hard, n) != hipSuccess ||
+            hipHostMalloc((void **)&s->h_llr, n * 4, hipHostMallocMapped) != hipSuccess ||
+            hipHostMalloc((void **)&s->h_hard, n, hipHostMallocMapped) != hipSuccess ||
+            hipHostMalloc((void **)&s->h_it, 4 * (size_t)s->slots, hipHostMallocMapped) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&s->h_llr_dev, s->h_llr, 0) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&s->h_hard_dev, s->h_hard, 0) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&s->h_it_dev, s->h_it, 0) != hipSuccess) {
+            syndrome_server_destroy(s.release());
+            return SYNDROME_DEVICE_ERROR;
+        }
+        s->st.resize(s->slots);
+        for (int i = s->slots - 1; i >= 0; --i) s->free_list.push_back(i);
+        s->lanes.resize(streams);
+        for (Lane &ln : s->lanes) {
+            ln.desc_bytes = (sizeof(SynItem) + 4 + 8 + 4 + 4 + 4) * (size_t)max_batch + 64;
+            if (hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) != hipSuccess ||
+                hipHostMalloc(&ln.h_desc, ln.desc_bytes, 0) != hipSuccess ||
+                hipMalloc(&ln.d_desc, ln.desc_bytes) != hipSuccess || hipMalloc((void **)&ln.d_head, 64) != hipSuccess ||
+                hipMalloc((void **)&ln.d_iters, 4 * (size_t)max_batch) != hipSuccess) {
+                syndrome_server_destroy(s.release());
+                return SYNDROME_DEVICE_ERROR;
+            }
+        }
+        SyndromeServer *raw = s.get();
+        s->worker = std::thread([raw] { raw->run(); });
+        *out = s.release();
+        return SYNDROME_OK;
+    });
+}
+
+void syndrome_server_destroy(SyndromeServer *s)
+{
+    if (!s) return;
+    try {
+        s->stop = true;
+        if (s->worker.joinable()) s->worker.join();
+        DeviceScope ds(s->device);
+        for (Lane &ln : s->lanes) {
+            if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+            if (ln.done) (void)hipEventDestroy(ln.done);
+            if (ln.stream) (void)hipStreamDestroy(ln.stream);
+            if (ln.h_desc) (void)hipHostFree(ln.h_desc);
+            for (void *p : {ln.d_desc, (void *)ln.d_head, (void *)ln.d_iters})
+                if (p) (void)hipFree(p);
+        }
+        for (void *p : {(void *)s->d_llr, (void *)s->d_hard, (void *)s->d_codes})
+            if (p) (void)hipFree(p);
+        for (void *p : {(void *)s->h_llr, (void *)s->h_hard, (void *)s->h_it})
+            if (p) (void)hipHostFree(p);
+    } catch (...) {
+    }
+    delete s;
+}
+
+int32_t syndrome_server_acquire(SyndromeServer *s, int32_t base_graph, size_t z, size_t n_filler, float **out_llr,
+                                uint32_t *out_slot)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !out_llr || !out_slot) return SYNDROME_NULL_POINTER;
+        const int zi = z <= 384 ? z_index((int)z) : -1;
+        if ((base_graph != 0 && base_graph != 1) || zi < 0) return SYNDROME_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->free_list.empty()) return SYNDROME_BUSY;
+        const int slot = s->free_list.back();
+        s->free_list.pop_back();
+        SyndromeServer::Slot &sl = s->st[slot];
+        sl.code = base_graph * 51 + zi;
+        sl.n = (base_graph == 0 ? 68 : 52) * (int)z;
+        sl.nfill = (int)std::min(n_filler, (size_t)INT_MAX);
+        sl.state = 1;
+        *out_llr = s->h_llr + (size_t)slot * NMAX;
+        *out_slot = (uint32_t)slot;
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_submit(SyndromeServer *s, uint32_t slot, uint64_t tag)
+{
+    return guard([&]() -> int32_t {
+        if (!s) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (slot >= (uint32_t)s->slots || s->st[slot].state != 1) return SYNDROME_INVALID_PARAM;
+        SyndromeServer::Slot &sl = s->st[slot];
+        sl.tag = tag;
+        sl.t_submit = now_ns();
+        sl.state = 2;
+        s->submitted.push_back((int)slot);
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_try_recv(SyndromeServer *s, SyndromeServed *out)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !out) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->done.empty()) return SYNDROME_BUSY;
+        const int slot = s->done.front();
+        s->done.pop_front();
+        SyndromeServer::Slot &sl = s->st[slot];
+        sl.state = 5;
+        out->slot = (uint32_t)slot;
+        out->tag = sl.tag;
+        out->iterations = s->h_it[slot];
+        out->hard = s->h_hard + (size_t)slot * NMAX;
+        out->n = (size_t)sl.n;
+        out->submit_ns = sl.t_submit;
+        out->done_ns = sl.t_done;
+        return out->iterations < 0 ? SYNDROME_DEVICE_ERROR : SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_release(SyndromeServer *s, uint32_t slot)
+{
+    return guard([&]() -> int32_t {
+        if (!s) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (slot >= (uint32_t)s->slots || (s->st[slot].state != 5 && s->st[slot].state != 1))
+            return SYNDROME_INVALID_PARAM;
+        s->st[slot].state = 0;
+        s->free_list.push_back((int)slot);
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_stats(const SyndromeServer *s, int64_t *out_batches, int64_t *out_decoded, int32_t *out_slots)
+{
+    return guard([&]() -> int32_t {
+        if (!s) return SYNDROME_NULL_POINTER;
+        if (out_batches) *out_batches = s->batches.load();
+        if (out_decoded) *out_decoded = s->decoded.load();
+        if (out_slots) *out_slots = s->slots;
+        return SYNDROME_OK;
+    });
+}
+
+}  // extern "C"

@@ -317,3 +317,77 @@ class DecodePlan:
         import torch
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_plan_dematch(self._h, d_e_llr.data_ptr(), d_llr_out.data_ptr(), s), "plan_dematch")
+
+
+def traffic_blocks(bg, z, n_filler, e_len, ebno_db, rate, seed, threads=0, want_info=True):
+    """Synthetic code-block traffic (include/syndrome_ldpc.h
+    syndrome_traffic_blocks).  Returns (e_llr, e_offsets, info, info_offsets)."""
+    bg = np.ascontiguousarray(bg, np.int32)
+    z = np.ascontiguousarray(z, np.int32)
+    nf = np.ascontiguousarray(n_filler, np.int32)
+    el = np.ascontiguousarray(e_len, np.int32)
+    eb = np.ascontiguousarray(ebno_db, np.float32)
+    rt = np.ascontiguousarray(rate, np.float32)
+    e_off = np.concatenate([[0], np.cumsum(el.astype(np.int64))[:-1]]).astype(np.int64)
+    e_llr = np.zeros(int(el.astype(np.int64).sum()), np.float32)
+    kp = np.where(bg == 0, 22, 10).astype(np.int64) * z - nf
+    i_off = np.concatenate([[0], np.cumsum(kp)[:-1]]).astype(np.int64)
+    info = np.zeros(int(kp.sum()), np.uint8) if want_info else None
+    check(lib().syndrome_traffic_blocks(_ptr(bg), _ptr(z), _ptr(nf), _ptr(el), _ptr(eb), _ptr(rt), seed, len(bg),
+                                        _ptr(e_llr), _ptr(e_off), _ptr(info) if want_info else None,
+                                        _ptr(i_off) if want_info else None, threads), "traffic_blocks")
+    return e_llr, e_off, info, i_off
+
+
+class Server:
+    """Serving loop (include/syndrome_ldpc.h Part 5): acquire / submit /
+    try_recv / release, as LdpcPipeline (src/ldpc_pipeline.rs:499-576)."""
+
+    def __init__(self, offset_beta, iterations, hbm_budget_bytes=256 << 20, max_batch=256, streams=3):
+        out = C.c_void_p()
+        check(lib().syndrome_server_create(offset_beta, iterations, hbm_budget_bytes, max_batch, streams,
+                                           C.byref(out)), "server_create")
+        self._h = out
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().syndrome_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def acquire(self, bg, z, n_filler=0):
+        """Returns (slot, numpy view of the slot's N decoder-input floats) or None if busy."""
+        p = C.POINTER(C.c_float)()
+        slot = C.c_uint32()
+        rc = lib().syndrome_server_acquire(self._h, bg, z, n_filler, C.byref(p), C.byref(slot))
+        if rc == _lib.BUSY:
+            return None
+        check(rc, "server_acquire")
+        n = (68 if bg == 0 else 52) * z
+        return slot.value, np.ctypeslib.as_array(p, shape=(n,))
+
+    def submit(self, slot, tag):
+        check(lib().syndrome_server_submit(self._h, slot, tag), "server_submit")
+
+    def try_recv(self):
+        """Returns (slot, tag, iterations, hard copy, submit_ns, done_ns) or None."""
+        r = _lib.Served()
+        rc = lib().syndrome_server_try_recv(self._h, C.byref(r))
+        if rc == _lib.BUSY:
+            return None
+        check(rc, "server_try_recv")
+        hard = np.ctypeslib.as_array(r.hard, shape=(r.n,)).copy()
+        return r.slot, r.tag, r.iterations, hard, r.submit_ns, r.done_ns
+
+    def release(self, slot):
+        check(lib().syndrome_server_release(self._h, slot), "server_release")
+
+    def stats(self):
+        b, d, s = C.c_int64(), C.c_int64(), C.c_int32()
+        check(lib().syndrome_server_stats(self._h, C.byref(b), C.byref(d), C.byref(s)), "server_stats")
+        return {"batches": b.value, "decoded": d.value, "slots": s.value}

@@ -100,3 +100,28 @@ def test_product_has_no_oracle_dependency():
             if f.endswith((".py", ".hip", ".cpp", ".h", ".inc")):
                 text = open(os.path.join(d, f)).read()
                 assert not re.search(r"\boracle\b|liborc|loms_oracle", text), f
+
+
+def test_traffic_generator_matches_oracle_pipeline(oracle):
+    """syndrome_traffic_blocks = encode -> rate_match (rv 0, Qm 1) -> AWGN,
+    each stage as the oracle restates the reference's."""
+    from syndrome_amd import traffic_blocks
+    cases = [(0, 384, 0, 25344, 1.5), (1, 208, 0, 10400, 0.7), (0, 52, 17, 3000, 2.0), (1, 13, 5, 900, 3.0),
+             (1, 96, 40, 200, 1.0)]
+    bg = [c[0] for c in cases]
+    z = [c[1] for c in cases]
+    nf = [c[2] for c in cases]
+    el = [c[3] for c in cases]
+    eb = [c[4] for c in cases]
+    rate = [((22 if b == 0 else 10) * zz - f) / e for b, zz, f, e, _ in cases]
+    seed = 1234
+    e_llr, e_off, info, i_off = traffic_blocks(bg, z, nf, el, eb, rate, seed, threads=3)
+    for i, (b, zz, f, e, ebno) in enumerate(cases):
+        code = oracle.code_3gpp(b, zz)
+        kp = code.k - f
+        inf = np.concatenate([info[i_off[i]:i_off[i] + kp], np.zeros(f, np.uint8)])
+        cw = oracle.encode(code, inf)
+        bits = oracle.rate_match(cw, e, 0, 1, b, zz, f)
+        s = (seed ^ ((0xD1B54A32D192ED03 * (i + 1)) & 0xFFFFFFFFFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
+        ref = oracle.awgn(np.float32(ebno), np.float32(rate[i]), s, bits)
+        assert np.array_equal(f32_bits(e_llr[e_off[i]:e_off[i] + e]), f32_bits(ref)), i

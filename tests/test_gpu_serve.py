@@ -1,0 +1,122 @@
+"""GPU tests of the serving loop: continuous batching over mixed code blocks.
+
+* Every code block served in the mix (batches of varying composition,
+  overlapping on several launch lanes) equals the oracle's decode of it
+  alone and the per-config batch decode of it alone: hard decisions and
+  iterations.
+* Admission by HBM budget: with the slots exhausted, acquire refuses
+  (Busy) until a slot is released.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from tests.helpers import say
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+MIX = [(0, 384), (1, 208), (0, 2), (1, 3), (0, 44), (1, 52), (0, 104), (1, 384), (0, 15), (1, 120), (0, 256),
+       (1, 7), (0, 352), (1, 26), (0, 96), (1, 160)]
+
+
+def _blocks(oracle, count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        b, z = MIX[int(rng.integers(0, len(MIX)))]
+        code = oracle.code_3gpp(b, z)
+        nf = int(rng.integers(0, min(48, code.k // 4) + 1))
+        info = oracle.xorshift_bits(seed * 7919 + i, code.k)
+        info[code.k - nf:] = 0
+        cw = oracle.encode(code, info)
+        llr = oracle.awgn(float(rng.uniform(0.0, 3.0)), code.k / code.n, seed + 31 * i, cw)
+        out.append((b, z, nf, llr))
+    return out
+
+
+def test_server_results_equal_alone_and_in_the_mix(oracle):
+    import syndrome_amd
+    from syndrome_amd import QcLdpcDecoder, Server, _lib as L
+    assert torch.cuda.is_available()
+    syndrome_amd.lib()
+    say("[serve] 160 mixed code blocks through 3 lanes, bursts of varying size")
+    blocks = _blocks(oracle, 160, 11)
+    srv = Server(0.5, 8, hbm_budget_bytes=96 << 20, max_batch=48, streams=3)
+    rng = np.random.default_rng(2)
+    results = {}
+    i = 0
+    t_end = time.time() + 60
+    while len(results) < len(blocks) and time.time() < t_end:
+        burst = int(rng.integers(1, 12))
+        for _ in range(burst):
+            if i >= len(blocks):
+                break
+            b, z, nf, llr = blocks[i]
+            got = srv.acquire(b, z, nf)
+            if got is None:
+                break
+            slot, buf = got
+            buf[:] = llr
+            srv.submit(slot, i)
+            i += 1
+        time.sleep(float(rng.uniform(0, 0.002)))
+        while True:
+            r = srv.try_recv()
+            if r is None:
+                break
+            slot, tag, its, hard, t0, t1 = r
+            results[tag] = (its, hard)
+            srv.release(slot)
+    assert len(results) == len(blocks), f"only {len(results)} of {len(blocks)} completed"
+    st = srv.stats()
+    say("  stats", st)
+    assert st["batches"] > 3  # really batched in several, varying groups
+    for tag, (b, z, nf, llr) in enumerate(blocks):
+        code = oracle.code_3gpp(b, z)
+        ri, _, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, llr, nf), 8)
+        its, hard = results[tag]
+        assert its == ri, f"block {tag} (BG{b + 1} Z={z}): iterations {its} != {ri}"
+        assert np.array_equal(hard, rh), f"block {tag}: hard decisions differ"
+    # and alone on the GPU
+    for tag in (0, 1, 2, 3):
+        b, z, nf, llr = blocks[tag]
+        dec = QcLdpcDecoder.with_lifting_size(b, z, 0.5)
+        x = torch.from_numpy(llr).cuda()[None, :]
+        h = torch.empty(x.shape, dtype=torch.uint8, device="cuda")
+        it = torch.empty(1, dtype=torch.int32, device="cuda")
+        dec.decode_batch_f32(x, None, h, it, 8, n_filler=nf, flags=L.INIT_5G | L.WRITE_HARD)
+        torch.cuda.synchronize()
+        assert int(it.item()) == results[tag][0]
+        assert np.array_equal(h.cpu().numpy()[0], results[tag][1])
+    srv.close()
+
+
+def test_server_admission_by_budget():
+    import syndrome_amd
+    from syndrome_amd import Server
+    syndrome_amd.lib()
+    say("[serve] admission: 3-slot budget")
+    slot_bytes = 68 * 384 * 5 + 64
+    srv = Server(0.5, 4, hbm_budget_bytes=3 * slot_bytes, max_batch=8, streams=1)
+    assert srv.stats()["slots"] == 3
+    held = [srv.acquire(1, 16) for _ in range(3)]
+    assert all(h is not None for h in held)
+    assert srv.acquire(1, 16) is None  # Busy: budget exhausted
+    srv.release(held[0][0])
+    again = srv.acquire(1, 16)
+    assert again is not None
+    # a strong all-zero block decodes to zeros in one iteration
+    slot, buf = again
+    buf[:] = 8.0
+    srv.submit(slot, 99)
+    t_end = time.time() + 20
+    r = None
+    while r is None and time.time() < t_end:
+        r = srv.try_recv()
+    assert r is not None and r[1] == 99 and r[2] == 1 and not r[3].any()
+    srv.release(slot)
+    for h in held[1:]:
+        srv.release(h[0])
+    srv.close()
