@@ -168,6 +168,231 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
     print("\n".join(lines))
 
 
+
+
+# ---------------------------------------------------------------------------
+# the other configurations, reported beside the headline number
+# ---------------------------------------------------------------------------
+
+RATES = (1 / 3, 1 / 2, 2 / 3, 3 / 4, 5 / 6, 8 / 9)
+# Approximate Eb/N0 (dB) of 10% BLER for BPSK/AWGN, 8 LOMS iterations, by
+# code rate (uncalibrated estimates; the workload draws +-1 dB around them).
+EBNO_10PCT = {RATES[0]: 1.2, RATES[1]: 1.7, RATES[2]: 2.5, RATES[3]: 3.0, RATES[4]: 3.6, RATES[5]: 4.3}
+VALID_Z = (2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40, 44, 48, 52,
+           56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384)
+
+
+def _timed(fn, reps, torch):
+    fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps
+
+
+def bench_int8(torch, args):
+    """5G NR BG2 Z=208 rate 1/5 (K=2080, 50Z sent), Eb/N0 ~ U[0,3] dB per
+    frame, quantize_llr_i16 at scale 8 fused into the load, int8 min-sum,
+    max 8 iterations, 16,384 frames per launch."""
+    import syndrome_amd
+    from syndrome_amd import QcLdpcDecoder, QuantParams, traffic_blocks, _lib as L
+    B, z = 16384, 208
+    dec = QcLdpcDecoder.with_lifting_size(syndrome_amd.BG2, z, args.beta)
+    n, k = dec.variable_node_count(), dec.info_bit_count_5g()
+    rng = np.random.default_rng(args.seed + 3)
+    e_llr, e_off, info, _ = traffic_blocks(np.ones(B), np.full(B, z), np.zeros(B), np.full(B, 50 * z),
+                                           rng.uniform(0, 3, B), np.full(B, k / (50 * z)), args.seed + 3)
+    llr = np.zeros((B, n), np.float32)
+    llr[:, 2 * z:] = e_llr.reshape(B, 50 * z)  # E = Ncb at rv 0, no filler: identity placement
+    d_in = torch.from_numpy(llr).cuda()
+    d_post = torch.empty((B, n), dtype=torch.int16, device="cuda")
+    d_hard = torch.empty((B, n), dtype=torch.uint8, device="cuda")
+    d_it = torch.empty(B, dtype=torch.int32, device="cuda")
+    q = QuantParams(8.0, 32767)
+    flags = L.INIT_5G | L.WRITE_POST | L.WRITE_HARD | L.QUANTIZE
+    fast = lambda: dec.decode_batch_i8(d_in, d_post, d_hard, d_it, args.iters, flags=flags, quant=q)
+    t = _timed(fast, 10, torch)
+    hard = d_hard.cpu().numpy()
+    bler = float(((hard[:, :k] != info.reshape(B, k)).any(axis=1)).mean())
+    its = d_it.cpu().numpy()
+    d_edge = torch.empty((B, dec.required_edge_buffer()), dtype=torch.int8, device="cuda")
+    plain = lambda: dec.decode_batch_i8(d_in, d_post, d_hard, d_it, args.iters, flags=flags | L.PLAIN,
+                                        d_edge=d_edge, quant=q)
+    tp = _timed(plain, 2, torch)
+    return {"workload": "BG2 Z=208 rate 1/5, Eb/N0 U[0,3] dB, int8 (scale 8, fused quantizer), max 8 it, "
+                        "batch 16,384", "value": round(B * k / t / 1e9, 4), "unit": "Gbit/s",
+            "ms_per_launch": round(t * 1e3, 4), "plain_value": round(B * k / tp / 1e9, 4),
+            "mean_iterations": round(float(its.mean()), 3), "bler": bler}
+
+
+def bench_uneven(torch, args):
+    """32,768 code blocks: BG1/BG2 at p=1/2, Z uniform over the 51 lifting
+    sizes, rate uniform over {1/3,...,8/9} via rate matching (rv 0, Qm 1),
+    Eb/N0 at the rate's approximate 10% BLER point +-1 dB; de-matched on
+    device and decoded in one persistent launch (f32 and int8)."""
+    from syndrome_amd import DecodePlan, QcLdpcDecoder, QuantParams, traffic_blocks, _lib as L
+    B = 32768
+    rng = np.random.default_rng(args.seed + 4)
+    bg = rng.integers(0, 2, B)
+    z = np.array(VALID_Z)[rng.integers(0, 51, B)]
+    r = np.array(RATES)[rng.integers(0, 6, B)]
+    k = np.where(bg == 0, 22, 10) * z
+    e_len = np.maximum(np.round(k / r).astype(np.int64), 1)
+    ebno = np.array([EBNO_10PCT[x] for x in r]) + rng.uniform(-1, 1, B)
+    e_llr, e_off, info, i_off = traffic_blocks(bg, z, np.zeros(B), e_len, ebno, k / e_len, args.seed + 4)
+    n = np.where(bg == 0, 68, 52) * z
+    offs = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+    plan = DecodePlan(bg, z, np.zeros(B), offs, args.beta)
+    plan.set_dematch(e_off, e_len)
+    d_e = torch.from_numpy(e_llr).cuda()
+    d_llr = torch.empty(int(n.sum()), dtype=torch.float32, device="cuda")
+    d_post = torch.empty_like(d_llr)
+    d_hard = torch.empty(int(n.sum()), dtype=torch.uint8, device="cuda")
+    d_it = torch.empty(B, dtype=torch.int32, device="cuda")
+    flags = L.INIT_5G | L.WRITE_POST | L.WRITE_HARD
+    plan.dematch(d_e, d_llr)
+    block = plan.tune(d_llr, d_post, d_hard, d_it, args.iters, flags)
+    full = lambda: (plan.dematch(d_e, d_llr), plan.run(d_llr, d_post, d_hard, d_it, args.iters, flags))
+    t = _timed(full, 5, torch)
+    t_dec = _timed(lambda: plan.run(d_llr, d_post, d_hard, d_it, args.iters, flags), 5, torch)
+    hard = d_hard.cpu().numpy()
+    its = d_it.cpu().numpy()
+    errs = sum(bool((hard[offs[i]:offs[i] + k[i]] != info[i_off[i]:i_off[i] + k[i]]).any()) for i in range(B))
+    # per-config launches: the same blocks grouped by (BG, Z), one launch each
+    groups = {}
+    for i in range(B):
+        groups.setdefault((int(bg[i]), int(z[i])), []).append(i)
+    per = []
+    for (b, zz), ids in groups.items():
+        dec = QcLdpcDecoder.with_lifting_size(b, zz, args.beta)
+        nn = (68 if b == 0 else 52) * zz
+        idx = torch.from_numpy(np.concatenate([np.arange(offs[i], offs[i] + nn) for i in ids])).cuda()
+        x = d_llr[idx].view(len(ids), nn).contiguous()
+        per.append((dec, x, torch.empty_like(x), torch.empty(x.shape, dtype=torch.uint8, device="cuda"),
+                    torch.empty(len(ids), dtype=torch.int32, device="cuda")))
+    def per_config():
+        for dec, x, p, h, it in per:
+            dec.decode_batch_f32(x, p, h, it, args.iters, flags=flags)
+    t_per = _timed(per_config, 3, torch)
+    # int8 variant on the same blocks (quantizer fused)
+    plan8 = DecodePlan(bg, z, np.zeros(B), offs, args.beta, int8=True)
+    d_post8 = torch.empty(int(n.sum()), dtype=torch.int16, device="cuda")
+    q = QuantParams(8.0, 32767)
+    f8 = flags | L.QUANTIZE
+    plan8.tune(d_llr, d_post8, d_hard, d_it, args.iters, f8, quant=q)
+    t8 = _timed(lambda: plan8.run(d_llr, d_post8, d_hard, d_it, args.iters, f8, quant=q), 5, torch)
+    bits = float(k.sum())
+    return {"workload": "32,768 code blocks, BG1/BG2, Z over the 51 sizes, rates 1/3..8/9 via rate matching, "
+                        "Eb/N0 at approx. 10% BLER point +-1 dB, max 8 it",
+            "value": round(bits / t / 1e9, 4), "unit": "Gbit/s", "ms_dematch_plus_decode": round(t * 1e3, 3),
+            "decode_only_value": round(bits / t_dec / 1e9, 4), "per_config_launches_value": round(bits / t_per / 1e9, 4),
+            "per_config_launches": len(groups), "int8_value": round(bits / t8 / 1e9, 4), "tuned_block": block,
+            "shape": plan.shape(), "mean_iterations": round(float(its.mean()), 3), "bler": round(errs / B, 4)}
+
+
+def bench_serving(torch, args, peak_ref_gbps):
+    """Poisson arrivals of transport blocks (TBS log-uniform in [1e3, 5e5]
+    bits, MCS rate in {1/3,1/2,2/3,3/4,5/6}, Qm 1) segmented into code blocks
+    (compute_segmentation), BPSK/AWGN 2.0 dB, rate de-matched; served with
+    continuous batching; offered load swept against the saturated peak."""
+    from syndrome_amd import DecodePlan, HostPool, Server, serve_loadgen, traffic_blocks
+    from syndrome_amd.transport import compute_segmentation
+    rng = np.random.default_rng(args.seed + 5)
+    n_tb = 96
+    cb_bg, cb_z, cb_nf, cb_e, cb_kp, tb_first, tb_num, tbs = [], [], [], [], [], [], [], []
+    for _ in range(n_tb):
+        a = int(np.exp(rng.uniform(np.log(1000), np.log(500000))))
+        r = float((1 / 3, 1 / 2, 2 / 3, 3 / 4, 5 / 6)[rng.integers(0, 5)])
+        sg = compute_segmentation(a, r)
+        tb_first.append(len(cb_bg))
+        tb_num.append(sg.c)
+        tbs.append(a)
+        e = int(round((a + 24) / r / sg.c))
+        for _ in range(sg.c):
+            cb_bg.append(sg.bg)
+            cb_z.append(sg.z)
+            cb_nf.append(sg.n_filler)
+            cb_e.append(max(e, 1))
+            cb_kp.append(sg.k_prime)
+    cb_bg, cb_z, cb_nf, cb_e, cb_kp = map(np.array, (cb_bg, cb_z, cb_nf, cb_e, cb_kp))
+    nb = len(cb_bg)
+    e_llr, e_off, _, _ = traffic_blocks(cb_bg, cb_z, cb_nf, cb_e, np.full(nb, 2.0), cb_kp / cb_e, args.seed + 5,
+                                        want_info=False)
+    n = np.where(cb_bg == 0, 68, 52) * cb_z
+    offs = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+    plan = DecodePlan(cb_bg, cb_z, cb_nf, offs, args.beta)
+    plan.set_dematch(e_off, cb_e)
+    d_llr = torch.empty(int(n.sum()), dtype=torch.float32, device="cuda")
+    plan.dematch(torch.from_numpy(e_llr).cuda(), d_llr)
+    pool = HostPool(int(n.sum()))
+    pool.array[:] = d_llr.cpu().numpy()
+    # info bits per code block: the TB payload spread over its blocks
+    info_bits = np.zeros(nb, np.int64)
+    for t in range(n_tb):
+        info_bits[tb_first[t]:tb_first[t] + tb_num[t]] = tbs[t] // tb_num[t]
+    srv = Server(args.beta, args.iters, hbm_budget_bytes=1 << 30, max_batch=1024, streams=3)
+    run = lambda rate, dur: serve_loadgen(srv, pool, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num, rate,
+                                          dur, seed=args.seed)
+    run(0, 0.5)  # warm
+    cbl, tbl, peak_bps, _ = run(0, 2.0)
+    mean_tb_bits = float(np.mean(tbs))
+    peak_tb_rate = peak_bps / mean_tb_bits
+    out = {"workload": "Poisson TB arrivals, TBS log-U[1e3,5e5], rate in {1/3..5/6}, BPSK/AWGN 2.0 dB, "
+                       f"{n_tb} TBs / {nb} code blocks cycled, HBM budget 1 GiB, 3 launch lanes",
+           "peak_info_gbps": round(peak_bps / 1e9, 4), "peak_cb_per_s": round(len(cbl) / 2.0, 1),
+           "peak_cb_latency_ms_p50": round(float(np.percentile(cbl, 50)) / 1e6, 4),
+           "loads": []}
+    for load in (0.1, 0.3, 0.5, 0.7, 0.9):
+        cbl, tbl, bps, waits = run(load * peak_tb_rate, 2.0)
+        out["loads"].append({"offered_fraction_of_peak": load, "achieved_info_gbps": round(bps / 1e9, 4),
+                             "cb_latency_ms_p50": round(float(np.percentile(cbl, 50)) / 1e6, 4),
+                             "cb_latency_ms_p99": round(float(np.percentile(cbl, 99)) / 1e6, 4),
+                             "tb_latency_ms_p50": round(float(np.percentile(tbl, 50)) / 1e6, 4),
+                             "tb_latency_ms_p99": round(float(np.percentile(tbl, 99)) / 1e6, 4),
+                             "code_blocks": int(len(cbl)), "admission_waits": int(waits)})
+    # the same offered load measured against the batched decoder's peak
+    tb_rate_ref = 0.7 * peak_ref_gbps * 1e9 / mean_tb_bits
+    cbl, tbl, bps, waits = run(tb_rate_ref, 2.0)
+    out["at_70pct_of_batched_peak"] = {"offered_info_gbps": round(0.7 * peak_ref_gbps, 4),
+                                       "achieved_info_gbps": round(bps / 1e9, 4),
+                                       "cb_latency_ms_p50": round(float(np.percentile(cbl, 50)) / 1e6, 4),
+                                       "cb_latency_ms_p99": round(float(np.percentile(cbl, 99)) / 1e6, 4)}
+    srv.close()
+    return out
+
+
+def bench_wifi(torch, args):
+    """Wi-Fi Z=27 rate 1/2 (n=648, k=324), seeded random info bits, BPSK/AWGN
+    2.0 dB, f32, the crate's default offset (0.5), max 10 iterations with
+    early exit, 10,000 frames in one launch; beside the CPU path."""
+    from syndrome_amd import QcLdpcDecoder, awgn_transmit, _lib as L
+    F = 10000
+    dec = QcLdpcDecoder.wifi(27, 1, 2, 0.5)
+    n, k = 648, 324
+    info = np.random.default_rng(args.seed + 1).integers(0, 2, (F, k), dtype=np.uint8)
+    cws = dec.encode_batch(info)
+    llr = np.stack([awgn_transmit(2.0, 0.5, f, cws[f]) for f in range(F)])
+    d_in = torch.from_numpy(llr).cuda()
+    d_post, d_hard = torch.empty_like(d_in), torch.empty((F, n), dtype=torch.uint8, device="cuda")
+    d_it, d_edge = torch.empty(F, dtype=torch.int32, device="cuda"), torch.empty((F, dec.required_edge_buffer()),
+                                                                                     device="cuda")
+    run = lambda: dec.decode_batch_f32(d_in, d_post, d_hard, d_it, 10, flags=L.WRITE_POST | L.WRITE_HARD,
+                                       d_edge=d_edge)
+    t = _timed(run, 10, torch)
+    out = {"workload": "Wi-Fi Z=27 R=1/2, 2.0 dB, beta 0.5, max 10 it, 10,000 frames", "value": round(F * k / t / 1e9, 4),
+           "unit": "Gbit/s", "ms_per_launch": round(t * 1e3, 3), "mean_iterations": round(float(d_it.float().mean()), 3)}
+    if not args.no_cpu:
+        from oracle import orc
+        code = orc.code_wifi(27, 1, 2)
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+        el, its, _ = orc.pipeline_run(code, 0.5, llr, 10, workers, init5g=False)
+        out["cpu_pipeline"] = {"value": round(F * k / el / 1e9, 5), "unit": "Gbit/s", "cores": workers,
+                               "kind": "port"}
+    return out
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-profiles":
         # bench.py --summarize-profiles TRACE_DIR FETCH_DIR WRITE_DIR OUT_PREFIX KERNEL CMD_TRACE CMD_PMC
@@ -185,6 +410,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only (no int8 / uneven / serving / Wi-Fi extras)")
     ap.add_argument("--latency-samples", type=int, default=200,
                     help="single-codeword launches timed for latency (0 = skip)")
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -298,6 +524,20 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(out)
+    if not args.quick:
+        extras = {}
+        for name, fn in (("int8_bg2_z208", lambda: bench_int8(torch, args)),
+                         ("uneven_persistent", lambda: bench_uneven(torch, args)),
+                         ("serving", lambda: bench_serving(torch, args, value)),
+                         ("wifi_z27", lambda: bench_wifi(torch, args))):
+            log(f"[bench] extra: {name}")
+            t0 = time.perf_counter()
+            try:
+                extras[name] = fn()
+            except Exception as e:  # an extra never hides the headline line
+                extras[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] extra {name} done in {time.perf_counter() - t0:.1f} s")
+        line["extras"] = extras
     print(json.dumps(line), flush=True)
 
 

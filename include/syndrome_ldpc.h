@@ -238,11 +238,32 @@ void syndrome_server_destroy(SyndromeServer *server);
 int32_t syndrome_server_acquire(SyndromeServer *server, int32_t base_graph, size_t z, size_t n_filler,
                                 float **out_llr, uint32_t *out_slot);
 int32_t syndrome_server_submit(SyndromeServer *server, uint32_t slot, uint64_t tag);
+/* Submit with the input read from `src` instead of the slot's staging:
+ * device-accessible memory (syndrome_host_alloc or HBM) holding the block's
+ * N floats, untouched until the block completes.  No CPU copy. */
+int32_t syndrome_server_submit_from(SyndromeServer *server, uint32_t slot, const float *src, uint64_t tag);
 /* One completed code block, or SYNDROME_BUSY when none is ready. */
 int32_t syndrome_server_try_recv(SyndromeServer *server, SyndromeServed *out);
 int32_t syndrome_server_release(SyndromeServer *server, uint32_t slot);
 int32_t syndrome_server_stats(const SyndromeServer *server, int64_t *out_batches, int64_t *out_decoded,
                               int32_t *out_slots);
+/* Pinned, device-mapped host memory (usable as a submit_from source). */
+void *syndrome_host_alloc(size_t bytes);
+void syndrome_host_free(void *p);
+/* Load generator: transport blocks drawn cyclically from a pool (TB t =
+ * code blocks tb_first_cb[t] .. + tb_num_cb[t]; block b's decoder input at
+ * pool + cb_off[b], pinned from syndrome_host_alloc) arrive as a Poisson
+ * process of tb_rate TBs per second (tb_rate <= 0: a saturating stream)
+ * for duration_s; every block is acquired, submitted from the pool and
+ * received.  Latency = completion - arrival (admission waits included):
+ * per code block into cb_lat_ns (first cb_cap), per TB (its last block)
+ * into tb_lat_ns.  *info_bps = decoded info bits (cb_info_bits) per second
+ * of wall time; *waits = acquire attempts refused for lack of a slot. */
+int32_t syndrome_server_loadgen(SyndromeServer *server, const float *pool, const int64_t *cb_off, const int32_t *cb_bg,
+                                const int32_t *cb_z, const int32_t *cb_nfill, const int32_t *cb_info_bits,
+                                const int32_t *tb_first_cb, const int32_t *tb_num_cb, size_t n_tb, double tb_rate,
+                                double duration_s, uint64_t seed, double *cb_lat_ns, size_t cb_cap, size_t *n_cb,
+                                double *tb_lat_ns, size_t tb_cap, size_t *n_tbs, double *info_bps, int64_t *waits);
 
 /* Synthetic code-block traffic, as the reference's benchmarks build it
  * (channel_sim.rs, QcLdpcEncoder, rate_match): for block i, K'_i = K_i -

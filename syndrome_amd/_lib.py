@@ -80,6 +80,12 @@ SIGNATURES = {
     "syndrome_server_try_recv": (I32, [P, P]),
     "syndrome_server_release": (I32, [P, C.c_uint32]),
     "syndrome_server_stats": (I32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(I32)]),
+    "syndrome_server_submit_from": (I32, [P, C.c_uint32, P, C.c_uint64]),
+    "syndrome_host_alloc": (P, [SZ]),
+    "syndrome_host_free": (None, [P]),
+    "syndrome_server_loadgen": (I32, [P, P, P, P, P, P, P, P, P, SZ, C.c_double, C.c_double, C.c_uint64, P, SZ,
+                                      C.POINTER(SZ), P, SZ, C.POINTER(SZ), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int64)]),
 }
 
 

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <limits.h>
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
@@ -53,12 +54,14 @@ int z_index(int z)
     return -1;
 }
 
-// Copy each batch entry's decoder input from the mapped pinned slot to HBM.
-__global__ void __launch_bounds__(256) gather(const float *__restrict__ host, float *__restrict__ dev,
+// Copy each batch entry's decoder input (mapped pinned host memory: its
+// slot's staging or a caller's pool) into its HBM slot.
+__global__ void __launch_bounds__(256) gather(const float *const *__restrict__ src, float *__restrict__ dev,
                                               const int *__restrict__ slot, const int *__restrict__ n)
 {
+    const float *s = src[blockIdx.x];
     const long long base = (long long)slot[blockIdx.x] * NMAX;
-    for (int i = threadIdx.x; i < n[blockIdx.x]; i += blockDim.x) dev[base + i] = host[base + i];
+    for (int i = threadIdx.x; i < n[blockIdx.x]; i += blockDim.x) dev[base + i] = s[i];
 }
 
 // Copy each batch entry's hard decisions and iteration count back.
@@ -106,6 +109,7 @@ struct SyndromeServer {
     std::vector<SynCode> codes;
     // slot state
     struct Slot {
+        const float *src = nullptr;  // device-accessible input
         int code = -1, n = 0, nfill = 0;
         uint64_t tag = 0;
         double t_submit = 0, t_done = 0;
@@ -187,7 +191,8 @@ int32_t SyndromeServer::launch_batch(Lane &ln, std::vector<int> &batch)
     const int ni = (int)items.size();
     // descriptor layout (pinned, one H2D copy)
     const size_t o_items = 0, o_ids = o_items + sizeof(SynItem) * ni, o_offs = (o_ids + 4 * nb + 7) & ~(size_t)7;
-    const size_t o_nf = o_offs + 8 * (size_t)nb, o_slot = o_nf + 4 * (size_t)nb, o_n = o_slot + 4 * (size_t)nb;
+    const size_t o_src = o_offs + 8 * (size_t)nb, o_nf = o_src + 8 * (size_t)nb;
+    const size_t o_slot = o_nf + 4 * (size_t)nb, o_n = o_slot + 4 * (size_t)nb;
     const size_t bytes = o_n + 4 * (size_t)nb;
     if (bytes > ln.desc_bytes) return SYNDROME_INVALID_PARAM;
     char *h = (char *)ln.h_desc;
@@ -196,14 +201,15 @@ int32_t SyndromeServer::launch_batch(Lane &ln, std::vector<int> &batch)
     for (int j = 0; j < nb; ++j) {
         const Slot &s = st[batch[j]];
         ((long long *)(h + o_offs))[j] = (long long)batch[j] * NMAX;
+        ((const float **)(h + o_src))[j] = s.src;
         ((int *)(h + o_nf))[j] = s.nfill;
         ((int *)(h + o_slot))[j] = batch[j];
         ((int *)(h + o_n))[j] = s.n;
     }
     char *d = (char *)ln.d_desc;
     if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ln.stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-    hipLaunchKernelGGL(gather, dim3(nb), dim3(256), 0, ln.stream, h_llr_dev, d_llr, (const int *)(d + o_slot),
-                       (const int *)(d + o_n));
+    hipLaunchKernelGGL(gather, dim3(nb), dim3(256), 0, ln.stream, (const float *const *)(d + o_src), d_llr,
+                       (const int *)(d + o_slot), (const int *)(d + o_n));
     if (syn::launch_reset(ln.d_head, ln.stream)) return SYNDROME_DEVICE_ERROR;
     SynLaunch a{};
     a.in = d_llr;
@@ -333,7 +339,7 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
         for (int i = s->slots - 1; i >= 0; --i) s->free_list.push_back(i);
         s->lanes.resize(streams);
         for (Lane &ln : s->lanes) {
-            ln.desc_bytes = (sizeof(SynItem) + 4 + 8 + 4 + 4 + 4) * (size_t)max_batch + 64;
+            ln.desc_bytes = (sizeof(SynItem) + 4 + 8 + 8 + 4 + 4 + 4) * (size_t)max_batch + 64;
             if (hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) != hipSuccess ||
                 hipHostMalloc(&ln.h_desc, ln.desc_bytes, 0) != hipSuccess ||
@@ -390,6 +396,7 @@ int32_t syndrome_server_acquire(SyndromeServer *s, int32_t base_graph, size_t z,
         sl.n = (base_graph == 0 ? 68 : 52) * (int)z;
         sl.nfill = (int)std::min(n_filler, (size_t)INT_MAX);
         sl.state = 1;
+        sl.src = s->h_llr_dev + (size_t)slot * NMAX;
         *out_llr = s->h_llr + (size_t)slot * NMAX;
         *out_slot = (uint32_t)slot;
         return SYNDROME_OK;
@@ -403,6 +410,22 @@ int32_t syndrome_server_submit(SyndromeServer *s, uint32_t slot, uint64_t tag)
         std::lock_guard<std::mutex> lk(s->mu);
         if (slot >= (uint32_t)s->slots || s->st[slot].state != 1) return SYNDROME_INVALID_PARAM;
         SyndromeServer::Slot &sl = s->st[slot];
+        sl.tag = tag;
+        sl.t_submit = now_ns();
+        sl.state = 2;
+        s->submitted.push_back((int)slot);
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_submit_from(SyndromeServer *s, uint32_t slot, const float *src, uint64_t tag)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !src) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (slot >= (uint32_t)s->slots || s->st[slot].state != 1) return SYNDROME_INVALID_PARAM;
+        SyndromeServer::Slot &sl = s->st[slot];
+        sl.src = src;
         sl.tag = tag;
         sl.t_submit = now_ns();
         sl.state = 2;
@@ -452,6 +475,118 @@ int32_t syndrome_server_stats(const SyndromeServer *s, int64_t *out_batches, int
         if (out_batches) *out_batches = s->batches.load();
         if (out_decoded) *out_decoded = s->decoded.load();
         if (out_slots) *out_slots = s->slots;
+        return SYNDROME_OK;
+    });
+}
+
+void *syndrome_host_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    try {
+        if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocMapped) != hipSuccess) return nullptr;
+    } catch (...) {
+        return nullptr;
+    }
+    return p;
+}
+
+void syndrome_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
+int32_t syndrome_server_loadgen(SyndromeServer *s, const float *pool, const int64_t *cb_off, const int32_t *cb_bg,
+                                const int32_t *cb_z, const int32_t *cb_nfill, const int32_t *cb_info_bits,
+                                const int32_t *tb_first_cb, const int32_t *tb_num_cb, size_t n_tb, double tb_rate,
+                                double duration_s, uint64_t seed, double *cb_lat_ns, size_t cb_cap, size_t *n_cb,
+                                double *tb_lat_ns, size_t tb_cap, size_t *n_tbs, double *info_bps, int64_t *waits)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !pool || !cb_off || !cb_bg || !cb_z || !tb_first_cb || !tb_num_cb || !n_tb) return SYNDROME_NULL_POINTER;
+        struct Tb {
+            double arrival;
+            int pool_tb, left;
+            double last_done;
+        };
+        struct Cb {
+            int tb_slot, pool_cb;
+            double arrival;
+        };
+        std::vector<Tb> tbs;
+        std::deque<Cb> pending;
+        std::vector<Cb> inflight(s->slots);
+        uint64_t rs = seed | 1;
+        auto urand = [&]() {
+            rs ^= rs << 13;
+            rs ^= rs >> 7;
+            rs ^= rs << 17;
+            return ((rs >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        };
+        const double t0 = now_ns(), t_end = t0 + duration_s * 1e9;
+        double next = t0;
+        size_t ncb = 0, ntb = 0, next_tb = 0;
+        long long info = 0, wait_count = 0;
+        double last_done = t0;
+        long long outstanding = 0;
+        for (;;) {
+            const double now = now_ns();
+            const bool open = now < t_end;
+            if (open) {
+                if (tb_rate > 0) {
+                    while (next <= now && next < t_end) {
+                        const int p = (int)(next_tb++ % n_tb);
+                        tbs.push_back(Tb{next, p, tb_num_cb[p], 0});
+                        for (int c = 0; c < tb_num_cb[p]; ++c) pending.push_back(Cb{(int)tbs.size() - 1, tb_first_cb[p] + c, next});
+                        next += -log(urand()) / tb_rate * 1e9;
+                    }
+                } else {
+                    while (pending.size() < 4 * (size_t)s->max_batch) {  // saturating stream
+                        const int p = (int)(next_tb++ % n_tb);
+                        tbs.push_back(Tb{now, p, tb_num_cb[p], 0});
+                        for (int c = 0; c < tb_num_cb[p]; ++c) pending.push_back(Cb{(int)tbs.size() - 1, tb_first_cb[p] + c, now});
+                    }
+                }
+            }
+            while (!pending.empty() && (open || tb_rate > 0)) {
+                const Cb cb = pending.front();
+                float *llr = nullptr;
+                uint32_t slot = 0;
+                const int32_t rc = syndrome_server_acquire(s, cb_bg[cb.pool_cb], (size_t)cb_z[cb.pool_cb],
+                                                           cb_nfill ? (size_t)cb_nfill[cb.pool_cb] : 0, &llr, &slot);
+                if (rc == SYNDROME_BUSY) {
+                    ++wait_count;
+                    break;
+                }
+                if (rc) return rc;
+                inflight[slot] = cb;
+                if (int32_t r2 = syndrome_server_submit_from(s, slot, pool + cb_off[cb.pool_cb], (uint64_t)slot)) return r2;
+                pending.pop_front();
+                ++outstanding;
+            }
+            if (!open && tb_rate <= 0) pending.clear();
+            SyndromeServed r;
+            while (syndrome_server_try_recv(s, &r) == SYNDROME_OK) {
+                const Cb &cb = inflight[r.slot];
+                if (ncb < cb_cap && cb_lat_ns) cb_lat_ns[ncb] = r.done_ns - cb.arrival;
+                ++ncb;
+                if (cb_info_bits) info += cb_info_bits[cb.pool_cb];
+                Tb &tb = tbs[cb.tb_slot];
+                tb.last_done = std::max(tb.last_done, r.done_ns);
+                last_done = std::max(last_done, r.done_ns);
+                if (--tb.left == 0) {
+                    if (ntb < tb_cap && tb_lat_ns) tb_lat_ns[ntb] = tb.last_done - tb.arrival;
+                    ++ntb;
+                }
+                (void)syndrome_server_release(s, r.slot);
+                --outstanding;
+            }
+            if (!open && pending.empty() && outstanding == 0) break;
+            if (now > t_end + 30e9) return SYNDROME_DEVICE_ERROR;  // drained for 30 s without finishing
+        }
+        if (n_cb) *n_cb = ncb;
+        if (n_tbs) *n_tbs = ntb;
+        if (info_bps) *info_bps = (double)info / ((last_done - t0) * 1e-9);
+        if (waits) *waits = wait_count;
         return SYNDROME_OK;
     });
 }

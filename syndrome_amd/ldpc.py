@@ -391,3 +391,46 @@ class Server:
         b, d, s = C.c_int64(), C.c_int64(), C.c_int32()
         check(lib().syndrome_server_stats(self._h, C.byref(b), C.byref(d), C.byref(s)), "server_stats")
         return {"batches": b.value, "decoded": d.value, "slots": s.value}
+
+
+class HostPool:
+    """Pinned, device-mapped host memory (syndrome_host_alloc) as a numpy array."""
+
+    def __init__(self, count, dtype=np.float32):
+        nbytes = int(count) * np.dtype(dtype).itemsize
+        self._p = lib().syndrome_host_alloc(nbytes)
+        if not self._p:
+            raise SyndromeError(_lib.DEVICE_ERROR, "host_alloc")
+        buf = (C.c_char * nbytes).from_address(self._p)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(count))
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def __del__(self):
+        try:
+            if self._p:
+                lib().syndrome_host_free(self._p)
+                self._p = None
+        except Exception:
+            pass
+
+
+def serve_loadgen(server, pool, cb_off, cb_bg, cb_z, cb_nfill, cb_info_bits, tb_first, tb_num, tb_rate, duration_s,
+                  seed=1):
+    """Run the C++ load generator (syndrome_server_loadgen).  Returns
+    (cb_latency_ns, tb_latency_ns, info_bits_per_s, refused_acquires)."""
+    arrs = [np.ascontiguousarray(x, t) for x, t in ((cb_off, np.int64), (cb_bg, np.int32), (cb_z, np.int32),
+                                                     (cb_nfill, np.int32), (cb_info_bits, np.int32),
+                                                     (tb_first, np.int32), (tb_num, np.int32))]
+    cap = 4_000_000
+    cb_lat = np.zeros(cap, np.float64)
+    tb_lat = np.zeros(cap // 4, np.float64)
+    n_cb, n_tb = C.c_size_t(), C.c_size_t()
+    bps = C.c_double()
+    waits = C.c_int64()
+    check(lib().syndrome_server_loadgen(server._h, pool.ptr, *[_ptr(a) for a in arrs], len(arrs[5]), tb_rate,
+                                        duration_s, seed, _ptr(cb_lat), cap, C.byref(n_cb), _ptr(tb_lat), cap // 4,
+                                        C.byref(n_tb), C.byref(bps), C.byref(waits)), "server_loadgen")
+    return cb_lat[:min(n_cb.value, cap)], tb_lat[:min(n_tb.value, cap // 4)], bps.value, waits.value
