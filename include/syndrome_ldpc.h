@@ -229,6 +229,8 @@ typedef struct {
     double done_ns;       /* steady-clock ns when its batch was seen complete */
 } SyndromeServed;
 
+/* hbm_budget_bytes buys one slot per 68 * 384 * 4 + 64 bytes (an f32 input
+ * of the largest code); 1 <= max_batch <= 65535; 1 <= streams <= 8. */
 int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_budget_bytes, int32_t max_batch,
                                int32_t streams, SyndromeServer **out_server);
 void syndrome_server_destroy(SyndromeServer *server);

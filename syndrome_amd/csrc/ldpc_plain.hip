@@ -2,6 +2,10 @@
 // dispatch for every decode kernel.
 #include "ldpc_device.h"
 
+#include <mutex>
+#include <set>
+#include <utility>
+
 namespace syn {
 
 // ---------------------------------------------------------------------------
@@ -209,6 +213,30 @@ const void *decode_kernel(int kind, bool i8, bool fast, int z)
     return nullptr;
 }
 
+// Dynamic LDS is a per-function attribute, global to the process.  Raising
+// it once, per function and device, to everything the device allows besides
+// the function's static LDS means that no launcher ever has to lower it, so
+// concurrent launchers with different footprints cannot undercut each other.
+int allow_max_dynamic_lds(const void *fn)
+{
+    static std::mutex mu;
+    static std::set<std::pair<const void *, int>> raised;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return SYN_DEVICE_ERROR;
+    std::lock_guard<std::mutex> lk(mu);
+    if (raised.count({fn, dev})) return SYN_OK;
+    int limit = 0;
+    hipFuncAttributes fa{};
+    if (hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        hipFuncGetAttributes(&fa, fn) != hipSuccess)
+        return SYN_DEVICE_ERROR;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, limit - (int)fa.sharedSizeBytes) !=
+        hipSuccess)
+        return SYN_DEVICE_ERROR;
+    raised.insert({fn, dev});
+    return SYN_OK;
+}
+
 int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, const SynLaunch &a, int block,
                   size_t lds, hipStream_t stream)
 {
@@ -216,8 +244,7 @@ int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, co
     if (grid <= 0) return SYN_OK;
     const void *fn = decode_kernel(kind, i8, fast, z);
     if (!fn) return SYN_INVALID_PARAM;
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        return SYN_DEVICE_ERROR;
+    if (int rc = allow_max_dynamic_lds(fn)) return rc;
     SynLaunch arg = a;
     void *args[] = {(void *)&d_code, (void *)&arg};
     if (hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, stream) != hipSuccess) return SYN_DEVICE_ERROR;

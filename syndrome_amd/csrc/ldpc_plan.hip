@@ -20,6 +20,7 @@ namespace syn {
 const void *persist_kernel_f32();
 const void *persist_kernel_i8();
 int launch_reset(int *head, hipStream_t stream);
+int allow_max_dynamic_lds(const void *fn);
 int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, int count, int max_n, hipStream_t stream);
 }  // namespace syn
 
@@ -136,8 +137,7 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     if (int32_t rc = upload(&p->d_items, flat)) return rc;
     if (int32_t rc = upload(&p->d_ids, ids)) return rc;
     const void *fn = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32();
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        return SYNDROME_DEVICE_ERROR;
+    if (syn::allow_max_dynamic_lds(fn)) return SYNDROME_DEVICE_ERROR;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;

@@ -4,17 +4,18 @@
 // (src/ldpc_pipeline.rs:499-576): acquire a slot, write the code block's
 // decoder input into it, submit, poll completions, release.  Behind it:
 //
-//  * admission by HBM budget: the device arena (inputs, hard decisions,
-//    iteration counts) is cut into fixed slots sized for the largest code
-//    (68 x 384); `budget / slot_bytes` slots exist and acquire refuses with
-//    SYNDROME_BUSY when none is free;
+//  * admission by HBM budget: the device input arena is cut into fixed
+//    slots sized for the largest code (68 x 384 f32); `budget / slot_bytes`
+//    slots exist and acquire refuses with SYNDROME_BUSY when none is free;
 //  * a dispatcher thread forms a batch from everything submitted since the
 //    last one as soon as one of `streams` launch lanes is free, so requests
 //    that arrive while earlier batches run on the GPU are admitted into the
 //    next batch and overlap with them on the device;
-//  * a batch is three launches on its lane: gather (pinned host -> HBM),
-//    the persistent mixed-code decode, scatter (hard bits and iterations
-//    back to the pinned slots); completions are observed by event.
+//  * a batch is two launches on its lane: a staging kernel (decoder inputs
+//    from pinned host memory or HBM into the slots, the descriptor into
+//    device memory) and the persistent mixed-code decode, which writes hard
+//    bits and iteration counts straight into mapped pinned memory;
+//    completions are observed by event.
 // Each code block's result is exactly what decoding it alone gives.
 #include <hip/hip_runtime.h>
 
@@ -37,7 +38,7 @@
 
 namespace syn {
 const void *persist_kernel_f32();
-int launch_reset(int *head, hipStream_t stream);
+int allow_max_dynamic_lds(const void *fn);
 }  // namespace syn
 
 namespace {
@@ -54,24 +55,48 @@ int z_index(int z)
     return -1;
 }
 
-// Copy each batch entry's decoder input (mapped pinned host memory: its
-// slot's staging or a caller's pool) into its HBM slot.
-__global__ void __launch_bounds__(256) gather(const float *const *__restrict__ src, float *__restrict__ dev,
-                                              const int *__restrict__ slot, const int *__restrict__ n)
-{
-    const float *s = src[blockIdx.x];
-    const long long base = (long long)slot[blockIdx.x] * NMAX;
-    for (int i = threadIdx.x; i < n[blockIdx.x]; i += blockDim.x) dev[base + i] = s[i];
-}
+// Batch descriptor, written by the host into mapped pinned memory:
+// items[ni] | ids[nb] | offs[nb] | src[nb] | nfill[nb] | slot[nb] | n[nb].
+struct DescLayout {
+    unsigned o_items, o_ids, o_offs, o_src, o_nf, o_slot, o_n;
+    int ni;
+};
 
-// Copy each batch entry's hard decisions and iteration count back.
-__global__ void __launch_bounds__(256) scatter(const uint8_t *__restrict__ dev_hard, uint8_t *__restrict__ host_hard,
-                                               const int *__restrict__ dev_it, int *__restrict__ host_it,
-                                               const int *__restrict__ slot, const int *__restrict__ n)
+constexpr int STAGE_THREADS = 256, STAGE_UNROLL = 8, STAGE_CHUNK = STAGE_THREADS * STAGE_UNROLL;
+
+// One launch stages a batch: block (x, j) copies elements
+// [x * STAGE_CHUNK, (x + 1) * STAGE_CHUNK) of entry j's decoder input (mapped
+// pinned host memory or HBM) into its HBM slot, every load of a thread issued
+// before its stores so that a PCIe round trip is paid once per chunk, not
+// once per element; blocks (0, j) also copy the descriptor fields the decode
+// reads into device memory, and block (0, 0) resets the work-queue head.
+__global__ void __launch_bounds__(STAGE_THREADS) stage(const char *__restrict__ hd, char *__restrict__ dd,
+                                                       DescLayout lay, float *__restrict__ dev, int *__restrict__ head)
 {
-    const long long base = (long long)slot[blockIdx.x] * NMAX;
-    for (int i = threadIdx.x; i < n[blockIdx.x]; i += blockDim.x) host_hard[base + i] = dev_hard[base + i];
-    if (threadIdx.x == 0) host_it[slot[blockIdx.x]] = dev_it[blockIdx.x];
+    const int j = blockIdx.y;
+    const int n = ((const int *)(hd + lay.o_n))[j];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (j < lay.ni) ((SynItem *)(dd + lay.o_items))[j] = ((const SynItem *)(hd + lay.o_items))[j];
+        ((int *)(dd + lay.o_ids))[j] = ((const int *)(hd + lay.o_ids))[j];
+        ((long long *)(dd + lay.o_offs))[j] = ((const long long *)(hd + lay.o_offs))[j];
+        ((int *)(dd + lay.o_nf))[j] = ((const int *)(hd + lay.o_nf))[j];
+        if (j == 0) *head = 0;
+    }
+    const int i0 = blockIdx.x * STAGE_CHUNK;
+    if (i0 >= n) return;
+    const float *s = ((const float *const *)(hd + lay.o_src))[j];
+    float *d = dev + (long long)((const int *)(hd + lay.o_slot))[j] * NMAX;
+    float v[STAGE_UNROLL];
+#pragma unroll
+    for (int k = 0; k < STAGE_UNROLL; ++k) {
+        const int i = i0 + k * STAGE_THREADS + (int)threadIdx.x;
+        v[k] = i < n ? s[i] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < STAGE_UNROLL; ++k) {
+        const int i = i0 + k * STAGE_THREADS + (int)threadIdx.x;
+        if (i < n) d[i] = v[k];
+    }
 }
 
 struct Lane {
@@ -79,10 +104,12 @@ struct Lane {
     hipEvent_t done = nullptr;
     bool busy = false;
     std::vector<int> slots;
-    // pinned descriptor block: items | ids | offs | nfill | slot | n
-    void *h_desc = nullptr, *d_desc = nullptr;
+    // descriptor: host-written mapped copy (h_desc, device view hd_desc) and
+    // the device copy the decode reads (d_desc)
+    void *h_desc = nullptr, *hd_desc = nullptr, *d_desc = nullptr;
     size_t desc_bytes = 0;
-    int *d_head = nullptr, *d_iters = nullptr;
+    int *d_head = nullptr;
+    int *h_iters = nullptr, *hd_iters = nullptr;  // per batch position, written by the decode
 };
 
 double now_ns()
@@ -101,10 +128,11 @@ struct SyndromeServer {
     int max_batch = 256;
     int slots = 0;
     int lds_limit = 0, cus = 0;
-    // arenas
+    // arenas: decoder inputs in HBM (d_llr), their pinned staging (h_llr),
+    // hard decisions written by the decode straight into pinned memory
     float *h_llr = nullptr, *d_llr = nullptr, *h_llr_dev = nullptr;
-    uint8_t *h_hard = nullptr, *d_hard = nullptr, *h_hard_dev = nullptr;
-    int *h_it = nullptr, *h_it_dev = nullptr;
+    uint8_t *h_hard = nullptr, *h_hard_dev = nullptr;
+    std::vector<int> h_it;  // iterations per slot
     SynCode *d_codes = nullptr;  // 102 codes: bg * 51 + z index
     std::vector<SynCode> codes;
     // slot state
@@ -164,7 +192,7 @@ size_t group_bytes(const SynCode &c, int cws)
 }  // namespace
 
 // Build the batch's work items (grouped by code, as many codewords per item
-// as fit 512 lanes and the LDS, longest first) and issue its three launches.
+// as fit 512 lanes and the LDS, longest first) and issue its two launches.
 int32_t SyndromeServer::launch_batch(Lane &ln, std::vector<int> &batch)
 {
     const int nb = (int)batch.size();
@@ -189,52 +217,56 @@ int32_t SyndromeServer::launch_batch(Lane &ln, std::vector<int> &batch)
     std::stable_sort(tmp.begin(), tmp.end(), [](auto &a, auto &b) { return a.first > b.first; });
     for (auto &x : tmp) items.push_back(x.second);
     const int ni = (int)items.size();
-    // descriptor layout (pinned, one H2D copy)
-    const size_t o_items = 0, o_ids = o_items + sizeof(SynItem) * ni, o_offs = (o_ids + 4 * nb + 7) & ~(size_t)7;
-    const size_t o_src = o_offs + 8 * (size_t)nb, o_nf = o_src + 8 * (size_t)nb;
-    const size_t o_slot = o_nf + 4 * (size_t)nb, o_n = o_slot + 4 * (size_t)nb;
-    const size_t bytes = o_n + 4 * (size_t)nb;
+    DescLayout lay;
+    lay.ni = ni;
+    lay.o_items = 0;
+    lay.o_ids = (unsigned)(sizeof(SynItem) * ni);
+    lay.o_offs = (lay.o_ids + 4u * nb + 7u) & ~7u;
+    lay.o_src = lay.o_offs + 8u * nb;
+    lay.o_nf = lay.o_src + 8u * nb;
+    lay.o_slot = lay.o_nf + 4u * nb;
+    lay.o_n = lay.o_slot + 4u * nb;
+    const size_t bytes = lay.o_n + 4u * (size_t)nb;
     if (bytes > ln.desc_bytes) return SYNDROME_INVALID_PARAM;
     char *h = (char *)ln.h_desc;
-    memcpy(h + o_items, items.data(), sizeof(SynItem) * ni);
-    memcpy(h + o_ids, ids.data(), 4 * (size_t)nb);
+    memcpy(h + lay.o_items, items.data(), sizeof(SynItem) * ni);
+    memcpy(h + lay.o_ids, ids.data(), 4 * (size_t)nb);
+    int n_max = 0;
     for (int j = 0; j < nb; ++j) {
         const Slot &s = st[batch[j]];
-        ((long long *)(h + o_offs))[j] = (long long)batch[j] * NMAX;
-        ((const float **)(h + o_src))[j] = s.src;
-        ((int *)(h + o_nf))[j] = s.nfill;
-        ((int *)(h + o_slot))[j] = batch[j];
-        ((int *)(h + o_n))[j] = s.n;
+        ((long long *)(h + lay.o_offs))[j] = (long long)batch[j] * NMAX;
+        ((const float **)(h + lay.o_src))[j] = s.src;
+        ((int *)(h + lay.o_nf))[j] = s.nfill;
+        ((int *)(h + lay.o_slot))[j] = batch[j];
+        ((int *)(h + lay.o_n))[j] = s.n;
+        n_max = std::max(n_max, s.n);
     }
     char *d = (char *)ln.d_desc;
-    if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ln.stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-    hipLaunchKernelGGL(gather, dim3(nb), dim3(256), 0, ln.stream, (const float *const *)(d + o_src), d_llr,
-                       (const int *)(d + o_slot), (const int *)(d + o_n));
-    if (syn::launch_reset(ln.d_head, ln.stream)) return SYNDROME_DEVICE_ERROR;
+    const dim3 sgrid((n_max + STAGE_CHUNK - 1) / STAGE_CHUNK, nb);
+    hipLaunchKernelGGL(stage, sgrid, dim3(STAGE_THREADS), 0, ln.stream, (const char *)ln.hd_desc, d, lay, d_llr,
+                       ln.d_head);
+    if (hipGetLastError() != hipSuccess) return SYNDROME_DEVICE_ERROR;
     SynLaunch a{};
     a.in = d_llr;
     a.post = nullptr;
-    a.hard = d_hard;
-    a.iters = ln.d_iters;
+    a.hard = h_hard_dev;  // same slot layout as d_llr: results land in pinned memory
+    a.iters = ln.hd_iters;
     a.batch = nb;
     a.iterations = iterations;
     a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD;
     a.beta = beta;
     const void *fn = syn::persist_kernel_f32();
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        return SYNDROME_DEVICE_ERROR;
+    if (syn::allow_max_dynamic_lds(fn)) return SYNDROME_DEVICE_ERROR;
     const SynCode *cp = d_codes;
-    const SynItem *ip = (const SynItem *)(d + o_items);
+    const SynItem *ip = (const SynItem *)(d + lay.o_items);
     int n_items = ni;
     int *head = ln.d_head;
-    const int *idp = (const int *)(d + o_ids);
-    const long long *op = (const long long *)(d + o_offs);
-    const int *nfp = (const int *)(d + o_nf);
+    const int *idp = (const int *)(d + lay.o_ids);
+    const long long *op = (const long long *)(d + lay.o_offs);
+    const int *nfp = (const int *)(d + lay.o_nf);
     void *args[] = {&cp, &ip, &n_items, &head, &idp, &op, &nfp, &a};
     const int grid = std::max(1, std::min(ni, cus));
     if (hipLaunchKernel(fn, dim3(grid), dim3(512), args, lds, ln.stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-    hipLaunchKernelGGL(scatter, dim3(nb), dim3(256), 0, ln.stream, d_hard, h_hard_dev, ln.d_iters, h_it_dev,
-                       (const int *)(d + o_slot), (const int *)(d + o_n));
     if (hipEventRecord(ln.done, ln.stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
     return SYNDROME_OK;
 }
@@ -249,7 +281,9 @@ void SyndromeServer::run()
             if (ln.busy && hipEventQuery(ln.done) == hipSuccess) {
                 const double t = now_ns();
                 std::lock_guard<std::mutex> lk(mu);
-                for (int s : ln.slots) {
+                for (size_t k = 0; k < ln.slots.size(); ++k) {
+                    const int s = ln.slots[k];
+                    h_it[s] = ln.h_iters[k];
                     st[s].state = 4;
                     st[s].t_done = t;
                     done.push_back(s);
@@ -303,8 +337,8 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
     return guard([&]() -> int32_t {
         if (!out) return SYNDROME_NULL_POINTER;
         *out = nullptr;
-        const size_t slot_bytes = (size_t)NMAX * 5 + 64;
-        if (hbm_budget_bytes < slot_bytes || streams < 1 || streams > 8 || max_batch < 1 || max_batch > 65536)
+        const size_t slot_bytes = (size_t)NMAX * 4 + 64;
+        if (hbm_budget_bytes < slot_bytes || streams < 1 || streams > 8 || max_batch < 1 || max_batch > 65535)
             return SYNDROME_INVALID_PARAM;
         std::unique_ptr<SyndromeServer> s(new SyndromeServer());
         s->beta = offset_beta;
@@ -325,16 +359,15 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
         if (hipMalloc((void **)&s->d_codes, sizeof(SynCode) * s->codes.size()) != hipSuccess ||
             hipMemcpy(s->d_codes, s->codes.data(), sizeof(SynCode) * s->codes.size(), hipMemcpyHostToDevice) !=
                 hipSuccess ||
-            hipMalloc((void **)&s->d_llr, n * 4) != hipSuccess || hipMalloc((void **)&s->d_hard, n) != hipSuccess ||
+            hipMalloc((void **)&s->d_llr, n * 4) != hipSuccess ||
             hipHostMalloc((void **)&s->h_llr, n * 4, hipHostMallocMapped) != hipSuccess ||
             hipHostMalloc((void **)&s->h_hard, n, hipHostMallocMapped) != hipSuccess ||
-            hipHostMalloc((void **)&s->h_it, 4 * (size_t)s->slots, hipHostMallocMapped) != hipSuccess ||
             hipHostGetDevicePointer((void **)&s->h_llr_dev, s->h_llr, 0) != hipSuccess ||
-            hipHostGetDevicePointer((void **)&s->h_hard_dev, s->h_hard, 0) != hipSuccess ||
-            hipHostGetDevicePointer((void **)&s->h_it_dev, s->h_it, 0) != hipSuccess) {
+            hipHostGetDevicePointer((void **)&s->h_hard_dev, s->h_hard, 0) != hipSuccess) {
             syndrome_server_destroy(s.release());
             return SYNDROME_DEVICE_ERROR;
         }
+        s->h_it.assign(s->slots, 0);
         s->st.resize(s->slots);
         for (int i = s->slots - 1; i >= 0; --i) s->free_list.push_back(i);
         s->lanes.resize(streams);
@@ -342,9 +375,11 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
             ln.desc_bytes = (sizeof(SynItem) + 4 + 8 + 8 + 4 + 4 + 4) * (size_t)max_batch + 64;
             if (hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) != hipSuccess ||
-                hipHostMalloc(&ln.h_desc, ln.desc_bytes, 0) != hipSuccess ||
+                hipHostMalloc(&ln.h_desc, ln.desc_bytes, hipHostMallocMapped) != hipSuccess ||
+                hipHostGetDevicePointer(&ln.hd_desc, ln.h_desc, 0) != hipSuccess ||
                 hipMalloc(&ln.d_desc, ln.desc_bytes) != hipSuccess || hipMalloc((void **)&ln.d_head, 64) != hipSuccess ||
-                hipMalloc((void **)&ln.d_iters, 4 * (size_t)max_batch) != hipSuccess) {
+                hipHostMalloc((void **)&ln.h_iters, 4 * (size_t)max_batch, hipHostMallocMapped) != hipSuccess ||
+                hipHostGetDevicePointer((void **)&ln.hd_iters, ln.h_iters, 0) != hipSuccess) {
                 syndrome_server_destroy(s.release());
                 return SYNDROME_DEVICE_ERROR;
             }
@@ -367,13 +402,14 @@ void syndrome_server_destroy(SyndromeServer *s)
             if (ln.stream) (void)hipStreamSynchronize(ln.stream);
             if (ln.done) (void)hipEventDestroy(ln.done);
             if (ln.stream) (void)hipStreamDestroy(ln.stream);
-            if (ln.h_desc) (void)hipHostFree(ln.h_desc);
-            for (void *p : {ln.d_desc, (void *)ln.d_head, (void *)ln.d_iters})
+            for (void *p : {ln.h_desc, (void *)ln.h_iters})
+                if (p) (void)hipHostFree(p);
+            for (void *p : {ln.d_desc, (void *)ln.d_head})
                 if (p) (void)hipFree(p);
         }
-        for (void *p : {(void *)s->d_llr, (void *)s->d_hard, (void *)s->d_codes})
+        for (void *p : {(void *)s->d_llr, (void *)s->d_codes})
             if (p) (void)hipFree(p);
-        for (void *p : {(void *)s->h_llr, (void *)s->h_hard, (void *)s->h_it})
+        for (void *p : {(void *)s->h_llr, (void *)s->h_hard})
             if (p) (void)hipHostFree(p);
     } catch (...) {
     }

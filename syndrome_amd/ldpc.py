@@ -419,8 +419,11 @@ class HostPool:
 
 def serve_loadgen(server, pool, cb_off, cb_bg, cb_z, cb_nfill, cb_info_bits, tb_first, tb_num, tb_rate, duration_s,
                   seed=1):
-    """Run the C++ load generator (syndrome_server_loadgen).  Returns
+    """Run the C++ load generator (syndrome_server_loadgen) over a pool of
+    decoder inputs: a HostPool (pinned host memory, read over PCIe) or a
+    device tensor (inputs already resident in HBM).  Returns
     (cb_latency_ns, tb_latency_ns, info_bits_per_s, refused_acquires)."""
+    pool_ptr = pool.ptr if isinstance(pool, HostPool) else pool.data_ptr()
     arrs = [np.ascontiguousarray(x, t) for x, t in ((cb_off, np.int64), (cb_bg, np.int32), (cb_z, np.int32),
                                                      (cb_nfill, np.int32), (cb_info_bits, np.int32),
                                                      (tb_first, np.int32), (tb_num, np.int32))]
@@ -430,7 +433,7 @@ def serve_loadgen(server, pool, cb_off, cb_bg, cb_z, cb_nfill, cb_info_bits, tb_
     n_cb, n_tb = C.c_size_t(), C.c_size_t()
     bps = C.c_double()
     waits = C.c_int64()
-    check(lib().syndrome_server_loadgen(server._h, pool.ptr, *[_ptr(a) for a in arrs], len(arrs[5]), tb_rate,
+    check(lib().syndrome_server_loadgen(server._h, pool_ptr, *[_ptr(a) for a in arrs], len(arrs[5]), tb_rate,
                                         duration_s, seed, _ptr(cb_lat), cap, C.byref(n_cb), _ptr(tb_lat), cap // 4,
                                         C.byref(n_tb), C.byref(bps), C.byref(waits)), "server_loadgen")
     return cb_lat[:min(n_cb.value, cap)], tb_lat[:min(n_tb.value, cap // 4)], bps.value, waits.value

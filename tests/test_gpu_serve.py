@@ -98,7 +98,7 @@ def test_server_admission_by_budget():
     from syndrome_amd import Server
     syndrome_amd.lib()
     say("[serve] admission: 3-slot budget")
-    slot_bytes = 68 * 384 * 5 + 64
+    slot_bytes = 68 * 384 * 4 + 64
     srv = Server(0.5, 4, hbm_budget_bytes=3 * slot_bytes, max_batch=8, streams=1)
     assert srv.stats()["slots"] == 3
     held = [srv.acquire(1, 16) for _ in range(3)]
@@ -125,7 +125,7 @@ def test_server_admission_by_budget():
 def test_loadgen_accounts_every_block():
     """The C++ load generator drives acquire/submit_from/try_recv/release:
     saturating and Poisson modes both finish every admitted transport block,
-    and latencies are positive and ordered (TB >= its blocks)."""
+    and a TB's latency is that of its last block (same arrival instant)."""
     import syndrome_amd
     from syndrome_amd import DecodePlan, HostPool, Server, serve_loadgen, traffic_blocks
     from syndrome_amd.transport import compute_segmentation
@@ -161,7 +161,7 @@ def test_loadgen_accounts_every_block():
     say(f"  poisson: {len(cbl)} blocks, {len(tbl)} TBs, p50 {np.median(cbl) / 1e6:.3f} ms")
     assert len(tbl) > 10  # ~60 arrivals expected
     assert len(cbl) == sum(num[i % 6] for i in range(len(tbl)))  # every admitted TB completed
-    assert (cbl > 0).all() and np.median(tbl) >= np.median(cbl) * 0.999
+    assert (cbl > 0).all() and tbl.max() == cbl.max() and tbl.min() >= cbl.min()
     st = srv.stats()
     assert st["decoded"] >= len(cbl)
     srv.close()
