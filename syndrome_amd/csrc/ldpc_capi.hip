@@ -142,7 +142,13 @@ int32_t run_batch(const SyndromeLdpcDecoder *d, bool i8, const void *in, void *p
     a.batch = (int)batch;
     a.iterations = (int)std::min(iterations, (size_t)INT_MAX);
     a.cws = s.cws;
-    a.flags = flags & 0xffu;
+    a.flags = flags & SYN_F_PUBLIC;
+    {
+        const auto al = [](const void *q) { return ((uintptr_t)q & 15u) == 0; };
+        if (syn::code_n(d->code) % 4 == 0 && al(in) && (!(flags & SYNDROME_BATCH_WRITE_POST) || al(post)) &&
+            (!(flags & SYNDROME_BATCH_WRITE_HARD) || al(hard)))
+            a.flags |= SYN_F_ALIGNED;  // contiguous codewords: offsets b * N
+    }
     a.n_filler = (long long)std::min(n_filler, (size_t)LLONG_MAX);
     a.beta = d->beta;
     a.scale = scale;

@@ -52,6 +52,8 @@ struct SynCode {
 #define SYN_F_WRITE_HARD 4u     // write hard decisions
 #define SYN_F_WRITE_EDGE 8u     // write final R messages (edge_r)
 #define SYN_F_QUANTIZE_IN 16u   // i8 path: input is f32 LLR, quantize_llr_i16 on load
+#define SYN_F_PUBLIC 31u        // the flags a caller may pass; the rest are set by the launchers
+#define SYN_F_ALIGNED 32u       // element offsets and N are multiples of 4, in/post/hard 16-byte aligned
 
 // Per-launch arguments shared by every decode kernel.
 struct SynLaunch {

@@ -122,73 +122,178 @@ __device__ __forceinline__ GroupIO contiguous_group(int cw0, int cws, int batch)
     return GroupIO{n < cws ? n : cws, cw0, nullptr, nullptr, nullptr};
 }
 
-// Load the channel input of the workgroup's codewords into LDS, applying the
-// 5G prologue (init_5g_llr, qc_ldpc.rs:711-732 / decode_5g_i8, 1505-1514),
-// quantisation (quantize_llr_i16, quantize.rs:331-341) and the int8 path's
-// entry clamp (qc_ldpc.rs:1582-1584) on the way.  `cws` codeword slots are
-// filled; slots past io.count get zeros.
+// Per-element input prologue: the 5G layout (init_5g_llr,
+// qc_ldpc.rs:711-732: the 2Z punctured positions read 0, the fillers
+// K'..K read the saturating 1e6), for the int8 path quantisation
+// (quantize_llr_i16, quantize.rs:331-341: v*s, NaN -> 0, round half away
+// from zero, clamp to +-127), the 5G layout in the int8 form (decode_5g_i8,
+// qc_ldpc.rs:1505-1514) and the entry clamp to +-C (qc_ldpc.rs:1582-1584).
+__device__ __forceinline__ float prologue_f32(float v, int pos, int Z2, int kp, int K, bool init5g)
+{
+    if (init5g) {
+        if (pos < Z2) v = 0.f;
+        else if (pos >= kp && pos < K) v = 1000000.0f;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int quantize_one(float x, float scale)
+{
+    float f = x * scale;
+    if (f != f) return 0;
+    f = roundf(f);  // ties away from zero, as f32::round
+    f = f < -127.f ? -127.f : (f > 127.f ? 127.f : f);
+    return (int)f;
+}
+
+__device__ __forceinline__ int prologue_i8(int v, int pos, int Z2, int kp, int K, bool init5g, int C)
+{
+    if (init5g) {
+        if (pos < Z2) v = 0;
+        else if (pos >= kp && pos < K) v = C;
+    }
+    return v < -C ? -C : (v > C ? C : v);
+}
+
+// Load the channel input of the workgroup's codewords into LDS with the
+// prologue above.  `cws` codeword slots are filled; slots past io.count get
+// zeros.  With SYN_F_ALIGNED (every element offset and N a multiple of 4,
+// base pointers 16-byte aligned) each lane moves 4 elements per access and
+// keeps 8 accesses in flight before it writes any of them to LDS, so a
+// workgroup pays a handful of HBM round trips, not one per element.
 template <bool I8>
 __device__ __forceinline__ void load_input(const SynLaunch &a, int Z, int n_b, int m_b, int N, int cws,
                                            const GroupIO &io, void *lds)
 {
+    const int K = (n_b - m_b) * Z, Z2 = 2 * Z;
+    const bool init5g = (a.flags & SYN_F_INIT_5G) != 0;
+    const bool qin = I8 && (a.flags & SYN_F_QUANTIZE_IN) != 0;
+    const int C = a.app_clamp;
+    if ((a.flags & SYN_F_ALIGNED) && (N & 3) == 0) {
+        constexpr int U = 8;
+        const int n4 = N >> 2, total4 = cws * n4, step = (int)blockDim.x;
+        for (int b = threadIdx.x; b < total4; b += U * step) {
+            float4 vf[U];
+            uint2 vh[U];
+            int kp[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                const int i4 = b + k * step;
+                const int cl = i4 / n4;
+                vf[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                vh[k] = make_uint2(0u, 0u);
+                kp[k] = K;
+                if (i4 < total4 && cl < io.count) {
+                    const long long g = io.off(cl, N) + 4LL * (i4 - cl * n4);
+                    const long long nf = io.n_filler(cl, a.n_filler);
+                    kp[k] = nf >= K ? 0 : K - (int)nf;
+                    if (!I8 || qin) vf[k] = *(const float4 *)((const float *)a.in + g);
+                    else vh[k] = *(const uint2 *)((const int16_t *)a.in + g);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                const int i4 = b + k * step;
+                if (i4 < total4) {
+                    const int cl = i4 / n4, p = 4 * (i4 - cl * n4);
+                    const bool live = cl < io.count;
+                    if constexpr (!I8) {
+                        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (live) {
+                            o.x = prologue_f32(vf[k].x, p, Z2, kp[k], K, init5g);
+                            o.y = prologue_f32(vf[k].y, p + 1, Z2, kp[k], K, init5g);
+                            o.z = prologue_f32(vf[k].z, p + 2, Z2, kp[k], K, init5g);
+                            o.w = prologue_f32(vf[k].w, p + 3, Z2, kp[k], K, init5g);
+                        }
+                        *(float4 *)((float *)lds + 4LL * i4) = o;
+                    } else {
+                        int e[4] = {0, 0, 0, 0};
+                        if (live) {
+                            if (qin) {
+                                e[0] = quantize_one(vf[k].x, a.scale);
+                                e[1] = quantize_one(vf[k].y, a.scale);
+                                e[2] = quantize_one(vf[k].z, a.scale);
+                                e[3] = quantize_one(vf[k].w, a.scale);
+                            } else {
+                                e[0] = (int16_t)(vh[k].x & 0xffffu);
+                                e[1] = (int16_t)(vh[k].x >> 16);
+                                e[2] = (int16_t)(vh[k].y & 0xffffu);
+                                e[3] = (int16_t)(vh[k].y >> 16);
+                            }
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) e[q] = prologue_i8(e[q], p + q, Z2, kp[k], K, init5g, C);
+                        }
+                        *(uint2 *)((int16_t *)lds + 4LL * i4) =
+                            make_uint2(((uint32_t)e[0] & 0xffffu) | ((uint32_t)e[1] << 16),
+                                       ((uint32_t)e[2] & 0xffffu) | ((uint32_t)e[3] << 16));
+                    }
+                }
+            }
+        }
+        return;
+    }
     const int total = cws * N;
-    const int K = (n_b - m_b) * Z;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const int cl = i / N, pos = i - cl * N;
         const bool live = cl < io.count;
         const long long g = live ? io.off(cl, N) + pos : 0;
         const long long nf = live ? io.n_filler(cl, a.n_filler) : 0;
-        const long long kp = nf >= K ? 0 : K - nf;
+        const int kp = nf >= K ? 0 : K - (int)nf;
         if constexpr (!I8) {
             float v = 0.f;
-            if (live) {
-                v = ((const float *)a.in)[g];
-                if (a.flags & SYN_F_INIT_5G) {
-                    if (pos < 2 * Z) v = 0.f;
-                    else if (pos >= kp && pos < K) v = 1000000.0f;
-                }
-            }
+            if (live) v = prologue_f32(((const float *)a.in)[g], pos, Z2, kp, K, init5g);
             ((float *)lds)[i] = v;
         } else {
             int v = 0;
             if (live) {
-                if (a.flags & SYN_F_QUANTIZE_IN) {
-                    float f = ((const float *)a.in)[g] * a.scale;
-                    if (f != f) v = 0;
-                    else {
-                        f = roundf(f);  // ties away from zero, as f32::round
-                        f = f < -127.f ? -127.f : (f > 127.f ? 127.f : f);
-                        v = (int)f;
-                    }
-                } else {
-                    v = ((const int16_t *)a.in)[g];
-                }
-                if (a.flags & SYN_F_INIT_5G) {
-                    if (pos < 2 * Z) v = 0;
-                    else if (pos >= kp && pos < K) v = a.app_clamp;
-                }
-                v = v < -a.app_clamp ? -a.app_clamp : (v > a.app_clamp ? a.app_clamp : v);
+                v = qin ? quantize_one(((const float *)a.in)[g], a.scale) : ((const int16_t *)a.in)[g];
+                v = prologue_i8(v, pos, Z2, kp, K, init5g, C);
             }
             ((int16_t *)lds)[i] = (int16_t)v;
         }
     }
 }
 
+// Write the posterior and/or hard decisions of the live codewords back;
+// with SYN_F_ALIGNED 4 elements per lane and access (hard decisions as one
+// 32-bit word).
 template <bool I8>
 __device__ __forceinline__ void store_output(const SynLaunch &a, int N, const GroupIO &io, const void *lds)
 {
+    const bool post = (a.flags & SYN_F_WRITE_POST) != 0, hard = (a.flags & SYN_F_WRITE_HARD) != 0;
+    if ((a.flags & SYN_F_ALIGNED) && (N & 3) == 0) {
+        const int n4 = N >> 2, total4 = io.count * n4;
+        for (int i4 = threadIdx.x; i4 < total4; i4 += blockDim.x) {
+            const int cl = i4 / n4;
+            const long long g = io.off(cl, N) + 4LL * (i4 - cl * n4);
+            uint32_t h;
+            if constexpr (!I8) {
+                const float4 v = *(const float4 *)((const float *)lds + 4LL * i4);
+                if (post) *(float4 *)((float *)a.post + g) = v;
+                h = (v.x < 0.f ? 1u : 0u) | (v.y < 0.f ? 1u << 8 : 0u) | (v.z < 0.f ? 1u << 16 : 0u) |
+                    (v.w < 0.f ? 1u << 24 : 0u);
+            } else {
+                const uint2 v = *(const uint2 *)((const int16_t *)lds + 4LL * i4);
+                if (post) *(uint2 *)((int16_t *)a.post + g) = v;
+                h = ((v.x >> 15) & 1u) | (((v.x >> 31) & 1u) << 8) | (((v.y >> 15) & 1u) << 16) |
+                    (((v.y >> 31) & 1u) << 24);
+            }
+            if (hard) *(uint32_t *)(a.hard + g) = h;
+        }
+        return;
+    }
     const int total = io.count * N;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const int cl = i / N, pos = i - cl * N;
         const long long g = io.off(cl, N) + pos;
         if constexpr (!I8) {
             const float v = ((const float *)lds)[i];
-            if (a.flags & SYN_F_WRITE_POST) ((float *)a.post)[g] = v;
-            if (a.flags & SYN_F_WRITE_HARD) a.hard[g] = v < 0.f ? 1 : 0;
+            if (post) ((float *)a.post)[g] = v;
+            if (hard) a.hard[g] = v < 0.f ? 1 : 0;
         } else {
             const int16_t v = ((const int16_t *)lds)[i];
-            if (a.flags & SYN_F_WRITE_POST) ((int16_t *)a.post)[g] = v;
-            if (a.flags & SYN_F_WRITE_HARD) a.hard[g] = v < 0 ? 1 : 0;
+            if (post) ((int16_t *)a.post)[g] = v;
+            if (hard) a.hard[g] = v < 0 ? 1 : 0;
         }
     }
 }

@@ -33,6 +33,7 @@ struct SyndromeLdpcPlan {
     // per code block
     std::vector<int> bg, z, nfill;
     std::vector<long long> offs;
+    bool offs_aligned = true;  // every offset a multiple of 4 elements
     // unique codes
     std::vector<SynCode> codes;
     std::vector<int> cb_code;
@@ -160,6 +161,12 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
     a.batch = (int)p->bg.size();
     a.iterations = (int)std::min(iterations, (size_t)INT_MAX);
     a.flags = flags & (SYN_F_INIT_5G | SYN_F_WRITE_POST | SYN_F_WRITE_HARD | SYN_F_QUANTIZE_IN);
+    {
+        const auto al = [](const void *q) { return ((uintptr_t)q & 15u) == 0; };
+        if (p->offs_aligned && al(in) && (!(a.flags & SYN_F_WRITE_POST) || al(post)) &&
+            (!(a.flags & SYN_F_WRITE_HARD) || al(hard)))
+            a.flags |= SYN_F_ALIGNED;  // every 3GPP N = 68Z or 52Z is a multiple of 4
+    }
     a.beta = p->beta;
     a.scale = scale;
     a.app_clamp = app_clamp < 0 ? 0 : app_clamp;
@@ -220,6 +227,7 @@ int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int
             p->z.push_back(z[i]);
             p->nfill.push_back(n_filler ? n_filler[i] : 0);
             p->offs.push_back(offsets[i]);
+            p->offs_aligned = p->offs_aligned && (offsets[i] % 4 == 0);
             max_z = std::max(max_z, (int)z[i]);
         }
         DeviceScope ds(p->device);

@@ -253,7 +253,7 @@ int32_t SyndromeServer::launch_batch(Lane &ln, std::vector<int> &batch)
     a.iters = ln.hd_iters;
     a.batch = nb;
     a.iterations = iterations;
-    a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD;
+    a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD | SYN_F_ALIGNED;  // slot offsets are multiples of NMAX
     a.beta = beta;
     const void *fn = syn::persist_kernel_f32();
     if (syn::allow_max_dynamic_lds(fn)) return SYNDROME_DEVICE_ERROR;
