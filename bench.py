@@ -292,15 +292,16 @@ def bench_uneven(torch, args):
             "shape": plan.shape(), "mean_iterations": round(float(its.mean()), 3), "bler": round(errs / B, 4)}
 
 
-def bench_serving(torch, args, peak_ref_gbps):
-    """Poisson arrivals of transport blocks (TBS log-uniform in [1e3, 5e5]
-    bits, MCS rate in {1/3,1/2,2/3,3/4,5/6}, Qm 1) segmented into code blocks
-    (compute_segmentation), BPSK/AWGN 2.0 dB, rate de-matched; served with
-    continuous batching; offered load swept against the saturated peak."""
-    from syndrome_amd import DecodePlan, HostPool, Server, serve_loadgen, traffic_blocks
+def serving_traffic(torch, args, n_tb=96):
+    """The serving workload's pool: n_tb transport blocks (TBS log-uniform
+    in [1e3, 5e5] bits, MCS rate in {1/3,1/2,2/3,3/4,5/6}, Qm 1) segmented
+    into code blocks (compute_segmentation), encoded, rate matched, sent
+    over BPSK/AWGN at 2.0 dB and de-matched on the device into decoder
+    inputs.  Returns a dict of the per-block and per-TB arrays and the
+    inputs in HBM (d_llr) and in pinned host memory (host_pool)."""
+    from syndrome_amd import DecodePlan, HostPool, traffic_blocks
     from syndrome_amd.transport import compute_segmentation
     rng = np.random.default_rng(args.seed + 5)
-    n_tb = 96
     cb_bg, cb_z, cb_nf, cb_e, cb_kp, tb_first, tb_num, tbs = [], [], [], [], [], [], [], []
     for _ in range(n_tb):
         a = int(np.exp(rng.uniform(np.log(1000), np.log(500000))))
@@ -332,35 +333,56 @@ def bench_serving(torch, args, peak_ref_gbps):
     info_bits = np.zeros(nb, np.int64)
     for t in range(n_tb):
         info_bits[tb_first[t]:tb_first[t] + tb_num[t]] = tbs[t] // tb_num[t]
-    mean_tb_bits = float(np.mean(tbs))
-    srv = Server(args.beta, args.iters, hbm_budget_bytes=1 << 30, max_batch=1024, streams=3)
+    return {"n_tb": n_tb, "nb": nb, "offs": offs, "bg": cb_bg, "z": cb_z, "nf": cb_nf, "info_bits": info_bits,
+            "tb_first": tb_first, "tb_num": tb_num, "mean_tb_bits": float(np.mean(tbs)), "d_llr": d_llr,
+            "host_pool": host_pool}
+
+
+def bench_serving(torch, args, peak_ref_gbps):
+    """The serving workload (serving_traffic) offered as Poisson arrivals of
+    transport blocks to the continuous-batching server; offered load swept
+    against the saturated peak, inputs resident in HBM and in pinned host
+    memory."""
+    from syndrome_amd import Server, serve_loadgen
+    tr = serving_traffic(torch, args)
+    n_tb, nb, offs, cb_bg, cb_z, cb_nf = tr["n_tb"], tr["nb"], tr["offs"], tr["bg"], tr["z"], tr["nf"]
+    info_bits, tb_first, tb_num, mean_tb_bits = tr["info_bits"], tr["tb_first"], tr["tb_num"], tr["mean_tb_bits"]
+    d_llr, host_pool = tr["d_llr"], tr["host_pool"]
     out = {"workload": "Poisson TB arrivals, TBS log-U[1e3,5e5], rate in {1/3..5/6}, BPSK/AWGN 2.0 dB, "
-                       f"{n_tb} TBs / {nb} code blocks cycled, HBM budget 1 GiB, 3 launch lanes"}
+                       f"{n_tb} TBs / {nb} code blocks cycled, HBM budget 1 GiB",
+           "modes": {"resident": "one grid stays on the device and pulls code blocks from a queue",
+                     "lanes": "batches of what is pending, launched on 3 streams"}}
 
     def pct(x, q):
         return round(float(np.percentile(x, q)) / 1e6, 4)
 
-    for where, pool in (("inputs_in_hbm", d_llr), ("inputs_in_pinned_host", host_pool)):
-        run = lambda rate, dur: serve_loadgen(srv, pool, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num, rate,
-                                              dur, seed=args.seed)
-        run(0, 0.5)  # warm
-        cbl, tbl, peak_bps, _ = run(0, 2.0)
-        peak_tb_rate = peak_bps / mean_tb_bits
-        res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_cb_per_s": round(len(cbl) / 2.0, 1), "loads": []}
-        for load in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0):
-            cbl, tbl, bps, waits = run(load * peak_tb_rate, 2.0)
-            res["loads"].append({"offered_fraction_of_peak": load, "achieved_info_gbps": round(bps / 1e9, 4),
-                                 "cb_latency_ms_p50": pct(cbl, 50), "cb_latency_ms_p99": pct(cbl, 99),
-                                 "tb_latency_ms_p50": pct(tbl, 50), "tb_latency_ms_p99": pct(tbl, 99),
-                                 "code_blocks": int(len(cbl)), "admission_waits": int(waits)})
-        out[where] = res
-    # the same traffic offered at 70% of the batched BG1 Z=384 decoder's peak
-    cbl, tbl, bps, waits = serve_loadgen(srv, d_llr, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
-                                         0.7 * peak_ref_gbps * 1e9 / mean_tb_bits, 2.0, seed=args.seed)
-    out["inputs_in_hbm_at_70pct_of_batched_bg1_peak"] = {
-        "offered_info_gbps": round(0.7 * peak_ref_gbps, 4), "achieved_info_gbps": round(bps / 1e9, 4),
-        "cb_latency_ms_p50": pct(cbl, 50), "cb_latency_ms_p99": pct(cbl, 99)}
-    srv.close()
+    for mode, streams, pools in (("resident", 0, (("inputs_in_hbm", d_llr), ("inputs_in_pinned_host", host_pool))),
+                                 ("lanes", 3, (("inputs_in_hbm", d_llr),))):
+        srv = Server(args.beta, args.iters, hbm_budget_bytes=1 << 30, max_batch=1024, streams=streams)
+        for where, pool in pools:
+            run = lambda rate, dur: serve_loadgen(srv, pool, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
+                                                  rate, dur, seed=args.seed)
+            run(0, 0.5)  # warm
+            cbl, tbl, peak_bps, _ = run(0, 2.0)
+            peak_tb_rate = peak_bps / mean_tb_bits
+            res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_cb_per_s": round(len(cbl) / 2.0, 1),
+                   "loads": []}
+            for load in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0):
+                cbl, tbl, bps, waits = run(load * peak_tb_rate, 2.0)
+                res["loads"].append({"offered_fraction_of_peak": load, "achieved_info_gbps": round(bps / 1e9, 4),
+                                     "cb_latency_ms_p50": pct(cbl, 50), "cb_latency_ms_p99": pct(cbl, 99),
+                                     "tb_latency_ms_p50": pct(tbl, 50), "tb_latency_ms_p99": pct(tbl, 99),
+                                     "code_blocks": int(len(cbl)), "admission_waits": int(waits)})
+            out[f"{mode}_{where}"] = res
+        if mode == "resident":
+            # the same traffic offered at 70% of the batched BG1 Z=384 decoder's peak
+            cbl, tbl, bps, waits = serve_loadgen(srv, d_llr, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
+                                                 0.7 * peak_ref_gbps * 1e9 / mean_tb_bits, 2.0, seed=args.seed)
+            out["resident_inputs_in_hbm_at_70pct_of_batched_bg1_peak"] = {
+                "offered_info_gbps": round(0.7 * peak_ref_gbps, 4), "achieved_info_gbps": round(bps / 1e9, 4),
+                "cb_latency_ms_p50": pct(cbl, 50), "cb_latency_ms_p99": pct(cbl, 99),
+                "tb_latency_ms_p99": pct(tbl, 99)}
+        srv.close()
     return out
 
 

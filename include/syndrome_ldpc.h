@@ -230,7 +230,11 @@ typedef struct {
 } SyndromeServed;
 
 /* hbm_budget_bytes buys one slot per 68 * 384 * 4 + 64 bytes (an f32 input
- * of the largest code); 1 <= max_batch <= 65535; 1 <= streams <= 8. */
+ * of the largest code); 1 <= max_batch <= 65535.  streams 1..8: batches of
+ * what is pending are launched on that many streams; streams == 0: the
+ * resident decoder, one grid that stays on the device while work flows and
+ * pulls code blocks from a queue in pinned memory as they are submitted
+ * (leaving after 2 ms without work, relaunched on the next submission). */
 int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_budget_bytes, int32_t max_batch,
                                int32_t streams, SyndromeServer **out_server);
 void syndrome_server_destroy(SyndromeServer *server);

@@ -83,6 +83,26 @@ struct SynItem {
     int count;
 };
 
+// Resident serving decoder: one queued item = up to SYN_WORK_MAX code
+// blocks of one code, named by their server slots.  Items live in a ring in
+// host memory; `tail` (items published) and `stop` in a control block there.
+#define SYN_WORK_MAX 32
+struct SynWork {
+    int code;     // index into the server's code table
+    int count;    // code blocks in the item
+    int aligned;  // every input pointer 16-byte aligned
+    int pad;
+    int slot[SYN_WORK_MAX];          // output slot of each block
+    int nfill[SYN_WORK_MAX];         // its filler count
+    const void *src[SYN_WORK_MAX];   // its decoder input (HBM or mapped host memory)
+};
+
+struct SynResidentCtl {
+    unsigned int tail;  // items published; written by the host after the item
+    unsigned int stop;  // non-zero: every workgroup leaves at its next wait
+    unsigned int pad[14];
+};
+
 // One code block for the device rate de-matcher.
 struct SynDematchCb {
     long long e_off;    // offset of its E received LLRs

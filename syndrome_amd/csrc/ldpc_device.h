@@ -99,27 +99,40 @@ struct SFor {
 
 // Where a workgroup's codewords live: `count` codewords, ids base + j (or
 // ids[j]), element offsets id * N (or offs[id]), filler counts the launch's
-// (or nfill[id]).  The contiguous form serves the plain batch launches; the
-// indirect form serves work-queue items over a packed arena of mixed codes.
+// (or nfill[id]); inputs at the launch's `in` + offset.  A resident-server
+// item instead brings, per position j, its own input pointer srcs[j] and
+// filler count nfill_pos[j].  The contiguous form serves the plain batch
+// launches; the indirect forms serve work-queue items over a packed arena
+// of mixed codes and the resident server.
 struct GroupIO {
     int count;
     int base;
     const int *ids;
     const long long *offs;
     const int *nfill;
+    const void *const *srcs;
+    const int *nfill_pos;
 
     __device__ __forceinline__ int id(int j) const { return ids ? ids[j] : base + j; }
     __device__ __forceinline__ long long off(int j, int N) const
     {
         return offs ? offs[id(j)] : (long long)id(j) * N;
     }
-    __device__ __forceinline__ long long n_filler(int j, long long dflt) const { return nfill ? nfill[id(j)] : dflt; }
+    __device__ __forceinline__ long long n_filler(int j, long long dflt) const
+    {
+        return nfill_pos ? nfill_pos[j] : (nfill ? nfill[id(j)] : dflt);
+    }
+    // first input byte of codeword j (elements of esz bytes)
+    __device__ __forceinline__ const char *src(int j, int N, const void *in, int esz) const
+    {
+        return srcs ? (const char *)srcs[j] : (const char *)in + off(j, N) * esz;
+    }
 };
 
 __device__ __forceinline__ GroupIO contiguous_group(int cw0, int cws, int batch)
 {
     const int n = batch - cw0;
-    return GroupIO{n < cws ? n : cws, cw0, nullptr, nullptr, nullptr};
+    return GroupIO{n < cws ? n : cws, cw0, nullptr, nullptr, nullptr, nullptr, nullptr};
 }
 
 // Per-element input prologue: the 5G layout (init_5g_llr,
@@ -184,11 +197,12 @@ __device__ __forceinline__ void load_input(const SynLaunch &a, int Z, int n_b, i
                 vh[k] = make_uint2(0u, 0u);
                 kp[k] = K;
                 if (i4 < total4 && cl < io.count) {
-                    const long long g = io.off(cl, N) + 4LL * (i4 - cl * n4);
                     const long long nf = io.n_filler(cl, a.n_filler);
                     kp[k] = nf >= K ? 0 : K - (int)nf;
-                    if (!I8 || qin) vf[k] = *(const float4 *)((const float *)a.in + g);
-                    else vh[k] = *(const uint2 *)((const int16_t *)a.in + g);
+                    if (!I8 || qin)
+                        vf[k] = *(const float4 *)(io.src(cl, N, a.in, 4) + 16LL * (i4 - cl * n4));
+                    else
+                        vh[k] = *(const uint2 *)(io.src(cl, N, a.in, 2) + 8LL * (i4 - cl * n4));
                 }
             }
 #pragma unroll
@@ -236,17 +250,17 @@ __device__ __forceinline__ void load_input(const SynLaunch &a, int Z, int n_b, i
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const int cl = i / N, pos = i - cl * N;
         const bool live = cl < io.count;
-        const long long g = live ? io.off(cl, N) + pos : 0;
         const long long nf = live ? io.n_filler(cl, a.n_filler) : 0;
         const int kp = nf >= K ? 0 : K - (int)nf;
         if constexpr (!I8) {
             float v = 0.f;
-            if (live) v = prologue_f32(((const float *)a.in)[g], pos, Z2, kp, K, init5g);
+            if (live) v = prologue_f32(((const float *)io.src(cl, N, a.in, 4))[pos], pos, Z2, kp, K, init5g);
             ((float *)lds)[i] = v;
         } else {
             int v = 0;
             if (live) {
-                v = qin ? quantize_one(((const float *)a.in)[g], a.scale) : ((const int16_t *)a.in)[g];
+                v = qin ? quantize_one(((const float *)io.src(cl, N, a.in, 4))[pos], a.scale)
+                        : ((const int16_t *)io.src(cl, N, a.in, 2))[pos];
                 v = prologue_i8(v, pos, Z2, kp, K, init5g, C);
             }
             ((int16_t *)lds)[i] = (int16_t)v;

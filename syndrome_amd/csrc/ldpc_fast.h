@@ -506,7 +506,7 @@ __device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, 
         if (w >= n_items) return;
         const SynItem it = items[w];
         const SynCode *code = codes + it.code;
-        const GroupIO io{it.count, 0, ids + it.first, offs, nfill};
+        const GroupIO io{it.count, 0, ids + it.first, offs, nfill, nullptr, nullptr};
         if (it.kind == SYN_KIND_BG1) fast_group<SYN_KIND_BG1, 0, I8>(code, code->z, it.count, io, a);
         else fast_group<SYN_KIND_BG2, 0, I8>(code, code->z, it.count, io, a);
     }

@@ -1,0 +1,131 @@
+// Resident serving decoder (f32): a grid of one workgroup per CU stays on
+// the device while work flows and pulls queued items one at a time.
+//
+// The host publishes items (SynWork: one code, up to SYN_WORK_MAX code
+// blocks named by their server slots) into a ring in coherent pinned host
+// memory and then advances ctl->tail.  A workgroup whose last item is
+// finished claims the next index by compare-and-swap on a device counter,
+// but only an index below the tail it has seen, so nothing is ever claimed
+// before it exists and every claimed item is decoded.  Waiting workgroups
+// read a device-memory copy of the tail; one of them at a time (a lock)
+// refreshes the copy from host memory.  Each item is
+// decoded with the register-resident body for its base graph (the
+// compile-time-Z instances for BG1 Z=384 and BG2 Z=208), its inputs read
+// straight from each block's source pointer (HBM or pinned host memory),
+// its hard decisions and iteration counts written straight into pinned host
+// memory; then a system-scope fence and the item's completion word
+// done[w % cap] = w + 1.
+//
+// A workgroup waiting for work leaves once no workgroup has claimed an item
+// for `idle_ticks` of the device wall clock (counted from the grid's start
+// at the earliest, so the whole grid stays up while anything flows and goes
+// down together), once the kernel is older than `life_ticks`, or when the
+// host sets ctl->stop: every wait ends, and the host relaunches the grid
+// when work is published after it has left.
+#include "ldpc_fast.h"
+
+namespace syn {
+
+// Device-side queue words (one 64-byte block in device memory): the claim
+// counter, the device's copy of the host's tail and stop words, and the
+// lock that lets one waiting workgroup at a time refresh that copy from
+// host memory (so idle workgroups spin on device memory, not on PCIe).
+struct ResidentQueue {
+    unsigned head, tail, stop, poll_lock;
+};
+
+__global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict__ codes,
+                                                        const SynWork *__restrict__ ring, unsigned cap,
+                                                        SynResidentCtl *ctl, unsigned *done, ResidentQueue *q,
+                                                        long long *last_claim, const long long *__restrict__ offs,
+                                                        SynLaunch a, long long idle_ticks, long long life_ticks)
+{
+    __shared__ unsigned s_w;
+    __shared__ int s_have, s_code, s_count, s_aligned;
+    __shared__ int s_slots[SYN_WORK_MAX], s_nf[SYN_WORK_MAX];
+    __shared__ const void *s_src[SYN_WORK_MAX];
+    const long long t0 = wall_clock64();
+    for (;;) {
+        __syncthreads();  // the previous item is finished with LDS and the s_* words
+        if (threadIdx.x == 0) {
+            unsigned w = 0;
+            int have = 0;
+            for (;;) {
+                const unsigned h = __hip_atomic_load(&q->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                unsigned t = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if ((int)(t - h) <= 0) {
+                    unsigned unlocked = 0;
+                    if (__hip_atomic_compare_exchange_strong(&q->poll_lock, &unlocked, 1u, __ATOMIC_ACQUIRE,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        t = __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                        const unsigned stop = __hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                        // the lock makes this the only writer: a plain store keeps the
+                        // copy monotonic modulo 2^32
+                        __hip_atomic_store(&q->tail, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (stop) __hip_atomic_store(&q->stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(&q->poll_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+                if ((int)(t - h) > 0) {
+                    unsigned expect = h;
+                    if (__hip_atomic_compare_exchange_strong(&q->head, &expect, h + 1, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        w = h;
+                        have = 1;
+                        __hip_atomic_store(last_claim, (long long)wall_clock64(), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    continue;
+                }
+                // idle = no claim by any workgroup since this grid started
+                const long long now = wall_clock64();
+                const long long lc = __hip_atomic_load(last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const long long since = now - (lc > t0 ? lc : t0);
+                if (__hip_atomic_load(&q->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || since > idle_ticks ||
+                    now - t0 > life_ticks)
+                    break;
+                __builtin_amdgcn_s_sleep(8);
+            }
+            s_w = w;
+            s_have = have;
+            if (have) {
+                const SynWork *wk = ring + w % cap;
+                s_code = wk->code;
+                s_count = wk->count;
+                s_aligned = wk->aligned;
+            }
+        }
+        __syncthreads();
+        if (!s_have) return;
+        const unsigned w = s_w;
+        if ((int)threadIdx.x < s_count) {
+            const SynWork *wk = ring + w % cap;
+            s_slots[threadIdx.x] = wk->slot[threadIdx.x];
+            s_nf[threadIdx.x] = wk->nfill[threadIdx.x];
+            s_src[threadIdx.x] = wk->src[threadIdx.x];
+        }
+        __syncthreads();
+        SynLaunch la = a;
+        if (s_aligned) la.flags |= SYN_F_ALIGNED;
+        const SynCode *code = codes + s_code;
+        const GroupIO io{s_count, 0, s_slots, offs, nullptr, s_src, s_nf};
+        if (code->kind == SYN_KIND_BG1) {
+            if (code->z == 384) fast_group<SYN_KIND_BG1, 384, false>(code, 384, s_count, io, la);
+            else fast_group<SYN_KIND_BG1, 0, false>(code, code->z, s_count, io, la);
+        } else {
+            if (code->z == 208) fast_group<SYN_KIND_BG2, 208, false>(code, 208, s_count, io, la);
+            else fast_group<SYN_KIND_BG2, 0, false>(code, code->z, s_count, io, la);
+        }
+        // Every lane's hard decisions and counts happen before the barrier;
+        // the system-scope release by lane 0 then publishes all of them
+        // with the item's completion word.
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(done + w % cap, w + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+const void *resident_kernel() { return (const void *)resident_decode; }
+
+}  // namespace syn

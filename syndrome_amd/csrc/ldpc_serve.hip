@@ -38,12 +38,21 @@
 
 namespace syn {
 const void *persist_kernel_f32();
+const void *resident_kernel();
 int allow_max_dynamic_lds(const void *fn);
 }  // namespace syn
 
 namespace {
 
 constexpr int NMAX = 68 * 384;
+// Pinned host memory the device reads and writes while kernels run: mapped
+// into the device's address space and coherent (never held in device
+// caches), so a resident kernel sees the host's writes and the host sees
+// the kernel's without a kernel boundary in between.
+constexpr unsigned HOST_FLAGS = hipHostMallocMapped | hipHostMallocCoherent;
+constexpr unsigned RING_ITEMS = 8192;
+constexpr double RESIDENT_IDLE_US = 2000.0;     // grid leaves after this long with nothing claimed
+constexpr double RESIDENT_LIFE_S = 600.0;       // and never waits beyond this age
 constexpr int VALID_Z[51] = {2,  3,  4,  5,  6,  7,  8,  9,  10,  11,  12,  13,  14,  15,  16,  18,  20,
                              22, 24, 26, 28, 30, 32, 36, 40, 44,  48,  52,  56,  60,  64,  72,  80,  88,
                              96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384};
@@ -132,7 +141,7 @@ struct SyndromeServer {
     // hard decisions written by the decode straight into pinned memory
     float *h_llr = nullptr, *d_llr = nullptr, *h_llr_dev = nullptr;
     uint8_t *h_hard = nullptr, *h_hard_dev = nullptr;
-    std::vector<int> h_it;  // iterations per slot
+    int *h_it = nullptr, *h_it_dev = nullptr;  // iterations per slot
     SynCode *d_codes = nullptr;  // 102 codes: bg * 51 + z index
     std::vector<SynCode> codes;
     // slot state
@@ -153,8 +162,37 @@ struct SyndromeServer {
     std::atomic<long long> batches{0}, decoded{0};
     int error = 0;
 
+    // resident mode: one grid stays on the device and pulls items from a
+    // ring in coherent pinned memory (see ldpc_resident.hip)
+    bool resident = false;
+    struct Outstanding {
+        unsigned id;
+        int n;
+        int slot[SYN_WORK_MAX];
+    };
+    struct Resident {
+        unsigned cap = 0;
+        SynWork *ring = nullptr, *ring_dev = nullptr;
+        SynResidentCtl *ctl = nullptr, *ctl_dev = nullptr;
+        unsigned *done = nullptr, *done_dev = nullptr;
+        unsigned *d_head = nullptr;
+        long long *d_last = nullptr, *d_offs = nullptr;
+        hipStream_t stream = nullptr;
+        hipEvent_t ev = nullptr;
+        bool running = false;
+        unsigned published = 0;
+        std::vector<Outstanding> outstanding;
+        std::vector<int> cws;  // code blocks per item, per code
+        int grid = 0;
+        size_t lds = 0;
+        long long idle_ticks = 0, life_ticks = 0;
+    } rs;
+
     int32_t launch_batch(Lane &ln, std::vector<int> &batch);
+    int32_t init_resident();
+    int32_t launch_resident();
     void run();
+    void run_resident();
 };
 
 namespace {
@@ -329,6 +367,182 @@ void SyndromeServer::run()
         if (ln.busy) (void)hipEventSynchronize(ln.done);
 }
 
+int32_t SyndromeServer::init_resident()
+{
+    const void *fn = syn::resident_kernel();
+    hipFuncAttributes fa{};
+    int khz = 0, per_cu = 0;
+    if (syn::allow_max_dynamic_lds(fn) || hipFuncGetAttributes(&fa, fn) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
+        return SYNDROME_DEVICE_ERROR;
+    const size_t room = (size_t)lds_limit - fa.sharedSizeBytes;
+    rs.cws.resize(codes.size());
+    rs.lds = 0;
+    for (size_t c = 0; c < codes.size(); ++c) {
+        int k = std::min(SYN_WORK_MAX, std::max(1, 512 / codes[c].z));
+        while (k > 1 && group_bytes(codes[c], k) > room) --k;
+        if (group_bytes(codes[c], k) > room) return SYNDROME_INVALID_PARAM;
+        rs.cws[c] = k;
+        rs.lds = std::max(rs.lds, group_bytes(codes[c], k));
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, rs.lds) != hipSuccess || per_cu < 1)
+        return SYNDROME_DEVICE_ERROR;
+    rs.grid = per_cu * cus;
+    rs.idle_ticks = (long long)(RESIDENT_IDLE_US * khz / 1000.0);
+    rs.life_ticks = (long long)(RESIDENT_LIFE_S * khz * 1000.0);
+    rs.cap = RING_ITEMS;
+    std::vector<long long> offs(slots);
+    for (int i = 0; i < slots; ++i) offs[i] = (long long)i * NMAX;
+    if (hipHostMalloc((void **)&rs.ring, sizeof(SynWork) * rs.cap, HOST_FLAGS) != hipSuccess ||
+        hipHostMalloc((void **)&rs.ctl, sizeof(SynResidentCtl), HOST_FLAGS) != hipSuccess ||
+        hipHostMalloc((void **)&rs.done, 4 * (size_t)rs.cap, HOST_FLAGS) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rs.ring_dev, rs.ring, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rs.ctl_dev, rs.ctl, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rs.done_dev, rs.done, 0) != hipSuccess ||
+        hipMalloc((void **)&rs.d_head, 64) != hipSuccess || hipMalloc((void **)&rs.d_last, 64) != hipSuccess ||
+        hipMalloc((void **)&rs.d_offs, 8 * (size_t)slots) != hipSuccess ||
+        hipMemcpy(rs.d_offs, offs.data(), 8 * (size_t)slots, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(rs.d_head, 0, 64) != hipSuccess || hipMemset(rs.d_last, 0, 64) != hipSuccess ||
+        hipStreamCreateWithFlags(&rs.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&rs.ev, hipEventDisableTiming) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    memset(rs.ring, 0, sizeof(SynWork) * rs.cap);
+    memset(rs.ctl, 0, sizeof(SynResidentCtl));
+    memset(rs.done, 0, 4 * (size_t)rs.cap);
+    return SYNDROME_OK;
+}
+
+int32_t SyndromeServer::launch_resident()
+{
+    SynLaunch a{};
+    a.hard = h_hard_dev;
+    a.iters = h_it_dev;
+    a.iterations = iterations;
+    a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD;  // SYN_F_ALIGNED per item
+    a.beta = beta;
+    const SynCode *cp = d_codes;
+    const SynWork *ring = rs.ring_dev;
+    unsigned cap = rs.cap;
+    SynResidentCtl *ctl = rs.ctl_dev;
+    unsigned *dn = rs.done_dev;
+    void *queue = rs.d_head;  // head | tail copy | stop copy | poll lock
+    long long *last = rs.d_last;
+    const long long *offs = rs.d_offs;
+    long long idle = rs.idle_ticks, life = rs.life_ticks;
+    void *args[] = {&cp, &ring, &cap, &ctl, &dn, &queue, &last, &offs, &a, &idle, &life};
+    if (hipLaunchKernel(syn::resident_kernel(), dim3(rs.grid), dim3(512), args, rs.lds, rs.stream) != hipSuccess ||
+        hipEventRecord(rs.ev, rs.stream) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    rs.running = true;
+    batches++;
+    return SYNDROME_OK;
+}
+
+// Resident dispatcher: completions by polling each outstanding item's done
+// word; pending submissions grouped by code into items (no waiting: an item
+// holds what is pending now) and published; the grid relaunched whenever
+// it has left while items are outstanding.  Once the grid is seen finished,
+// every item it claimed has its done word set, so items still outstanding
+// after the scan that follows were never claimed.
+void SyndromeServer::run_resident()
+{
+    DeviceScope ds(device);
+    std::vector<int> pend;
+    std::vector<std::vector<int>> by_code(codes.size());
+    while (!stop.load()) {
+        bool progressed = false;
+        const bool grid_done = rs.running && hipEventQuery(rs.ev) == hipSuccess;
+        if (grid_done) rs.running = false;
+        for (size_t k = 0; k < rs.outstanding.size();) {
+            const Outstanding &o = rs.outstanding[k];
+            if (__atomic_load_n(rs.done + o.id % rs.cap, __ATOMIC_ACQUIRE) == o.id + 1u) {
+                const double t = now_ns();
+                std::lock_guard<std::mutex> lk(mu);
+                for (int j = 0; j < o.n; ++j) {
+                    st[o.slot[j]].state = 4;
+                    st[o.slot[j]].t_done = t;
+                    done.push_back(o.slot[j]);
+                }
+                decoded += o.n;
+                rs.outstanding[k] = rs.outstanding.back();
+                rs.outstanding.pop_back();
+                progressed = true;
+            } else {
+                ++k;
+            }
+        }
+        pend.clear();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            while (!submitted.empty()) {
+                pend.push_back(submitted.front());
+                submitted.pop_front();
+            }
+        }
+        if (!pend.empty()) {
+            for (int sl : pend) by_code[st[sl].code].push_back(sl);
+            std::vector<int> back;  // what does not fit in the ring now
+            for (int sl : pend) {
+                std::vector<int> &v = by_code[st[sl].code];
+                if (v.empty()) continue;
+                const int per = rs.cws[st[sl].code];
+                size_t j = 0;
+                for (; j < v.size(); j += (size_t)per) {
+                    if (rs.outstanding.size() + 1 >= rs.cap) break;
+                    const int cnt = (int)std::min(v.size() - j, (size_t)per);
+                    SynWork &wk = rs.ring[rs.published % rs.cap];
+                    Outstanding o{rs.published, cnt, {}};
+                    int aligned = 1;
+                    for (int q = 0; q < cnt; ++q) {
+                        const Slot &x = st[v[j + q]];
+                        wk.slot[q] = v[j + q];
+                        wk.nfill[q] = x.nfill;
+                        wk.src[q] = x.src;
+                        aligned &= ((uintptr_t)x.src & 15u) == 0;
+                        o.slot[q] = v[j + q];
+                    }
+                    wk.code = st[v[j]].code;
+                    wk.count = cnt;
+                    wk.aligned = aligned;
+                    rs.outstanding.push_back(o);
+                    ++rs.published;
+                }
+                for (; j < v.size(); ++j) back.push_back(v[j]);
+                v.clear();
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (int sl : pend) st[sl].state = 3;
+                for (auto it = back.rbegin(); it != back.rend(); ++it) {
+                    st[*it].state = 2;
+                    submitted.push_front(*it);
+                }
+            }
+            __atomic_store_n(&rs.ctl->tail, rs.published, __ATOMIC_RELEASE);
+            progressed = true;
+        }
+        if (!rs.running && !rs.outstanding.empty()) {
+            if (int32_t rc = launch_resident()) {
+                std::lock_guard<std::mutex> lk(mu);
+                error = rc;
+                for (const Outstanding &o : rs.outstanding)
+                    for (int j = 0; j < o.n; ++j) {
+                        st[o.slot[j]].state = 4;
+                        st[o.slot[j]].t_done = now_ns();
+                        h_it[o.slot[j]] = -1;
+                        done.push_back(o.slot[j]);
+                    }
+                rs.outstanding.clear();
+            }
+            progressed = true;
+        }
+        if (!progressed) std::this_thread::yield();
+    }
+    __atomic_store_n(&rs.ctl->stop, 1u, __ATOMIC_RELEASE);
+    if (rs.running) (void)hipEventSynchronize(rs.ev);
+}
+
 extern "C" {
 
 int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_budget_bytes, int32_t max_batch,
@@ -338,7 +552,7 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
         if (!out) return SYNDROME_NULL_POINTER;
         *out = nullptr;
         const size_t slot_bytes = (size_t)NMAX * 4 + 64;
-        if (hbm_budget_bytes < slot_bytes || streams < 1 || streams > 8 || max_batch < 1 || max_batch > 65535)
+        if (hbm_budget_bytes < slot_bytes || streams < 0 || streams > 8 || max_batch < 1 || max_batch > 65535)
             return SYNDROME_INVALID_PARAM;
         std::unique_ptr<SyndromeServer> s(new SyndromeServer());
         s->beta = offset_beta;
@@ -360,25 +574,38 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
             hipMemcpy(s->d_codes, s->codes.data(), sizeof(SynCode) * s->codes.size(), hipMemcpyHostToDevice) !=
                 hipSuccess ||
             hipMalloc((void **)&s->d_llr, n * 4) != hipSuccess ||
-            hipHostMalloc((void **)&s->h_llr, n * 4, hipHostMallocMapped) != hipSuccess ||
-            hipHostMalloc((void **)&s->h_hard, n, hipHostMallocMapped) != hipSuccess ||
+            hipHostMalloc((void **)&s->h_llr, n * 4, HOST_FLAGS) != hipSuccess ||
+            hipHostMalloc((void **)&s->h_hard, n, HOST_FLAGS) != hipSuccess ||
+            hipHostMalloc((void **)&s->h_it, 4 * (size_t)s->slots, HOST_FLAGS) != hipSuccess ||
             hipHostGetDevicePointer((void **)&s->h_llr_dev, s->h_llr, 0) != hipSuccess ||
-            hipHostGetDevicePointer((void **)&s->h_hard_dev, s->h_hard, 0) != hipSuccess) {
+            hipHostGetDevicePointer((void **)&s->h_hard_dev, s->h_hard, 0) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&s->h_it_dev, s->h_it, 0) != hipSuccess) {
             syndrome_server_destroy(s.release());
             return SYNDROME_DEVICE_ERROR;
         }
-        s->h_it.assign(s->slots, 0);
+        memset(s->h_it, 0, 4 * (size_t)s->slots);
         s->st.resize(s->slots);
         for (int i = s->slots - 1; i >= 0; --i) s->free_list.push_back(i);
+        if (streams == 0) {
+            s->resident = true;
+            if (int32_t rc = s->init_resident()) {
+                syndrome_server_destroy(s.release());
+                return rc;
+            }
+            SyndromeServer *raw = s.get();
+            s->worker = std::thread([raw] { raw->run_resident(); });
+            *out = s.release();
+            return SYNDROME_OK;
+        }
         s->lanes.resize(streams);
         for (Lane &ln : s->lanes) {
             ln.desc_bytes = (sizeof(SynItem) + 4 + 8 + 8 + 4 + 4 + 4) * (size_t)max_batch + 64;
             if (hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) != hipSuccess ||
-                hipHostMalloc(&ln.h_desc, ln.desc_bytes, hipHostMallocMapped) != hipSuccess ||
+                hipHostMalloc(&ln.h_desc, ln.desc_bytes, HOST_FLAGS) != hipSuccess ||
                 hipHostGetDevicePointer(&ln.hd_desc, ln.h_desc, 0) != hipSuccess ||
                 hipMalloc(&ln.d_desc, ln.desc_bytes) != hipSuccess || hipMalloc((void **)&ln.d_head, 64) != hipSuccess ||
-                hipHostMalloc((void **)&ln.h_iters, 4 * (size_t)max_batch, hipHostMallocMapped) != hipSuccess ||
+                hipHostMalloc((void **)&ln.h_iters, 4 * (size_t)max_batch, HOST_FLAGS) != hipSuccess ||
                 hipHostGetDevicePointer((void **)&ln.hd_iters, ln.h_iters, 0) != hipSuccess) {
                 syndrome_server_destroy(s.release());
                 return SYNDROME_DEVICE_ERROR;
@@ -407,9 +634,18 @@ void syndrome_server_destroy(SyndromeServer *s)
             for (void *p : {ln.d_desc, (void *)ln.d_head})
                 if (p) (void)hipFree(p);
         }
+        SyndromeServer::Resident &r = s->rs;
+        if (r.ctl) __atomic_store_n(&r.ctl->stop, 1u, __ATOMIC_RELEASE);
+        if (r.stream) (void)hipStreamSynchronize(r.stream);
+        if (r.ev) (void)hipEventDestroy(r.ev);
+        if (r.stream) (void)hipStreamDestroy(r.stream);
+        for (void *p : {(void *)r.ring, (void *)r.ctl, (void *)r.done})
+            if (p) (void)hipHostFree(p);
+        for (void *p : {(void *)r.d_head, (void *)r.d_last, (void *)r.d_offs})
+            if (p) (void)hipFree(p);
         for (void *p : {(void *)s->d_llr, (void *)s->d_codes})
             if (p) (void)hipFree(p);
-        for (void *p : {(void *)s->h_llr, (void *)s->h_hard})
+        for (void *p : {(void *)s->h_llr, (void *)s->h_hard, (void *)s->h_it})
             if (p) (void)hipHostFree(p);
     } catch (...) {
     }
@@ -519,7 +755,7 @@ void *syndrome_host_alloc(size_t bytes)
 {
     void *p = nullptr;
     try {
-        if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocMapped) != hipSuccess) return nullptr;
+        if (hipHostMalloc(&p, bytes ? bytes : 1, HOST_FLAGS) != hipSuccess) return nullptr;
     } catch (...) {
         return nullptr;
     }

@@ -341,7 +341,10 @@ def traffic_blocks(bg, z, n_filler, e_len, ebno_db, rate, seed, threads=0, want_
 
 class Server:
     """Serving loop (include/syndrome_ldpc.h Part 5): acquire / submit /
-    try_recv / release, as LdpcPipeline (src/ldpc_pipeline.rs:499-576)."""
+    try_recv / release, as LdpcPipeline (src/ldpc_pipeline.rs:499-576).
+    streams >= 1: batches launched on that many streams; streams == 0: the
+    resident decoder (one grid stays on the device and pulls code blocks
+    from a queue as they are submitted)."""
 
     def __init__(self, offset_beta, iterations, hbm_budget_bytes=256 << 20, max_batch=256, streams=3):
         out = C.c_void_p()
