@@ -490,8 +490,10 @@ def main():
             traffic = None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "syn::fast_f32<BG1>", "kernel_ms": round(kern_ms, 4),
-            "algorithmic_bytes_per_launch": alg_bytes}
+            "kernel": "syn::fast_f32<0, 384> (BG1, Z=384)", "kernel_ms": round(kern_ms, 4),
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "measured_hbm_gbs": round(traffic / (kern_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "profile": "profiles/fast_f32_bg1_z384_b4096.md"}
 
     out = {}
     if args.copy_calibrate:
