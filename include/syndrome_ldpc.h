@@ -204,6 +204,15 @@ int32_t syndrome_ldpc_plan_tune(SyndromeLdpcPlan *plan, const void *d_in, void *
 int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *plan, const int64_t *e_offsets, const int32_t *e_lengths,
                                        const int32_t *qm, const int32_t *rv);
 int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *plan, const float *d_e_llr, float *d_llr_out, void *stream);
+/* HARQ combining on device (HarqBuffer::combine, harq.rs:142-158, feeding
+ * prepare_cb_llr, transport_block.rs:473-496): as syndrome_ldpc_plan_dematch,
+ * but each block's de-matched LLRs are added, in the reference's order, to
+ * what its decoder-input slot of d_acc already holds ([2Z zeros | Ncb
+ * accumulated LLRs]), so d_acc is at once the HARQ buffer and the decoder
+ * input.  rv 0..3 overrides every block's redundancy version for this
+ * transmission; rv < 0 keeps set_dematch's.  Flush = zero the slot. */
+int32_t syndrome_ldpc_plan_harq_combine(SyndromeLdpcPlan *plan, const float *d_e_llr, float *d_acc, int32_t rv,
+                                        void *stream);
 
 /* ======================================================================
  * Part 5: serving loop (continuous batching of code blocks on one GPU)

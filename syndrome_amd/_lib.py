@@ -72,6 +72,7 @@ SIGNATURES = {
     "syndrome_ldpc_plan_tune": (I32, [P, P, P, P, P, SZ, C.c_float, C.c_int16, C.c_uint32, I32, C.POINTER(I32)]),
     "syndrome_ldpc_plan_set_dematch": (I32, [P, P, P, P, P]),
     "syndrome_ldpc_plan_dematch": (I32, [P, P, P, P]),
+    "syndrome_ldpc_plan_harq_combine": (I32, [P, P, P, I32, P]),
     "syndrome_traffic_blocks": (I32, [P, P, P, P, P, P, C.c_uint64, SZ, P, P, P, P, I32]),
     "syndrome_server_create": (I32, [C.c_float, SZ, SZ, I32, I32, C.POINTER(P)]),
     "syndrome_server_destroy": (None, [P]),

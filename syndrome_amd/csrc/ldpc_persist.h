@@ -29,8 +29,14 @@ __global__ void __launch_bounds__(512) persist_decode(const SynCode *__restrict_
 //     increasing k, as the reference's sequential "+=" does;
 //   * the decoder input is [2Z punctured zeros | circular buffer]; filler
 //     and unselected positions are 0 (decode_5g pins the fillers).
+// With `combine` set the circular buffer is a HARQ accumulator
+// (HarqBuffer::combine, src/harq.rs:142-158): each position starts from
+// the value it already holds and the new transmission's LLRs are added to
+// it in increasing k, exactly as the reference's "+=" into `acc`.  `rv`
+// >= 0 overrides every block's redundancy version for this launch.
 static __global__ void __launch_bounds__(256) dematch(const float *__restrict__ e_llr, float *__restrict__ out,
-                                                      const SynDematchCb *__restrict__ cbs, int count)
+                                                      const SynDematchCb *__restrict__ cbs, int count, int combine,
+                                                      int rv)
 {
   for (int cbi = blockIdx.y; cbi < count; cbi += gridDim.y) {
     const SynDematchCb cb = cbs[cbi];
@@ -42,11 +48,11 @@ static __global__ void __launch_bounds__(256) dematch(const float *__restrict__ 
     const int fe = k >= 2 * z ? k - 2 * z : 0;
     const int m = ncb - (fe - fs);                 // selectable positions per cycle
     const int j0tab[2][4] = {{0, 17, 33, 56}, {0, 13, 25, 43}};
-    const int k0 = j0tab[cb.bg][cb.rv & 3] * z;
+    const int k0 = j0tab[cb.bg][(rv >= 0 ? rv : cb.rv) & 3] * z;
     const int rows = cb.qm > 0 ? cb.e / cb.qm : 0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        float v = 0.f;
         const int c = p - 2 * z;
+        float v = (combine && c >= 0) ? out[cb.out_off + p] : 0.f;
         if (c >= 0 && !(c >= fs && c < fe) && m > 0) {
             // rank of c among the non-filler positions walked from k0
             auto below = [&](int x) {  // non-filler positions in [0, x)

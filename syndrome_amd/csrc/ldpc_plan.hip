@@ -21,7 +21,8 @@ const void *persist_kernel_f32();
 const void *persist_kernel_i8();
 int launch_reset(int *head, hipStream_t stream);
 int allow_max_dynamic_lds(const void *fn);
-int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, int count, int max_n, hipStream_t stream);
+int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, int count, int max_n, bool combine,
+                   int rv, hipStream_t stream);
 }  // namespace syn
 
 struct SyndromeLdpcPlan {
@@ -357,7 +358,22 @@ int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *p, const float *d_e_llr, fl
         if (!p || !d_e_llr || !d_llr_out) return SYNDROME_NULL_POINTER;
         if (!p->d_dm) return SYNDROME_INVALID_PARAM;
         DeviceScope ds(p->device);
-        return syn::launch_dematch(d_e_llr, d_llr_out, p->d_dm, (int)p->bg.size(), p->dm_max_n, (hipStream_t)stream)
+        return syn::launch_dematch(d_e_llr, d_llr_out, p->d_dm, (int)p->bg.size(), p->dm_max_n, false, -1,
+                                   (hipStream_t)stream)
+                   ? SYNDROME_DEVICE_ERROR
+                   : SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_ldpc_plan_harq_combine(SyndromeLdpcPlan *p, const float *d_e_llr, float *d_acc, int32_t rv,
+                                        void *stream)
+{
+    return guard([&]() -> int32_t {
+        if (!p || !d_e_llr || !d_acc) return SYNDROME_NULL_POINTER;
+        if (!p->d_dm || rv > 3) return SYNDROME_INVALID_PARAM;  // rv < 0: each block's own
+        DeviceScope ds(p->device);
+        return syn::launch_dematch(d_e_llr, d_acc, p->d_dm, (int)p->bg.size(), p->dm_max_n, true, rv < 0 ? -1 : rv,
+                                   (hipStream_t)stream)
                    ? SYNDROME_DEVICE_ERROR
                    : SYNDROME_OK;
     });

@@ -318,6 +318,15 @@ class DecodePlan:
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_plan_dematch(self._h, d_e_llr.data_ptr(), d_llr_out.data_ptr(), s), "plan_dematch")
 
+    def harq_combine(self, d_e_llr, d_acc, rv=-1, stream=None):
+        """HarqBuffer::combine on device (src/harq.rs:142-158): add this
+        transmission's de-matched LLRs to the decoder-input slots of d_acc.
+        rv 0..3 overrides every block's redundancy version."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().syndrome_ldpc_plan_harq_combine(self._h, d_e_llr.data_ptr(), d_acc.data_ptr(), int(rv), s),
+              "plan_harq_combine")
+
 
 def traffic_blocks(bg, z, n_filler, e_len, ebno_db, rate, seed, threads=0, want_info=True):
     """Synthetic code-block traffic (include/syndrome_ldpc.h
