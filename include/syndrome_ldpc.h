@@ -196,11 +196,16 @@ int32_t syndrome_ldpc_plan_shape(const SyndromeLdpcPlan *plan, int32_t *out_bloc
 int32_t syndrome_ldpc_plan_tune(SyndromeLdpcPlan *plan, const void *d_in, void *d_post, uint8_t *d_hard,
                                 int32_t *d_iters, size_t iterations, float scale, int16_t app_clamp, uint32_t flags,
                                 int32_t reps, int32_t *out_block);
-/* Rate de-matching on device (rate_dematch_llr, rate_matching.rs:307-369, and
- * the input placement of prepare_cb_llr, transport_block.rs:473-496): block
- * i's e_lengths[i] received LLRs at element e_offsets[i] of d_e_llr, with
- * modulation order qm[i] and redundancy version rv[i] (NULL = 1 / 0), become
- * its N_i-element decoder input at offsets[i] of d_llr_out. */
+/* Rate de-matching on device, as the reference's decode chain does it
+ * (RateMatchCache::rate_dematch_llr_into, rate_matching.rs:531-551 and
+ * 675-715, into a fresh HARQ buffer, then the input placement of
+ * prepare_cb_llr, transport_block.rs:473-496): block i's e_lengths[i]
+ * received LLRs at element e_offsets[i] of d_e_llr, with modulation order
+ * qm[i] and redundancy version rv[i] (NULL = 1 / 0), become its N_i-element
+ * decoder input at offsets[i] of d_llr_out.  Repetitions are added in
+ * received order; selections past the reference walk's 2*Ncb guard land on
+ * circular-buffer position 0, as there.  set_dematch synchronises with the
+ * device (it replaces the block table). */
 int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *plan, const int64_t *e_offsets, const int32_t *e_lengths,
                                        const int32_t *qm, const int32_t *rv);
 int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *plan, const float *d_e_llr, float *d_llr_out, void *stream);

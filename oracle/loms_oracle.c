@@ -584,7 +584,7 @@ int orc_rate_match(const uint8_t *cw, size_t cw_len, uint8_t *e_out, size_t e_le
     const size_t kp = n_filler > k ? 0 : k - n_filler, two_z = 2 * (size_t)z;
     const size_t fs = kp > two_z ? kp - two_z : 0, fe = k > two_z ? k - two_z : 0;
     const size_t k0 = rv_k0(bg, rv, z);
-    uint8_t *e = (uint8_t *)malloc(e_len ? e_len : 1);
+    uint8_t *e = (uint8_t *)calloc(e_len ? e_len : 1, 1); /* slots past the guard stay 0 */
     size_t ks = 0, j = 0;
     while (ks < e_len) {
         size_t pos = (k0 + j) % ncb;
@@ -624,5 +624,41 @@ int orc_rate_dematch(const float *e_llr, size_t e_len, float *cb, size_t cb_len,
         if (j > 2 * ncb) break;
     }
     free(d);
+    return 0;
+}
+
+/* RateMatchCache::rate_dematch_llr_into (rate_matching.rs:531-551 selection
+ * table, 675-715 accumulation), the de-matcher behind HarqBuffer::combine
+ * (harq.rs:142-158) and so behind prepare_cb_llr: the selection walk of
+ * selection_walk (rate_matching.rs:418-458) fills a table of E circular-buffer
+ * positions, leaving 0 in the slots after its 2*Ncb guard stops it; the
+ * received LLRs are then added in their received (interleaved) order q, each
+ * to position table[(q % rows) * qm + q / rows]. */
+int orc_rate_dematch_cached(const float *e_llr, size_t e_len, float *cb, size_t cb_len, int rv, int qm, int bg,
+                            int z, size_t n_filler)
+{
+    if (rv < 0 || rv >= 4 || qm <= 0 || z <= 0 || e_len % (size_t)qm) return -1;
+    const size_t ncb = (size_t)(bg == 0 ? 66 : 50) * z, k = (size_t)(bg == 0 ? 22 : 10) * z;
+    const size_t kp = n_filler > k ? 0 : k - n_filler, two_z = 2 * (size_t)z;
+    const size_t fs = kp > two_z ? kp - two_z : 0, fe = k > two_z ? k - two_z : 0;
+    size_t *sel = (size_t *)calloc(e_len ? e_len : 1, sizeof(size_t));
+    const size_t k0 = rv_k0(bg, rv, z);
+    size_t ks = 0, j = 0;
+    while (ks < e_len) {
+        const size_t pos = (k0 + j) % ncb;
+        if (pos >= fs && pos < fe) {
+            ++j;
+            continue;
+        }
+        sel[ks++] = pos;
+        ++j;
+        if (j > 2 * ncb) break;
+    }
+    const size_t rows = e_len / (size_t)qm;
+    for (size_t q = 0; q < e_len; ++q) {
+        const size_t idx = sel[(q % rows) * (size_t)qm + q / rows];
+        if (idx < cb_len) cb[idx] += e_llr[q];
+    }
+    free(sel);
     return 0;
 }

@@ -106,6 +106,12 @@ int orc_rate_match(const uint8_t *codeword, size_t cw_len, uint8_t *e_out, size_
 /* rate_dematch_llr (rate_matching.rs:307-369): cb_llr [Ncb] accumulates. */
 int orc_rate_dematch(const float *e_llr, size_t e_len, float *cb_llr, size_t cb_len, int rv, int qm, int bg, int z,
                      size_t n_filler);
+/* RateMatchCache::rate_dematch_llr_into (rate_matching.rs:531-551, 675-715),
+ * as HarqBuffer::combine uses it: same selection, but a table lookup per
+ * received index q, accumulated in received order, with slots past the walk's
+ * 2*Ncb guard mapped to position 0. */
+int orc_rate_dematch_cached(const float *e_llr, size_t e_len, float *cb_llr, size_t cb_len, int rv, int qm, int bg,
+                            int z, size_t n_filler);
 
 /* Knuth multiplicative-hash info bits used by the reference's exporters
  * (ldpc_convergence_export.rs:145-147, ldpc_pipeline_bench.rs:247-250). */

@@ -81,6 +81,8 @@ def lib():
         L.orc_knuth_info_bits.argtypes = [P, C.c_size_t]
         L.orc_rate_match.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t]
         L.orc_rate_dematch.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t]
+        L.orc_rate_dematch_cached.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
+                                              C.c_size_t]
         _lib = L
     return _lib
 
@@ -283,11 +285,14 @@ def rate_match(codeword, e_len, rv, qm, bg, z, n_filler):
     return out
 
 
-def rate_dematch(e_llr, rv, qm, bg, z, n_filler, cb=None):
+def rate_dematch(e_llr, rv, qm, bg, z, n_filler, cb=None, cached=False):
+    """rate_dematch_llr (cached=False) or RateMatchCache::rate_dematch_llr_into
+    (cached=True, the HARQ path); accumulates into cb (default zeros)."""
     e = np.ascontiguousarray(e_llr, np.float32)
     ncb = (66 if bg == 0 else 50) * z
     cb = np.zeros(ncb, np.float32) if cb is None else np.array(cb, np.float32, copy=True)
-    assert lib().orc_rate_dematch(_p(e), e.size, _p(cb), cb.size, rv, qm, bg, z, n_filler) == 0
+    fn = lib().orc_rate_dematch_cached if cached else lib().orc_rate_dematch
+    assert fn(_p(e), e.size, _p(cb), cb.size, rv, qm, bg, z, n_filler) == 0
     return cb
 
 

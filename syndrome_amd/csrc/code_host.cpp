@@ -235,12 +235,14 @@ bool rate_match(const uint8_t *cw, size_t cw_len, uint8_t *e_out, size_t e_len, 
     const size_t ncb = (size_t)(bg == 0 ? 66 : 50) * z, k = (size_t)(bg == 0 ? 22 : 10) * z, twoz = 2 * (size_t)z;
     const size_t kp = n_filler > k ? 0 : k - n_filler;
     const size_t fill_lo = kp > twoz ? kp - twoz : 0, fill_hi = k > twoz ? k - twoz : 0;
-    std::vector<uint8_t> sel(e_len);
+    std::vector<uint8_t> sel(e_len);  // slots the walk does not reach stay 0
     size_t pos = (size_t)k0_table[bg][rv] * z, taken = 0, steps = 0;
-    while (taken < e_len && steps <= 2 * ncb) {
-        if (!(pos >= fill_lo && pos < fill_hi)) sel[taken++] = pos + twoz < cw_len ? cw[pos + twoz] : 0;
+    while (taken < e_len) {
+        const bool filler = pos >= fill_lo && pos < fill_hi;
+        if (!filler) sel[taken++] = pos + twoz < cw_len ? cw[pos + twoz] : 0;
         pos = pos + 1 == ncb ? 0 : pos + 1;
         ++steps;
+        if (!filler && steps > 2 * ncb) break;  // the walk's guard, checked after a selection
     }
     const size_t rows = e_len / (size_t)qm;  // write rows of Qm, read columns
     for (size_t r = 0; r < rows; ++r)

@@ -107,7 +107,9 @@ def test_traffic_generator_matches_oracle_pipeline(oracle):
     each stage as the oracle restates the reference's."""
     from syndrome_amd import traffic_blocks
     cases = [(0, 384, 0, 25344, 1.5), (1, 208, 0, 10400, 0.7), (0, 52, 17, 3000, 2.0), (1, 13, 5, 900, 3.0),
-             (1, 96, 40, 200, 1.0)]
+             (1, 96, 40, 200, 1.0),
+             (0, 8, 40, 1100, 1.0),   # past the selection walk's 2*Ncb guard
+             (1, 4, 30, 450, 2.0)]    # the same with a filler range
     bg = [c[0] for c in cases]
     z = [c[1] for c in cases]
     nf = [c[2] for c in cases]

@@ -116,7 +116,8 @@ def test_device_dematch_matches_oracle(dev_ok, oracle):
         q = [1, 2, 4, 6][i % 4]
         f = int(rng.integers(0, min(40, k // 4) + 1))
         ncb = (66 if b == 0 else 50) * zz
-        e = int(rng.integers(max(q, ncb // 3), 2 * ncb)) // q * q  # punctured and repeated lengths
+        hi = 3 * ncb if i % 6 == 5 else 2 * ncb
+        e = int(rng.integers(max(q, ncb // 3), hi)) // q * q  # punctured, repeated, past the walk's guard
         bg.append(b)
         z.append(zz)
         nf.append(f)
@@ -135,7 +136,7 @@ def test_device_dematch_matches_oracle(dev_ok, oracle):
     torch.cuda.synchronize()
     out = d_out.cpu().numpy()
     for i in range(20):
-        cb = oracle.rate_dematch(e_llr[e_offs[i]:e_offs[i] + e_len[i]], rv[i], qm[i], bg[i], z[i], nf[i])
+        cb = oracle.rate_dematch(e_llr[e_offs[i]:e_offs[i] + e_len[i]], rv[i], qm[i], bg[i], z[i], nf[i], cached=True)
         ref = oracle.decoder_input_from_cb(cb, z[i])
         assert np.array_equal(f32_bits(out[offs[i]:offs[i] + n[i]]), f32_bits(ref)), f"block {i}"
 
@@ -152,7 +153,7 @@ def test_device_harq_combine_matches_oracle(dev_ok, oracle):
     """Four transmissions (rv 0, 2, 3, 1) of mixed code blocks accumulated on
     the device equal the oracle's HarqBuffer restatement after every one."""
     from syndrome_amd import DecodePlan
-    say("[harq] 14 code blocks x 4 transmissions, rv 0/2/3/1")
+    say("[harq] 14 code blocks x 4 transmissions, rv 0/2/3/1, Qm 1/2/4/6, E up to 3 cycles")
     rng = np.random.default_rng(9)
     bg = np.array([b for b, _ in MIX])
     z = np.array([zz for _, zz in MIX])
@@ -160,11 +161,14 @@ def test_device_harq_combine_matches_oracle(dev_ok, oracle):
     nf = np.array([int(rng.integers(0, min(40, kk // 4) + 1)) for kk in k])
     ncb = np.where(bg == 0, 66, 50) * z
     n = ncb + 2 * z
-    e_len = np.array([int(rng.integers(max(1, c // 4), c + c // 3)) for c in ncb])
+    qm = np.array([[1, 2, 4, 6][i % 4] for i in range(len(bg))])
+    # lengths from punctured to beyond the selection walk's guard (> 2 cycles)
+    e_len = np.array([int(rng.integers(max(1, c // 4), c * (3 if i % 5 == 0 else 2) + 7)) // q * q
+                      for i, (c, q) in enumerate(zip(ncb, qm))])
     offs = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
     e_off = np.concatenate([[0], np.cumsum(e_len)[:-1]]).astype(np.int64)
     plan = DecodePlan(bg, z, nf, offs, 0.5)
-    plan.set_dematch(e_off, e_len)
+    plan.set_dematch(e_off, e_len, qm)
     d_acc = torch.zeros(int(n.sum()), dtype=torch.float32, device="cuda")
     acc = [np.zeros(c, np.float32) for c in ncb]
     for rv in (0, 2, 3, 1):
@@ -173,8 +177,8 @@ def test_device_harq_combine_matches_oracle(dev_ok, oracle):
         torch.cuda.synchronize()
         got = d_acc.cpu().numpy()
         for i in range(len(bg)):
-            acc[i] = oracle.rate_dematch(e_llr[e_off[i]:e_off[i] + e_len[i]], rv, 1, int(bg[i]), int(z[i]), int(nf[i]),
-                                         cb=acc[i])
+            acc[i] = oracle.rate_dematch(e_llr[e_off[i]:e_off[i] + e_len[i]], rv, int(qm[i]), int(bg[i]), int(z[i]),
+                                         int(nf[i]), cb=acc[i], cached=True)
             ref = oracle.decoder_input_from_cb(acc[i], int(z[i]))
             assert np.array_equal(f32_bits(got[offs[i]:offs[i] + n[i]]), f32_bits(ref)), f"rv {rv} block {i}"
 
@@ -214,7 +218,7 @@ def test_harq_recovers_blocks_that_fail_alone(dev_ok, oracle):
         torch.cuda.synchronize()
         hard, its = d_hard.cpu().numpy(), d_it.cpu().numpy()
         for j in range(cnt):
-            acc[j] = oracle.rate_dematch(es[j], rv, 1, bgv, zv, nfv, cb=acc[j])
+            acc[j] = oracle.rate_dematch(es[j], rv, 1, bgv, zv, nfv, cb=acc[j], cached=True)
             inp = oracle.init_5g_f32(code, oracle.decoder_input_from_cb(acc[j], zv), nfv)
             ri, _, _, rh = oracle.decode_f32(code, 0.25, inp, 20)
             h = hard[j * n:(j + 1) * n]

@@ -55,7 +55,8 @@ def test_plan_dematch_and_decode_replay_in_a_graph(oracle):
     # the eager result is the oracle's, block by block
     for i in range(len(bg)):
         code = oracle.code_3gpp(int(bg[i]), int(z[i]))
-        cb = oracle.rate_dematch(e_llr[e_off[i]:e_off[i] + e_len[i]], 0, 1, int(bg[i]), int(z[i]), int(nf[i]))
+        cb = oracle.rate_dematch(e_llr[e_off[i]:e_off[i] + e_len[i]], 0, 1, int(bg[i]), int(z[i]), int(nf[i]),
+                                 cached=True)
         inp = oracle.decoder_input_from_cb(cb, int(z[i]))
         assert np.array_equal(f32_bits(ref[0][offs[i]:offs[i] + n[i]]), f32_bits(inp)), f"block {i}: de-match"
         ri, rp, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, inp, int(nf[i])), 8)
