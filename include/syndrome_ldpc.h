@@ -209,6 +209,21 @@ int32_t syndrome_ldpc_plan_tune(SyndromeLdpcPlan *plan, const void *d_in, void *
 int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *plan, const int64_t *e_offsets, const int32_t *e_lengths,
                                        const int32_t *qm, const int32_t *rv);
 int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *plan, const float *d_e_llr, float *d_llr_out, void *stream);
+/* Transport-block checks on device (DlSchDecoder::decode after its
+ * code-block loop, transport_block.rs:963-1044; CRCs per crc.rs): transport
+ * block t is the plan's code blocks [tb_first_cb[t], + tb_num_cb[t]), which
+ * share K' = K - n_filler; each contributes P = K' - 24 payload bits (K'
+ * when alone), the reassembled stream carries tb_size[t] bits and then their
+ * CRC-24A.  set_tbs validates (equal K', num * P >= tb_size + 24) and
+ * uploads the table (synchronising with the device); check_tbs reads the
+ * decoded hard bits at the plan's offsets of d_hard and writes
+ * d_cb_ok[cb] (CRC-24B passed, or no CRC-24B), d_tb_ok[t] (every block ok
+ * and CRC-24A passed) and, only when d_tb_ok[t], the tb_size[t] payload
+ * bits (one per byte) at element tb_out_off[t] of d_tb_out. */
+int32_t syndrome_ldpc_plan_set_tbs(SyndromeLdpcPlan *plan, size_t n_tb, const int32_t *tb_first_cb,
+                                   const int32_t *tb_num_cb, const int32_t *tb_size, const int64_t *tb_out_off);
+int32_t syndrome_ldpc_plan_check_tbs(SyndromeLdpcPlan *plan, const uint8_t *d_hard, uint8_t *d_tb_out,
+                                     uint8_t *d_tb_ok, uint8_t *d_cb_ok, void *stream);
 /* HARQ combining on device (HarqBuffer::combine, harq.rs:142-158, feeding
  * prepare_cb_llr, transport_block.rs:473-496): as syndrome_ldpc_plan_dematch,
  * but each block's de-matched LLRs are added, in the reference's order, to

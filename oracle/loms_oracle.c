@@ -662,3 +662,73 @@ int orc_rate_dematch_cached(const float *e_llr, size_t e_len, float *cb, size_t 
     free(sel);
     return 0;
 }
+
+/* bit_serial_step / Crc24::compute (crc.rs:119-126, 258-300) for a CRC of
+ * `len` bits: MSB-first register, initial value 0, no reflection, no final
+ * XOR.  The byte-table form of the reference computes the same register. */
+uint32_t orc_crc(const uint8_t *bits, size_t n, int len, uint32_t poly)
+{
+    const uint32_t mask = (len >= 32) ? 0xFFFFFFFFu : ((1u << len) - 1u);
+    uint32_t reg = 0;
+    poly &= mask;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t fb = ((reg >> (len - 1)) ^ (uint32_t)(bits[i] & 1u)) & 1u;
+        reg = (reg << 1) & mask;
+        if (fb) reg ^= poly;
+    }
+    return reg;
+}
+
+/* segment (segmentation.rs:361-410): the transport block with its CRC-24A
+ * (b bits) cut into c blocks of P = K' - 24 payload bits (K' when c == 1),
+ * the tail zero-padded, each block followed by its CRC-24B when c > 1. */
+int orc_segment(const uint8_t *tb_with_crc, size_t b, size_t c, size_t k_prime, uint8_t *blocks)
+{
+    const size_t p = c > 1 ? k_prime - 24 : k_prime;
+    if (c == 0 || k_prime < (c > 1 ? 25u : 1u) || c * p < b) return -1;
+    for (size_t ci = 0; ci < c; ++ci) {
+        uint8_t *blk = blocks + ci * k_prime;
+        for (size_t i = 0; i < p; ++i) {
+            const size_t t = ci * p + i;
+            blk[i] = t < b ? (tb_with_crc[t] & 1u) : 0;
+        }
+        if (c > 1) {
+            const uint32_t r = orc_crc(blk, p, 24, 0x800063u);
+            for (int i = 0; i < 24; ++i) blk[p + (size_t)i] = (uint8_t)((r >> (23 - i)) & 1u);
+        }
+    }
+    return 0;
+}
+
+/* The tail of DlSchDecoder::decode (transport_block.rs:963-1044): block ci's
+ * first K' hard bits (at hard + cb_off[ci]) pass CRC-24B unless c == 1; its
+ * first P bits go to all_info[ci*P ...]; the transport block passes when
+ * every block passed and CRC-24A of all_info[0, tb_size) equals
+ * all_info[tb_size, +24).  tb_out gets the tb_size bits only on a pass.
+ * Returns 1 (pass), 0 (fail) or -1 (bad sizes). */
+int orc_tb_check(const uint8_t *hard, const int64_t *cb_off, size_t c, size_t k_prime, size_t tb_size,
+                 uint8_t *cb_ok, uint8_t *tb_out)
+{
+    const size_t p = c > 1 ? k_prime - 24 : k_prime;
+    if (c == 0 || c * p < tb_size + 24) return -1;
+    uint8_t *all = (uint8_t *)malloc(c * p);
+    int every = 1;
+    for (size_t ci = 0; ci < c; ++ci) {
+        const uint8_t *info = hard + cb_off[ci];
+        int ok = 1;
+        if (c > 1) {
+            uint32_t rx = 0;
+            for (int i = 0; i < 24; ++i) rx = (rx << 1) | (info[p + (size_t)i] & 1u);
+            ok = orc_crc(info, p, 24, 0x800063u) == rx;
+        }
+        cb_ok[ci] = (uint8_t)ok;
+        every &= ok;
+        for (size_t i = 0; i < p; ++i) all[ci * p + i] = info[i] & 1u;
+    }
+    uint32_t rx = 0;
+    for (int i = 0; i < 24; ++i) rx = (rx << 1) | all[tb_size + (size_t)i];
+    const int pass = every && orc_crc(all, tb_size, 24, 0x864CFBu) == rx;
+    if (pass) memcpy(tb_out, all, tb_size);
+    free(all);
+    return pass;
+}

@@ -113,6 +113,13 @@ int orc_rate_dematch(const float *e_llr, size_t e_len, float *cb_llr, size_t cb_
 int orc_rate_dematch_cached(const float *e_llr, size_t e_len, float *cb_llr, size_t cb_len, int rv, int qm, int bg,
                             int z, size_t n_filler);
 
+/* CRCs and transport-block assembly (crc.rs, segmentation.rs:361-410,
+ * transport_block.rs:963-1044). */
+uint32_t orc_crc(const uint8_t *bits, size_t n, int len, uint32_t poly);
+int orc_segment(const uint8_t *tb_with_crc, size_t b, size_t c, size_t k_prime, uint8_t *blocks);
+int orc_tb_check(const uint8_t *hard, const int64_t *cb_off, size_t c, size_t k_prime, size_t tb_size,
+                 uint8_t *cb_ok, uint8_t *tb_out);
+
 /* Knuth multiplicative-hash info bits used by the reference's exporters
  * (ldpc_convergence_export.rs:145-147, ldpc_pipeline_bench.rs:247-250). */
 void orc_knuth_info_bits(uint8_t *bits, size_t k);

@@ -81,6 +81,10 @@ def lib():
         L.orc_knuth_info_bits.argtypes = [P, C.c_size_t]
         L.orc_rate_match.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t]
         L.orc_rate_dematch.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t]
+        L.orc_crc.argtypes = [P, C.c_size_t, C.c_int, C.c_uint32]
+        L.orc_crc.restype = C.c_uint32
+        L.orc_segment.argtypes = [P, C.c_size_t, C.c_size_t, C.c_size_t, P]
+        L.orc_tb_check.argtypes = [P, P, C.c_size_t, C.c_size_t, C.c_size_t, P, P]
         L.orc_rate_dematch_cached.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
                                               C.c_size_t]
         _lib = L
@@ -299,3 +303,37 @@ def rate_dematch(e_llr, rv, qm, bg, z, n_filler, cb=None, cached=False):
 def decoder_input_from_cb(cb, z):
     """prepare_cb_llr placement (transport_block.rs:473-496): [2Z zeros | cb]."""
     return np.concatenate([np.zeros(2 * z, np.float32), np.asarray(cb, np.float32)])
+
+
+CRC24A, CRC24B = 0x864CFB, 0x800063
+
+
+def crc(bits, length=24, poly=CRC24A):
+    """Crc24::compute (crc.rs:258-300) for a CRC of `length` bits."""
+    b = np.ascontiguousarray(bits, np.uint8)
+    return int(lib().orc_crc(_p(b), b.size, length, poly))
+
+
+def attach_crc(bits, poly=CRC24A):
+    """Crc24::attach (crc.rs:323-330): bits followed by their 24-bit CRC, MSB first."""
+    r = crc(bits, 24, poly)
+    return np.concatenate([np.asarray(bits, np.uint8), np.array([(r >> (23 - i)) & 1 for i in range(24)], np.uint8)])
+
+
+def segment(tb_with_crc, c, k_prime):
+    """segment (segmentation.rs:361-410): c blocks of K' bits, [c, K']."""
+    t = np.ascontiguousarray(tb_with_crc, np.uint8)
+    out = np.zeros((c, k_prime), np.uint8)
+    assert lib().orc_segment(_p(t), t.size, c, k_prime, _p(out)) == 0
+    return out
+
+
+def tb_check(hard, cb_off, c, k_prime, tb_size):
+    """DlSchDecoder::decode's CRC and reassembly tail: (ok, cb_ok[c], tb bits or None)."""
+    h = np.ascontiguousarray(hard, np.uint8)
+    offs = np.ascontiguousarray(cb_off, np.int64)
+    cb_ok = np.zeros(c, np.uint8)
+    out = np.zeros(tb_size, np.uint8)
+    r = lib().orc_tb_check(_p(h), _p(offs), c, k_prime, tb_size, _p(cb_ok), _p(out))
+    assert r >= 0
+    return bool(r), cb_ok, (out if r else None)

@@ -73,6 +73,8 @@ SIGNATURES = {
     "syndrome_ldpc_plan_set_dematch": (I32, [P, P, P, P, P]),
     "syndrome_ldpc_plan_dematch": (I32, [P, P, P, P]),
     "syndrome_ldpc_plan_harq_combine": (I32, [P, P, P, I32, P]),
+    "syndrome_ldpc_plan_set_tbs": (I32, [P, SZ, P, P, P, P]),
+    "syndrome_ldpc_plan_check_tbs": (I32, [P, P, P, P, P, P]),
     "syndrome_traffic_blocks": (I32, [P, P, P, P, P, P, C.c_uint64, SZ, P, P, P, P, I32]),
     "syndrome_server_create": (I32, [C.c_float, SZ, SZ, I32, I32, C.POINTER(P)]),
     "syndrome_server_destroy": (None, [P]),

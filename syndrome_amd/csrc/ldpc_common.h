@@ -103,6 +103,18 @@ struct SynResidentCtl {
     unsigned int pad[14];
 };
 
+// One transport block for the device TB checks: code blocks
+// [first_cb, first_cb + num_cb) of a plan, each with K' = k_prime bits
+// (its CRC-24B included when num_cb > 1), carrying tb_size payload bits
+// followed by the 24-bit CRC-24A; the payload goes to tb_out + out_off.
+struct SynTb {
+    long long out_off;
+    int first_cb;
+    int num_cb;
+    int k_prime;
+    int tb_size;
+};
+
 // One code block for the device rate de-matcher.
 struct SynDematchCb {
     long long e_off;    // offset of its E received LLRs

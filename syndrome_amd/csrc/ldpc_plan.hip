@@ -23,6 +23,8 @@ int launch_reset(int *head, hipStream_t stream);
 int allow_max_dynamic_lds(const void *fn);
 int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, int count, int max_n, bool combine,
                    int rv, hipStream_t stream);
+int launch_tb_check(const uint8_t *hard, const long long *offs, const SynTb *tbs, int n_tb, uint8_t *tb_out,
+                    uint8_t *tb_ok, uint8_t *cb_ok, hipStream_t stream);
 }  // namespace syn
 
 struct SyndromeLdpcPlan {
@@ -49,6 +51,8 @@ struct SyndromeLdpcPlan {
     long long *d_offs = nullptr;
     SynDematchCb *d_dm = nullptr;
     int dm_max_n = 0;
+    SynTb *d_tbs = nullptr;
+    int n_tbs = 0;
     std::mutex mu;
 };
 
@@ -250,7 +254,7 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *p)
     try {
         DeviceScope ds(p->device);
         for (void *q : {(void *)p->d_codes, (void *)p->d_items, (void *)p->d_ids, (void *)p->d_nfill,
-                        (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm})
+                        (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm, (void *)p->d_tbs})
             if (q) (void)hipFree(q);
     } catch (...) {
     }
@@ -360,6 +364,50 @@ int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *p, const float *d_e_llr, fl
         DeviceScope ds(p->device);
         return syn::launch_dematch(d_e_llr, d_llr_out, p->d_dm, (int)p->bg.size(), p->dm_max_n, false, -1,
                                    (hipStream_t)stream)
+                   ? SYNDROME_DEVICE_ERROR
+                   : SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_ldpc_plan_set_tbs(SyndromeLdpcPlan *p, size_t n_tb, const int32_t *tb_first_cb,
+                                   const int32_t *tb_num_cb, const int32_t *tb_size, const int64_t *tb_out_off)
+{
+    return guard([&]() -> int32_t {
+        if (!p || (n_tb && (!tb_first_cb || !tb_num_cb || !tb_size || !tb_out_off))) return SYNDROME_NULL_POINTER;
+        if (n_tb > (size_t)INT_MAX) return SYNDROME_INVALID_PARAM;
+        std::vector<SynTb> v(n_tb);
+        const long long count = (long long)p->bg.size();
+        for (size_t t = 0; t < n_tb; ++t) {
+            const long long f = tb_first_cb[t], c = tb_num_cb[t];
+            if (f < 0 || c < 1 || f + c > count || tb_size[t] < 1 || tb_out_off[t] < 0) return SYNDROME_INVALID_PARAM;
+            // K' of every block: K - n_filler (all blocks of a TB share it, segmentation.rs:298-303)
+            const auto kprime = [&](long long i) {
+                return (p->bg[i] == 0 ? 22 : 10) * p->z[i] - p->nfill[i];
+            };
+            const int kp = kprime(f);
+            for (long long i = f; i < f + c; ++i)
+                if (kprime(i) != kp) return SYNDROME_INVALID_PARAM;
+            const long long pay = c > 1 ? kp - 24 : kp;
+            if (pay < 1 || c * pay < (long long)tb_size[t] + 24) return SYNDROME_INVALID_PARAM;
+            v[t] = SynTb{tb_out_off[t], (int)f, (int)c, kp, tb_size[t]};
+        }
+        std::lock_guard<std::mutex> lk(p->mu);
+        DeviceScope ds(p->device);
+        if (int32_t rc = upload(&p->d_tbs, v)) return rc;
+        p->n_tbs = (int)n_tb;
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_ldpc_plan_check_tbs(SyndromeLdpcPlan *p, const uint8_t *d_hard, uint8_t *d_tb_out, uint8_t *d_tb_ok,
+                                     uint8_t *d_cb_ok, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if (!p || !d_hard || !d_tb_out || !d_tb_ok || !d_cb_ok) return SYNDROME_NULL_POINTER;
+        if (!p->d_tbs) return SYNDROME_INVALID_PARAM;
+        DeviceScope ds(p->device);
+        return syn::launch_tb_check(d_hard, p->d_offs, p->d_tbs, p->n_tbs, d_tb_out, d_tb_ok, d_cb_ok,
+                                    (hipStream_t)stream)
                    ? SYNDROME_DEVICE_ERROR
                    : SYNDROME_OK;
     });

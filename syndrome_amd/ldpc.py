@@ -318,6 +318,22 @@ class DecodePlan:
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_plan_dematch(self._h, d_e_llr.data_ptr(), d_llr_out.data_ptr(), s), "plan_dematch")
 
+    def set_tbs(self, tb_first_cb, tb_num_cb, tb_size, tb_out_off):
+        """Transport blocks over the plan's code blocks (syndrome_ldpc_plan_set_tbs)."""
+        self._tf = np.ascontiguousarray(tb_first_cb, np.int32)
+        self._tn = np.ascontiguousarray(tb_num_cb, np.int32)
+        self._ts = np.ascontiguousarray(tb_size, np.int32)
+        self._to = np.ascontiguousarray(tb_out_off, np.int64)
+        check(lib().syndrome_ldpc_plan_set_tbs(self._h, len(self._tf), _ptr(self._tf), _ptr(self._tn), _ptr(self._ts),
+                                               _ptr(self._to)), "plan_set_tbs")
+
+    def check_tbs(self, d_hard, d_tb_out, d_tb_ok, d_cb_ok, stream=None):
+        """CRC-24B per block, reassembly and CRC-24A per transport block."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().syndrome_ldpc_plan_check_tbs(self._h, d_hard.data_ptr(), d_tb_out.data_ptr(), d_tb_ok.data_ptr(),
+                                                 d_cb_ok.data_ptr(), s), "plan_check_tbs")
+
     def harq_combine(self, d_e_llr, d_acc, rv=-1, stream=None):
         """HarqBuffer::combine on device (src/harq.rs:142-158): add this
         transmission's de-matched LLRs to the decoder-input slots of d_acc.

@@ -45,3 +45,75 @@ def test_rejects_out_of_range():
         with pytest.raises(ValueError):
             compute_segmentation(a, 0.5)
     compute_segmentation(MAX_TB_SIZE_BITS, 0.5)
+
+
+# --- CRCs and transport-block assembly in the oracle (crc.rs, segmentation.rs:361-410,
+# transport_block.rs:963-1044) ---
+
+def _ascii_bits(s):
+    return [(b >> (7 - i)) & 1 for b in s for i in range(8)]
+
+
+def test_crc_catalogue_check_values(oracle):
+    # the published check values (CRC of ASCII "123456789") of the CRC-24/LTE-A,
+    # CRC-24/LTE-B and CRC-16/XMODEM parameter sets: poly as crc.rs:84-92,
+    # init 0, no reflection, no final XOR
+    bits = _ascii_bits(b"123456789")
+    assert oracle.crc(bits, 24, 0x864CFB) == 0xCDE703
+    assert oracle.crc(bits, 24, 0x800063) == 0x23EF52
+    assert oracle.crc(bits, 16, 0x11021) == 0x31C3
+
+
+def test_crc_algebra(oracle):
+    import numpy as np
+    rng = np.random.default_rng(5)
+    for poly in (oracle.CRC24A, oracle.CRC24B):
+        assert oracle.crc(np.zeros(100, np.uint8), 24, poly) == 0            # all_zeros_produces_zero_remainder
+        g = [1] + [(poly >> (23 - i)) & 1 for i in range(24)]                 # g(x) itself: x^24 g mod g = 0
+        assert oracle.crc(g, 24, poly) == 0
+        a, b = rng.integers(0, 2, 500).astype(np.uint8), rng.integers(0, 2, 500).astype(np.uint8)
+        assert oracle.crc(a ^ b, 24, poly) == oracle.crc(a, 24, poly) ^ oracle.crc(b, 24, poly)  # linearity
+        m = oracle.attach_crc(a, poly)
+        assert oracle.crc(m[:-24], 24, poly) == int("".join(map(str, m[-24:])), 2)  # attach_then_check_roundtrip
+        for i in (0, 17, 499):                                                       # single_bit_flip_detected
+            m2 = m.copy()
+            m2[i] ^= 1
+            assert oracle.crc(m2[:-24], 24, poly) != int("".join(map(str, m2[-24:])), 2)
+        assert oracle.crc([0] * 7 + list(a), 24, poly) == oracle.crc(a, 24, poly)   # leading zeros change nothing
+
+
+def test_segment_round_trip_8425(oracle):
+    # segment_round_trip_awkward_tb_size_8425 (segmentation.rs:606-645)
+    import numpy as np
+    a = 8425
+    tb = np.array([1 if (i * 7 + 3) % 5 < 2 else 0 for i in range(a)], np.uint8)
+    tbc = oracle.attach_crc(tb, oracle.CRC24A)
+    p = compute_segmentation(a, 0.5)
+    assert len(tbc) == p.b
+    blocks = oracle.segment(tbc, p.c, p.k_prime)
+    assert blocks.shape == (p.c, p.k_prime)
+    pay = p.k_prime - 24
+    rec = blocks[:, :pay].reshape(-1)
+    assert np.array_equal(rec[:p.b], tbc) and not rec[p.b:].any()
+    for blk in blocks:  # each block carries its CRC-24B
+        assert oracle.crc(blk[:pay], 24, oracle.CRC24B) == int("".join(map(str, blk[pay:])), 2)
+
+
+@pytest.mark.parametrize("a,rate", [(100, 0.5), (8425, 0.5), (30000, 2 / 3)])
+def test_tb_check_passes_and_catches(oracle, a, rate):
+    import numpy as np
+    rng = np.random.default_rng(a)
+    tb = rng.integers(0, 2, a).astype(np.uint8)
+    p = compute_segmentation(a, rate)
+    blocks = oracle.segment(oracle.attach_crc(tb, oracle.CRC24A), p.c, p.k_prime)
+    stride = p.k_prime + 11  # blocks need not be packed
+    hard = np.zeros(p.c * stride, np.uint8)
+    offs = np.arange(p.c) * stride
+    for i in range(p.c):
+        hard[offs[i]:offs[i] + p.k_prime] = blocks[i]
+    ok, cb_ok, out = oracle.tb_check(hard, offs, p.c, p.k_prime, a)
+    assert ok and cb_ok.all() and np.array_equal(out, tb)
+    hard[offs[-1] + 3] ^= 1  # one payload bit of the last block
+    ok, cb_ok, out = oracle.tb_check(hard, offs, p.c, p.k_prime, a)
+    assert not ok and out is None
+    assert (cb_ok[-1] == 0) == (p.c > 1) and cb_ok[:-1].all()
