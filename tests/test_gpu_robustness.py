@@ -1,0 +1,126 @@
+"""Robustness and API-equivalence checks on the GPU, after the reference's
+own: LLRs with NaN, +-inf, subnormals and signed zeros (tests/robustness.rs
+fuzz values) decode without error and as the oracle does; the workspace path
+equals the raw-buffer path (qc_ldpc.rs:2787-2816); concurrent decodes on one
+handle from several threads (the decoder is Send + Sync, qc_ldpc.rs:480-484)
+each give the oracle's result."""
+import threading
+
+import numpy as np
+import pytest
+
+from tests.helpers import channel, codewords, f32_bits, say
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _fuzz_llrs(rng, n):
+    """The fuzz distribution of robustness.rs:101-116."""
+    kind = rng.integers(0, 16, n)
+    raw = (rng.integers(0, 20001, n) / 100.0).astype(np.float32) * np.where(rng.integers(0, 2, n) == 1, -1, 1)
+    special = np.array([np.nan, -np.nan, np.inf, -np.inf, np.float32(1.17549435e-38) / 4,
+                        -np.float32(1.17549435e-38) / 4, 0.0, -0.0], np.float32)
+    out = raw.astype(np.float32)
+    m = kind < 8
+    out[m] = special[kind[m]]
+    return out
+
+
+def _same_f32(a, b):
+    """Bit-equal, except that NaN payloads may differ (IEEE leaves the
+    propagated payload to the implementation)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.array_equal(f32_bits(a[~na]), f32_bits(b[~nb]))
+
+
+@pytest.mark.parametrize("plain", [False, True], ids=["fast", "plain"])
+def test_special_float_llrs_decode_like_the_oracle(oracle, plain):
+    import syndrome_amd
+    from syndrome_amd import QcLdpcDecoder, _lib as L
+    syndrome_amd.lib()
+    rng = np.random.default_rng(17)
+    for bg, z in ((0, 16), (1, 52), (0, 384)):
+        say(f"[fuzz] BG{bg + 1} Z={z} plain={plain}: NaN / inf / subnormal / signed-zero LLRs")
+        code = oracle.code_3gpp(bg, z)
+        dec = QcLdpcDecoder.with_lifting_size(bg, z, 0.5)
+        B = 6
+        llr = np.stack([_fuzz_llrs(rng, code.n) for _ in range(B)])
+        d_in = torch.from_numpy(llr).cuda()
+        d_post = torch.empty_like(d_in)
+        d_hard = torch.empty(llr.shape, dtype=torch.uint8, device="cuda")
+        d_edge = torch.empty((B, dec.required_edge_buffer()), dtype=torch.float32, device="cuda")
+        d_it = torch.empty(B, dtype=torch.int32, device="cuda")
+        flags = L.WRITE_POST | L.WRITE_HARD | L.WRITE_EDGE | (L.PLAIN if plain else 0)
+        dec.decode_batch_f32(d_in, d_post, d_hard, d_it, 6, flags=flags, d_edge=d_edge)
+        torch.cuda.synchronize()
+        post, hard, edge, its = (d_post.cpu().numpy(), d_hard.cpu().numpy(), d_edge.cpu().numpy(),
+                                 d_it.cpu().numpy())
+        for b in range(B):
+            ri, rp, re, rh = oracle.decode_f32(code, 0.5, llr[b], 6)
+            assert its[b] == ri and np.array_equal(hard[b], rh), f"BG{bg + 1} Z={z} codeword {b}"
+            assert _same_f32(post[b], rp) and _same_f32(edge[b], re), f"BG{bg + 1} Z={z} codeword {b}: values"
+        # the int8 path quantizes them (NaN -> 0, +-inf -> +-127) and stays exact
+        d_post8 = torch.empty(llr.shape, dtype=torch.int16, device="cuda")
+        dec.decode_batch_i8(d_in, d_post8, d_hard, d_it, 6, flags=L.WRITE_POST | L.WRITE_HARD | L.QUANTIZE |
+                            (L.PLAIN if plain else 0), d_edge=torch.empty((B, dec.required_edge_buffer()),
+                                                                           dtype=torch.int8, device="cuda"))
+        torch.cuda.synchronize()
+        post8, hard8, its8 = d_post8.cpu().numpy(), d_hard.cpu().numpy(), d_it.cpu().numpy()
+        for b in range(B):
+            ri, ra, _, rh = oracle.decode_i8(code, 0.5, oracle.quantize_i16(llr[b], 8.0), 6)
+            assert its8[b] == ri and np.array_equal(post8[b], ra) and np.array_equal(hard8[b], rh)
+
+
+def test_workspace_path_equals_raw_path(oracle):
+    from syndrome_amd import QcLdpcDecoder
+    say("[workspace] decode_with_workspace == decode_layered_offset_min_sum (BG1 Z=96)")
+    code = oracle.code_3gpp(0, 96)
+    dec = QcLdpcDecoder.with_lifting_size(0, 96, 0.5)
+    cw = codewords(oracle, code, 1, seed=21)[0]
+    llr = channel(oracle, code, cw[None, :], [1.0], seed=4)[0]
+    a = llr.copy()
+    er = np.zeros(dec.required_edge_buffer(), np.float32)
+    sc = np.zeros(dec.required_layer_buffer(), np.float32)
+    hd = np.zeros(dec.variable_node_count(), np.uint8)
+    it_raw = dec.decode_layered_offset_min_sum(a, er, sc, hd, 8)
+    b = llr.copy()
+    ws = dec.workspace()
+    it_ws = dec.decode_with_workspace(b, ws, 8)
+    assert it_raw == it_ws and np.array_equal(f32_bits(a), f32_bits(b))
+    assert np.array_equal(hd, ws.hard_output()) and np.array_equal(f32_bits(er), f32_bits(ws.edge_r))
+
+
+def test_concurrent_decodes_share_one_handle(oracle):
+    from syndrome_amd import QcLdpcDecoder
+    say("[threads] 4 threads x 5 decode_5g calls on one BG2 Z=104 handle")
+    code = oracle.code_3gpp(1, 104)
+    dec = QcLdpcDecoder.with_lifting_size(1, 104, 0.5)
+    cws = codewords(oracle, code, 20, seed=33)
+    llrs = channel(oracle, code, cws, list(np.linspace(0.0, 2.5, 20)), seed=8)
+    results = [None] * 20
+    errors = []
+
+    def work(tid):
+        try:
+            for j in range(tid, 20, 4):
+                a = llrs[j].copy()
+                er = np.zeros(dec.required_edge_buffer(), np.float32)
+                sc = np.zeros(dec.required_layer_buffer(), np.float32)
+                hd = np.zeros(dec.variable_node_count(), np.uint8)
+                used = dec.decode_5g(a, 0, er, sc, hd, 8)
+                results[j] = (used, a, er, hd)
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=60)
+    assert not errors and all(r is not None for r in results)
+    for j in range(20):
+        ri, rp, re, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, llrs[j], 0), 8)
+        used, a, er, hd = results[j]
+        assert used == ri and np.array_equal(f32_bits(a), f32_bits(rp)), f"decode {j}"
+        assert np.array_equal(f32_bits(er), f32_bits(re)) and np.array_equal(hd, rh), f"decode {j}"
