@@ -85,6 +85,23 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi)
     return t < hi ? t : hi;
 }
 
+// The median of three, which is clamp(x, lo, hi) whenever lo <= hi: one
+// v_med3 where the backend would otherwise emit a max and a min (it cannot
+// see that the running minimum never exceeds the running second minimum).
+__device__ __forceinline__ uint32_t med3u(uint32_t x, uint32_t lo, uint32_t hi)
+{
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+    return r;
+}
+
+__device__ __forceinline__ int med3i(int x, int lo, int hi)
+{
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+    return r;
+}
+
 template <int I, int N>
 struct SFor {
     template <class F>
