@@ -280,6 +280,37 @@ int32_t syndrome_server_submit_from(SyndromeServer *server, uint32_t slot, const
 /* One completed code block, or SYNDROME_BUSY when none is ready. */
 int32_t syndrome_server_try_recv(SyndromeServer *server, SyndromeServed *out);
 int32_t syndrome_server_release(SyndromeServer *server, uint32_t slot);
+
+/* Transport blocks through the resident server (streams == 0): the end of
+ * DlSchDecoder::decode (transport_block.rs:963-1044) on device.  The caller
+ * acquires num_cb slots of one code and n_filler, fills (or will point)
+ * their inputs, then submits them together, in code-block order, with the
+ * transport block's payload size and a device-accessible destination of
+ * tb_size bytes.  Each block, once decoded, checks its CRC-24B (num_cb > 1)
+ * and folds its share of the CRC-24A into the transport block's state
+ * straight from its posterior in LDS, and writes its payload bits (one per
+ * byte) to tb_out; the last block decides the transport block.  tb_out
+ * holds the payload whatever the outcome: it is the transport block only
+ * when crc_ok (the reference leaves its output untouched otherwise).
+ * Block slots of a submitted transport block are not delivered by
+ * try_recv; try_recv_tb reports the transport block once it is decided and
+ * all its blocks are complete, and release_tb frees its slots.  K' = K -
+ * n_filler; each block carries P = K' - 24 payload bits (K' when alone);
+ * num_cb * P >= tb_size + 24 is required.  SYNDROME_BUSY when 4096
+ * transport blocks are in flight. */
+typedef struct {
+    uint32_t tb;              /* handle for syndrome_server_release_tb */
+    uint64_t tag;
+    int32_t crc_ok;           /* every CRC-24B and the CRC-24A passed */
+    int32_t max_iterations;   /* over its blocks */
+    size_t tb_size, num_cb;
+    double submit_ns, done_ns;
+} SyndromeServedTb;
+
+int32_t syndrome_server_submit_tb(SyndromeServer *server, const uint32_t *slots, size_t num_cb, size_t tb_size,
+                                  uint8_t *tb_out, uint64_t tag);
+int32_t syndrome_server_try_recv_tb(SyndromeServer *server, SyndromeServedTb *out);
+int32_t syndrome_server_release_tb(SyndromeServer *server, uint32_t tb);
 int32_t syndrome_server_stats(const SyndromeServer *server, int64_t *out_batches, int64_t *out_decoded,
                               int32_t *out_slots);
 /* Pinned, device-mapped host memory (usable as a submit_from source). */

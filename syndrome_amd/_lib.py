@@ -82,6 +82,9 @@ SIGNATURES = {
     "syndrome_server_submit": (I32, [P, C.c_uint32, C.c_uint64]),
     "syndrome_server_try_recv": (I32, [P, P]),
     "syndrome_server_release": (I32, [P, C.c_uint32]),
+    "syndrome_server_submit_tb": (I32, [P, P, SZ, SZ, P, C.c_uint64]),
+    "syndrome_server_try_recv_tb": (I32, [P, P]),
+    "syndrome_server_release_tb": (I32, [P, C.c_uint32]),
     "syndrome_server_stats": (I32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(I32)]),
     "syndrome_server_submit_from": (I32, [P, C.c_uint32, P, C.c_uint64]),
     "syndrome_host_alloc": (P, [SZ]),
@@ -95,6 +98,12 @@ SIGNATURES = {
 class Served(C.Structure):
     _fields_ = [("slot", C.c_uint32), ("tag", C.c_uint64), ("iterations", C.c_int32),
                 ("hard", C.POINTER(C.c_uint8)), ("n", C.c_size_t), ("submit_ns", C.c_double),
+                ("done_ns", C.c_double)]
+
+
+class ServedTb(C.Structure):
+    _fields_ = [("tb", C.c_uint32), ("tag", C.c_uint64), ("crc_ok", C.c_int32), ("max_iterations", C.c_int32),
+                ("tb_size", C.c_size_t), ("num_cb", C.c_size_t), ("submit_ns", C.c_double),
                 ("done_ns", C.c_double)]
 
 _lib = None

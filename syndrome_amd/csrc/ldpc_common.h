@@ -95,6 +95,34 @@ struct SynWork {
     int slot[SYN_WORK_MAX];          // output slot of each block
     int nfill[SYN_WORK_MAX];         // its filler count
     const void *src[SYN_WORK_MAX];   // its decoder input (HBM or mapped host memory)
+    int tb[SYN_WORK_MAX];            // its transport block's ring index, or -1
+    int tb_ci[SYN_WORK_MAX];         // its index within that transport block
+};
+
+// A transport block of the resident server (ring in host memory): num_cb
+// blocks of P payload bits each carry tb_size payload bits then CRC-24A;
+// each block carries its CRC-24B after its P bits when num_cb > 1.
+struct SynTbWork {
+    unsigned id;     // submission number; its done word becomes id + 1
+    int num_cb;
+    int p;
+    int tb_size;
+    unsigned char *out;  // payload destination, one bit per byte (device-accessible)
+};
+
+// Device-side fold of one transport block's checks (device memory).
+struct SynTbState {
+    unsigned acc;    // XOR of the blocks' contributions to the CRC-24A remainder
+    unsigned rx;     // the received CRC-24A bits, OR-ed in by the blocks holding them
+    unsigned bad;    // some block failed its CRC-24B
+    unsigned count;  // blocks folded so far
+};
+
+struct SynResidentTb {
+    const SynTbWork *work;  // ring (host memory)
+    SynTbState *state;      // per ring index (device memory)
+    unsigned *done;         // per ring index: id + 1 once decided (host memory)
+    unsigned *ok;           // per ring index: 1 if every check passed (host memory)
 };
 
 struct SynResidentCtl {

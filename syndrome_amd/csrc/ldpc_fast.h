@@ -253,7 +253,7 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo,
         qs = __builtin_amdgcn_alignbit(qs, qb, 31);           // shift in q's sign
         const uint32_t a = qb & 0x7fffffffu;
         am = (a <= m1) ? (uint32_t)e : am;
-        m2 = med3u(a, m1, m2);  // min(m2, max(m1, a)), as m1 <= m2 always
+        m2 = clampu(a, m1, m2);  // min(m2, max(m1, a)) as m1 <= m2
         m1 = a < m1 ? a : m1;
     });
     float f1 = __uint_as_float(m1), f2 = __uint_as_float(m2);

@@ -22,9 +22,75 @@
 // down together), once the kernel is older than `life_ticks`, or when the
 // host sets ctl->stop: every wait ends, and the host relaunches the grid
 // when work is published after it has left.
+#include "ldpc_crc.h"
 #include "ldpc_fast.h"
 
 namespace syn {
+
+// One decoded code block's share of its transport block's checks, from its
+// posterior still in LDS (hard bit i = posterior[i] < 0), by the whole
+// workgroup:
+//   * its CRC-24B over its first P bits against the next 24 (when the
+//     transport block has more than one block);
+//   * the CRC-24A remainder of its slice [ci*P, ci*P + L) of the payload
+//     stream (L = its bits below tb_size), shifted by x^(tb_size - slice end)
+//     so that the XOR of all blocks' shares is the stream's remainder;
+//   * the received CRC-24A bits that fall in its P bits;
+//   * its payload bits written to the transport block's output.
+// Lane 0 folds these into the transport block's device state; the block
+// that completes the count decides the transport block and publishes
+// ok[] and done[] (the payload writes of every block are fenced at system
+// scope before that block's count increment).
+__device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, const float *post)
+{
+    __shared__ uint32_t red[512];
+    __shared__ unsigned s_id;
+    __shared__ int s_c, s_p, s_a;
+    __shared__ unsigned char *s_out;
+    if (threadIdx.x == 0) {
+        const SynTbWork *tw = tbs.work + ti;
+        s_id = tw->id;
+        s_c = tw->num_cb;
+        s_p = tw->p;
+        s_a = tw->tb_size;
+        s_out = tw->out;
+    }
+    __syncthreads();
+    const int c = s_c, p = s_p, a = s_a;
+    auto bit = [&](long long i) -> uint32_t { return post[i] < 0.f ? 1u : 0u; };
+    bool cb_ok = true;
+    if (c > 1) cb_ok = group_crc24(bit, p, CRC24B_POLY, red) == received24(bit, p);
+    const long long s0 = (long long)ci * p;
+    const long long len = a - s0 < 0 ? 0 : (a - s0 > p ? p : a - s0);
+    const uint32_t part = len > 0 ? group_crc24(bit, len, CRC24A_POLY, red) : 0u;
+    for (long long u = threadIdx.x; u < len; u += blockDim.x) s_out[s0 + u] = (unsigned char)bit(u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t share = crc24_mulmod(part, crc24_xpow(a - (s0 + len), CRC24A_POLY), CRC24A_POLY);
+        uint32_t rx = 0;
+        for (int i = 0; i < 24; ++i) {
+            const long long t = (long long)a + i;
+            if (t >= s0 && t < s0 + p) rx |= bit(t - s0) << (23 - i);
+        }
+        SynTbState *st = tbs.state + ti;
+        __threadfence_system();  // this block's payload bits are visible before it is counted
+        atomicXor(&st->acc, share);
+        atomicOr(&st->rx, rx);
+        if (!cb_ok) atomicOr(&st->bad, 1u);
+        __threadfence();
+        if (atomicAdd(&st->count, 1u) == (unsigned)(c - 1)) {
+            __threadfence();
+            const unsigned acc = atomicOr(&st->acc, 0u), rxv = atomicOr(&st->rx, 0u), bad = atomicOr(&st->bad, 0u);
+            atomicExch(&st->acc, 0u);  // ready for the ring index's next transport block
+            atomicExch(&st->rx, 0u);
+            atomicExch(&st->bad, 0u);
+            atomicExch(&st->count, 0u);
+            tbs.ok[ti] = (!bad && acc == rxv) ? 1u : 0u;
+            __hip_atomic_store(tbs.done + ti, s_id + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+    __syncthreads();
+}
 
 // Device-side queue words (one 64-byte block in device memory): the claim
 // counter, the device's copy of the host's tail and stop words, and the
@@ -38,11 +104,12 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
                                                         const SynWork *__restrict__ ring, unsigned cap,
                                                         SynResidentCtl *ctl, unsigned *done, ResidentQueue *q,
                                                         long long *last_claim, const long long *__restrict__ offs,
-                                                        SynLaunch a, long long idle_ticks, long long life_ticks)
+                                                        SynLaunch a, long long idle_ticks, long long life_ticks,
+                                                        SynResidentTb tbs)
 {
     __shared__ unsigned s_w;
-    __shared__ int s_have, s_code, s_count, s_aligned;
-    __shared__ int s_slots[SYN_WORK_MAX], s_nf[SYN_WORK_MAX];
+    __shared__ int s_have, s_code, s_count, s_aligned, s_any_tb;
+    __shared__ int s_slots[SYN_WORK_MAX], s_nf[SYN_WORK_MAX], s_tb[SYN_WORK_MAX], s_tbci[SYN_WORK_MAX];
     __shared__ const void *s_src[SYN_WORK_MAX];
     const long long t0 = wall_clock64();
     for (;;) {
@@ -99,11 +166,16 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         __syncthreads();
         if (!s_have) return;
         const unsigned w = s_w;
+        if (threadIdx.x == 0) s_any_tb = 0;
+        __syncthreads();
         if ((int)threadIdx.x < s_count) {
             const SynWork *wk = ring + w % cap;
             s_slots[threadIdx.x] = wk->slot[threadIdx.x];
             s_nf[threadIdx.x] = wk->nfill[threadIdx.x];
             s_src[threadIdx.x] = wk->src[threadIdx.x];
+            s_tb[threadIdx.x] = wk->tb[threadIdx.x];
+            s_tbci[threadIdx.x] = wk->tb_ci[threadIdx.x];
+            if (s_tb[threadIdx.x] >= 0) s_any_tb = 1;
         }
         __syncthreads();
         SynLaunch la = a;
@@ -116,6 +188,14 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         } else {
             if (code->z == 208) fast_group<SYN_KIND_BG2, 208, false>(code, 208, s_count, io, la);
             else fast_group<SYN_KIND_BG2, 0, false>(code, code->z, s_count, io, la);
+        }
+        if (s_any_tb) {
+            __syncthreads();  // the decode is finished with the posterior in LDS
+            const int pad = fast_pad(code->z, 4);
+            const int n = code->n_b * code->z;
+            extern __shared__ __align__(16) unsigned char smem[];
+            for (int j = 0; j < s_count; ++j)
+                if (s_tb[j] >= 0) tb_fold(tbs, s_tb[j], s_tbci[j], (const float *)(smem + pad) + (size_t)j * n);
         }
         // Every lane's hard decisions and counts happen before the barrier;
         // the system-scope release by lane 0 then publishes all of them

@@ -51,6 +51,7 @@ constexpr int NMAX = 68 * 384;
 // the kernel's without a kernel boundary in between.
 constexpr unsigned HOST_FLAGS = hipHostMallocMapped | hipHostMallocCoherent;
 constexpr unsigned RING_ITEMS = 8192;
+constexpr unsigned TB_RING = 4096;                  // transport blocks in flight (resident mode)
 constexpr double RESIDENT_IDLE_US = 2000.0;     // grid leaves after this long with nothing claimed
 constexpr double RESIDENT_LIFE_S = 600.0;       // and never waits beyond this age
 constexpr int VALID_Z[51] = {2,  3,  4,  5,  6,  7,  8,  9,  10,  11,  12,  13,  14,  15,  16,  18,  20,
@@ -148,6 +149,7 @@ struct SyndromeServer {
     struct Slot {
         const float *src = nullptr;  // device-accessible input
         int code = -1, n = 0, nfill = 0;
+        int tb = -1, tb_ci = 0;      // transport block (resident mode) and index in it
         uint64_t tag = 0;
         double t_submit = 0, t_done = 0;
         int state = 0;  // 0 free, 1 acquired, 2 submitted, 3 on gpu, 4 done, 5 delivered
@@ -186,6 +188,22 @@ struct SyndromeServer {
         int grid = 0;
         size_t lds = 0;
         long long idle_ticks = 0, life_ticks = 0;
+        // transport blocks (submit_tb): ring in coherent pinned memory,
+        // fold state in device memory
+        unsigned tb_cap = 0, tb_next = 0;
+        SynTbWork *tbw = nullptr, *tbw_dev = nullptr;
+        SynTbState *d_tbstate = nullptr;
+        unsigned *tb_done = nullptr, *tb_done_dev = nullptr, *tb_ok = nullptr, *tb_ok_dev = nullptr;
+        struct TbRec {
+            std::vector<int> slots;
+            uint64_t tag = 0;
+            size_t tb_size = 0;
+            double t_submit = 0, t_done = 0;
+            int state = 0;  // 0 free, 2 submitted, 4 done, 5 delivered
+        };
+        std::vector<TbRec> tbr;
+        std::vector<int> tb_free, tb_outstanding;
+        std::deque<int> tb_done_q;
     } rs;
 
     int32_t launch_batch(Lane &ln, std::vector<int> &batch);
@@ -410,6 +428,21 @@ int32_t SyndromeServer::init_resident()
     memset(rs.ring, 0, sizeof(SynWork) * rs.cap);
     memset(rs.ctl, 0, sizeof(SynResidentCtl));
     memset(rs.done, 0, 4 * (size_t)rs.cap);
+    rs.tb_cap = TB_RING;
+    if (hipHostMalloc((void **)&rs.tbw, sizeof(SynTbWork) * rs.tb_cap, HOST_FLAGS) != hipSuccess ||
+        hipHostMalloc((void **)&rs.tb_done, 4 * (size_t)rs.tb_cap, HOST_FLAGS) != hipSuccess ||
+        hipHostMalloc((void **)&rs.tb_ok, 4 * (size_t)rs.tb_cap, HOST_FLAGS) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rs.tbw_dev, rs.tbw, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rs.tb_done_dev, rs.tb_done, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rs.tb_ok_dev, rs.tb_ok, 0) != hipSuccess ||
+        hipMalloc((void **)&rs.d_tbstate, sizeof(SynTbState) * rs.tb_cap) != hipSuccess ||
+        hipMemset(rs.d_tbstate, 0, sizeof(SynTbState) * rs.tb_cap) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    memset(rs.tbw, 0, sizeof(SynTbWork) * rs.tb_cap);
+    memset(rs.tb_done, 0, 4 * (size_t)rs.tb_cap);
+    memset(rs.tb_ok, 0, 4 * (size_t)rs.tb_cap);
+    rs.tbr.resize(rs.tb_cap);
+    for (int i = (int)rs.tb_cap - 1; i >= 0; --i) rs.tb_free.push_back(i);
     return SYNDROME_OK;
 }
 
@@ -430,7 +463,8 @@ int32_t SyndromeServer::launch_resident()
     long long *last = rs.d_last;
     const long long *offs = rs.d_offs;
     long long idle = rs.idle_ticks, life = rs.life_ticks;
-    void *args[] = {&cp, &ring, &cap, &ctl, &dn, &queue, &last, &offs, &a, &idle, &life};
+    SynResidentTb tbs{rs.tbw_dev, rs.d_tbstate, rs.tb_done_dev, rs.tb_ok_dev};
+    void *args[] = {&cp, &ring, &cap, &ctl, &dn, &queue, &last, &offs, &a, &idle, &life, &tbs};
     if (hipLaunchKernel(syn::resident_kernel(), dim3(rs.grid), dim3(512), args, rs.lds, rs.stream) != hipSuccess ||
         hipEventRecord(rs.ev, rs.stream) != hipSuccess)
         return SYNDROME_DEVICE_ERROR;
@@ -462,7 +496,7 @@ void SyndromeServer::run_resident()
                 for (int j = 0; j < o.n; ++j) {
                     st[o.slot[j]].state = 4;
                     st[o.slot[j]].t_done = t;
-                    done.push_back(o.slot[j]);
+                    if (st[o.slot[j]].tb < 0) done.push_back(o.slot[j]);  // TB blocks are delivered per TB
                 }
                 decoded += o.n;
                 rs.outstanding[k] = rs.outstanding.back();
@@ -470,6 +504,24 @@ void SyndromeServer::run_resident()
                 progressed = true;
             } else {
                 ++k;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (size_t k = 0; k < rs.tb_outstanding.size();) {
+                const int ti = rs.tb_outstanding[k];
+                bool blocks_done = true;  // and every block's item seen complete, so release_tb can free them
+                for (int sl : rs.tbr[ti].slots) blocks_done = blocks_done && st[sl].state == 4;
+                if (blocks_done && __atomic_load_n(rs.tb_done + ti, __ATOMIC_ACQUIRE) == rs.tbw[ti].id + 1u) {
+                    rs.tbr[ti].state = 4;
+                    rs.tbr[ti].t_done = now_ns();
+                    rs.tb_done_q.push_back(ti);
+                    rs.tb_outstanding[k] = rs.tb_outstanding.back();
+                    rs.tb_outstanding.pop_back();
+                    progressed = true;
+                } else {
+                    ++k;
+                }
             }
         }
         pend.clear();
@@ -499,6 +551,8 @@ void SyndromeServer::run_resident()
                         wk.slot[q] = v[j + q];
                         wk.nfill[q] = x.nfill;
                         wk.src[q] = x.src;
+                        wk.tb[q] = x.tb;
+                        wk.tb_ci[q] = x.tb_ci;
                         aligned &= ((uintptr_t)x.src & 15u) == 0;
                         o.slot[q] = v[j + q];
                     }
@@ -639,9 +693,9 @@ void syndrome_server_destroy(SyndromeServer *s)
         if (r.stream) (void)hipStreamSynchronize(r.stream);
         if (r.ev) (void)hipEventDestroy(r.ev);
         if (r.stream) (void)hipStreamDestroy(r.stream);
-        for (void *p : {(void *)r.ring, (void *)r.ctl, (void *)r.done})
+        for (void *p : {(void *)r.ring, (void *)r.ctl, (void *)r.done, (void *)r.tbw, (void *)r.tb_done, (void *)r.tb_ok})
             if (p) (void)hipHostFree(p);
-        for (void *p : {(void *)r.d_head, (void *)r.d_last, (void *)r.d_offs})
+        for (void *p : {(void *)r.d_head, (void *)r.d_last, (void *)r.d_offs, (void *)r.d_tbstate})
             if (p) (void)hipFree(p);
         for (void *p : {(void *)s->d_llr, (void *)s->d_codes})
             if (p) (void)hipFree(p);
@@ -732,10 +786,98 @@ int32_t syndrome_server_release(SyndromeServer *s, uint32_t slot)
     return guard([&]() -> int32_t {
         if (!s) return SYNDROME_NULL_POINTER;
         std::lock_guard<std::mutex> lk(s->mu);
-        if (slot >= (uint32_t)s->slots || (s->st[slot].state != 5 && s->st[slot].state != 1))
+        if (slot >= (uint32_t)s->slots || (s->st[slot].state != 5 && s->st[slot].state != 1) || s->st[slot].tb >= 0)
             return SYNDROME_INVALID_PARAM;
         s->st[slot].state = 0;
         s->free_list.push_back((int)slot);
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_submit_tb(SyndromeServer *s, const uint32_t *slots, size_t num_cb, size_t tb_size,
+                                  uint8_t *tb_out, uint64_t tag)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !slots || !tb_out) return SYNDROME_NULL_POINTER;
+        if (!s->resident || num_cb == 0 || num_cb > (size_t)s->slots || tb_size == 0 || tb_size > (size_t)INT_MAX)
+            return SYNDROME_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(s->mu);
+        const SyndromeServer::Slot &f = s->st[slots[0] < (uint32_t)s->slots ? slots[0] : 0];
+        for (size_t i = 0; i < num_cb; ++i) {
+            if (slots[i] >= (uint32_t)s->slots) return SYNDROME_INVALID_PARAM;
+            const SyndromeServer::Slot &x = s->st[slots[i]];
+            if (x.state != 1 || x.code != f.code || x.nfill != f.nfill) return SYNDROME_INVALID_PARAM;
+            for (size_t j = 0; j < i; ++j)
+                if (slots[j] == slots[i]) return SYNDROME_INVALID_PARAM;
+        }
+        const SynCode &c = s->codes[f.code];
+        const long long kp = (long long)(c.n_b - c.m_b) * c.z - f.nfill;
+        const long long p = num_cb > 1 ? kp - 24 : kp;
+        if (p < 1 || (long long)num_cb * p < (long long)tb_size + 24) return SYNDROME_INVALID_PARAM;
+        if (s->rs.tb_free.empty()) return SYNDROME_BUSY;
+        const int ti = s->rs.tb_free.back();
+        s->rs.tb_free.pop_back();
+        const unsigned id = s->rs.tb_next++;
+        s->rs.tbw[ti] = SynTbWork{id, (int)num_cb, (int)p, (int)tb_size, tb_out};
+        SyndromeServer::Resident::TbRec &r = s->rs.tbr[ti];
+        r.slots.assign(slots, slots + num_cb);
+        r.tag = tag;
+        r.tb_size = tb_size;
+        r.t_submit = now_ns();
+        r.state = 2;
+        for (size_t i = 0; i < num_cb; ++i) {
+            SyndromeServer::Slot &x = s->st[slots[i]];
+            x.tb = ti;
+            x.tb_ci = (int)i;
+            x.tag = tag;
+            x.t_submit = r.t_submit;
+            x.state = 2;
+            s->submitted.push_back((int)slots[i]);
+        }
+        s->rs.tb_outstanding.push_back(ti);
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_try_recv_tb(SyndromeServer *s, SyndromeServedTb *out)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !out) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->rs.tb_done_q.empty()) return SYNDROME_BUSY;
+        const int ti = s->rs.tb_done_q.front();
+        s->rs.tb_done_q.pop_front();
+        SyndromeServer::Resident::TbRec &r = s->rs.tbr[ti];
+        r.state = 5;
+        int its = 0;
+        for (int sl : r.slots) its = std::max(its, s->h_it[sl]);
+        out->tb = (uint32_t)ti;
+        out->tag = r.tag;
+        out->crc_ok = __atomic_load_n(s->rs.tb_ok + ti, __ATOMIC_ACQUIRE) ? 1 : 0;
+        out->max_iterations = its;
+        out->tb_size = r.tb_size;
+        out->num_cb = r.slots.size();
+        out->submit_ns = r.t_submit;
+        out->done_ns = r.t_done;
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_release_tb(SyndromeServer *s, uint32_t tb)
+{
+    return guard([&]() -> int32_t {
+        if (!s) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->resident || tb >= s->rs.tb_cap || s->rs.tbr[tb].state != 5) return SYNDROME_INVALID_PARAM;
+        SyndromeServer::Resident::TbRec &r = s->rs.tbr[tb];
+        for (int sl : r.slots) {
+            s->st[sl].tb = -1;
+            s->st[sl].state = 0;
+            s->free_list.push_back(sl);
+        }
+        r.slots.clear();
+        r.state = 0;
+        s->rs.tb_free.push_back((int)tb);
         return SYNDROME_OK;
     });
 }

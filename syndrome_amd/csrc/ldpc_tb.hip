@@ -3,46 +3,19 @@
 // (DlSchDecoder::decode after the code-block loop, src/transport_block.rs:
 // 963-1044; CRC per src/crc.rs).
 //
-// CRCs here are the reference's: MSB-first register, initial value 0, no
-// reflection, no final XOR, remainder of M(x) * x^24 mod g(x) with
-// g = 0x864CFB (24A) or 0x800063 (24B).  A workgroup computes one CRC over
-// L bits by giving each of its T threads one W = ceil(L / T)-bit chunk of
-// the message left-padded with zeros to T * W bits (leading zeros leave the
-// remainder unchanged), then folding the T chunk remainders pairwise:
-// crc(A || B) = crc(A) * x^len(B) + crc(B) mod g, with x^(W * 2^l) mod g
-// prepared once per fold level.
+// CRCs per ldpc_crc.h.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
 
 #include "ldpc_common.h"
+#include "ldpc_crc.h"
 
 namespace syn {
 
 namespace {
 
 constexpr int TB_THREADS = 256;
-constexpr uint32_t CRC24A = 0x864CFBu, CRC24B = 0x800063u, MASK24 = 0xFFFFFFu;
-
-__device__ __forceinline__ uint32_t crc_step(uint32_t reg, uint32_t bit, uint32_t poly)
-{
-    const uint32_t fb = ((reg >> 23) ^ bit) & 1u;
-    reg = (reg << 1) & MASK24;
-    return fb ? reg ^ poly : reg;
-}
-
-// a * b mod g for 24-bit residues (carry-less, Horner over a's bits)
-__device__ __forceinline__ uint32_t mulmod(uint32_t a, uint32_t b, uint32_t poly)
-{
-    uint32_t r = 0;
-    for (int i = 23; i >= 0; --i) {
-        const uint32_t top = (r >> 23) & 1u;
-        r = (r << 1) & MASK24;
-        if (top) r ^= poly;
-        if ((a >> i) & 1u) r ^= b;
-    }
-    return r;
-}
 
 // Where bit t of a TB's reassembled payload stream lives: block t / P at
 // offset t % P (every block contributes P bits, transport_block.rs:566).
@@ -51,48 +24,12 @@ struct Stream {
     const long long *offs;  // per code block of the plan
     int first, p;
 
-    __device__ __forceinline__ uint32_t bit(long long t) const
+    __device__ __forceinline__ uint32_t operator()(long long t) const
     {
         const int ci = (int)(t / p);
         return hard[offs[first + ci] + (t - (long long)ci * p)] & 1u;
     }
 };
-
-// CRC over bits [0, len) of `s` by the whole workgroup; every thread returns
-// the remainder.  `red` is TB_THREADS words of LDS.
-__device__ uint32_t group_crc(const Stream &s, long long len, uint32_t poly, uint32_t *red)
-{
-    const int t = threadIdx.x;
-    const long long w = (len + TB_THREADS - 1) / TB_THREADS;
-    const long long pad = w * TB_THREADS - len;
-    uint32_t reg = 0;
-    long long b0 = t * w - pad;
-    for (long long i = 0; i < w; ++i) {
-        const long long b = b0 + i;
-        reg = crc_step(reg, b >= 0 ? s.bit(b) : 0u, poly);
-    }
-    red[t] = reg;
-    // x^w mod g, then squared per fold level
-    uint32_t xl = 1;
-    for (long long i = 0; i < w; ++i) xl = crc_step(xl, 0u, poly);
-    __syncthreads();
-    for (int stride = 1; stride < TB_THREADS; stride *= 2) {
-        if (t % (2 * stride) == 0) red[t] = mulmod(red[t], xl, poly) ^ red[t + stride];
-        xl = mulmod(xl, xl, poly);
-        __syncthreads();
-    }
-    const uint32_t r = red[0];
-    __syncthreads();
-    return r;
-}
-
-// the 24 bits at [at, at + 24) of `s`, MSB first
-__device__ uint32_t received24(const Stream &s, long long at)
-{
-    uint32_t v = 0;
-    for (int i = 0; i < 24; ++i) v = (v << 1) | s.bit(at + i);
-    return v;
-}
 
 __global__ void __launch_bounds__(TB_THREADS) tb_check(const uint8_t *__restrict__ hard,
                                                        const long long *__restrict__ offs,
@@ -113,7 +50,7 @@ __global__ void __launch_bounds__(TB_THREADS) tb_check(const uint8_t *__restrict
             bool ok = true;
             if (seg) {
                 const Stream cb{hard, offs, tb.first_cb + ci, tb.k_prime};
-                const uint32_t r = group_crc(cb, tb.k_prime - 24, CRC24B, red);
+                const uint32_t r = group_crc24(cb, tb.k_prime - 24, CRC24B_POLY, red);
                 ok = r == received24(cb, tb.k_prime - 24);
             }
             if (threadIdx.x == 0) {
@@ -123,10 +60,10 @@ __global__ void __launch_bounds__(TB_THREADS) tb_check(const uint8_t *__restrict
         }
         // CRC-24A of the reassembled transport block
         const Stream s{hard, offs, tb.first_cb, p};
-        const uint32_t r = group_crc(s, tb.tb_size, CRC24A, red);
+        const uint32_t r = group_crc24(s, tb.tb_size, CRC24A_POLY, red);
         const bool ok = r == received24(s, tb.tb_size) && s_all;
         if (ok)  // the payload is delivered only when every check passed (transport_block.rs:1036-1040)
-            for (long long t = threadIdx.x; t < tb.tb_size; t += TB_THREADS) tb_out[tb.out_off + t] = (uint8_t)s.bit(t);
+            for (long long t = threadIdx.x; t < tb.tb_size; t += TB_THREADS) tb_out[tb.out_off + t] = (uint8_t)s(t);
         if (threadIdx.x == 0) tb_ok[ti] = ok ? 1 : 0;
         __syncthreads();
     }

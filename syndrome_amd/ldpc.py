@@ -415,6 +415,34 @@ class Server:
     def release(self, slot):
         check(lib().syndrome_server_release(self._h, slot), "server_release")
 
+    def submit_tb(self, slots, tb_size, tb_out, tag):
+        """Submit acquired, filled slots of one code as one transport block
+        (syndrome_server_submit_tb).  tb_out: a HostPool of uint8, a device
+        uint8 tensor, or a device-accessible address, of at least tb_size
+        bytes.  Returns False when busy."""
+        sl = np.ascontiguousarray(slots, np.uint32)
+        if isinstance(tb_out, int):
+            out = tb_out
+        else:
+            out = tb_out.ptr if isinstance(tb_out, HostPool) else tb_out.data_ptr()
+        rc = lib().syndrome_server_submit_tb(self._h, _ptr(sl), sl.size, tb_size, out, tag)
+        if rc == _lib.BUSY:
+            return False
+        check(rc, "server_submit_tb")
+        return True
+
+    def try_recv_tb(self):
+        """Returns (tb, tag, crc_ok, max_iterations, submit_ns, done_ns) or None."""
+        r = _lib.ServedTb()
+        rc = lib().syndrome_server_try_recv_tb(self._h, C.byref(r))
+        if rc == _lib.BUSY:
+            return None
+        check(rc, "server_try_recv_tb")
+        return r.tb, r.tag, bool(r.crc_ok), r.max_iterations, r.submit_ns, r.done_ns
+
+    def release_tb(self, tb):
+        check(lib().syndrome_server_release_tb(self._h, tb), "server_release_tb")
+
     def stats(self):
         b, d, s = C.c_int64(), C.c_int64(), C.c_int32()
         check(lib().syndrome_server_stats(self._h, C.byref(b), C.byref(d), C.byref(s)), "server_stats")

@@ -170,3 +170,76 @@ def test_loadgen_accounts_every_block(streams):
     st = srv.stats()
     assert st["decoded"] >= len(cbl)
     srv.close()
+
+
+def test_server_transport_blocks_match_oracle(oracle):
+    """Transport blocks through the resident server: each block's CRC-24B,
+    the CRC-24A and the reassembled payload are decided on the device from
+    the decoded posteriors; they equal the oracle's decode and TB check of
+    the same blocks, and plain code blocks submitted in between are
+    delivered as before."""
+    import syndrome_amd
+    from syndrome_amd import HostPool, Server
+    from tests.test_gpu_tb import TBS, _build
+    syndrome_amd.lib()
+    cbs, tbs = _build(oracle)
+    say(f"[serve tb] {len(tbs)} transport blocks ({len(cbs)} code blocks) + loose blocks, resident mode")
+    inputs = []
+    for bg, z, nf, e_llr in cbs:
+        cb = oracle.rate_dematch(e_llr, 0, 1, bg, z, nf, cached=True)
+        inputs.append(oracle.decoder_input_from_cb(cb, z))
+    total = int(sum(t[2] for t in tbs))
+    out = HostPool(total, np.uint8)
+    out.array[:] = 9
+    out_off = np.concatenate([[0], np.cumsum([t[2] for t in tbs])[:-1]]).astype(np.int64)
+    loose = _blocks(oracle, 12, 5)
+    srv = Server(0.5, 8, hbm_budget_bytes=256 << 20, max_batch=64, streams=0)
+    for ti, (first, c, a, kp, tb) in enumerate(tbs):
+        slots = []
+        for ci in range(c):
+            bg, z, nf, _ = cbs[first + ci]
+            got = srv.acquire(bg, z, nf)
+            assert got is not None
+            slots.append(got[0])
+            got[1][:] = inputs[first + ci]
+        assert srv.submit_tb(slots, a, out.ptr + int(out_off[ti]), 1000 + ti)
+        if ti < len(loose):  # interleave loose code blocks
+            b, z, nf, llr = loose[ti]
+            slot, buf = srv.acquire(b, z, nf)
+            buf[:] = llr
+            srv.submit(slot, ti)
+    got_tb, got_cb = {}, {}
+    t_end = time.time() + 60
+    while (len(got_tb) < len(tbs) or len(got_cb) < min(len(loose), len(tbs))) and time.time() < t_end:
+        r = srv.try_recv_tb()
+        if r is not None:
+            got_tb[r[1] - 1000] = r
+            srv.release_tb(r[0])
+        r = srv.try_recv()
+        if r is not None:
+            got_cb[r[1]] = r
+            srv.release(r[0])
+    assert len(got_tb) == len(tbs), f"{len(got_tb)} of {len(tbs)} transport blocks"
+    for ti, (first, c, a, kp, tb) in enumerate(tbs):
+        hards, its = [], []
+        for ci in range(c):
+            bg, z, nf, _ = cbs[first + ci]
+            code = oracle.code_3gpp(bg, z)
+            ri, _, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, inputs[first + ci], nf), 8)
+            hards.append(rh)
+            its.append(ri)
+        n = hards[0].size
+        ok, _, _ = oracle.tb_check(np.concatenate(hards), np.arange(c) * n, c, kp, a)
+        _, tag, crc_ok, max_it, _, _ = got_tb[ti]
+        assert crc_ok == ok and max_it == max(its), f"TB {ti}: crc_ok {crc_ok} (oracle {ok}), iterations"
+        p = kp - 24 if c > 1 else kp
+        stream = np.concatenate([h[:p] for h in hards])[:a]
+        assert np.array_equal(out.array[out_off[ti]:out_off[ti] + a], stream), f"TB {ti}: payload bits"
+        if ok:
+            assert np.array_equal(stream, tb)
+    for t, r in got_cb.items():
+        b, z, nf, llr = loose[t]
+        code = oracle.code_3gpp(b, z)
+        ri, _, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, llr, nf), 8)
+        assert r[2] == ri and np.array_equal(r[3], rh), f"loose block {t}"
+    srv.close()
