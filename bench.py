@@ -360,33 +360,37 @@ def bench_uneven(torch, args):
 
 def serving_traffic(torch, args, n_tb=96):
     """The serving workload's pool: n_tb transport blocks (TBS log-uniform
-    in [1e3, 5e5] bits, MCS rate in {1/3,1/2,2/3,3/4,5/6}, Qm 1) segmented
-    into code blocks (compute_segmentation), encoded, rate matched, sent
+    in [1e3, 5e5] bits, MCS rate in {1/3,1/2,2/3,3/4,5/6}, Qm 1): random
+    payload, CRC-24A attached, segmented into code blocks with their
+    CRC-24B (compute_segmentation / segment), encoded, rate matched, sent
     over BPSK/AWGN at 2.0 dB and de-matched on the device into decoder
     inputs.  Returns a dict of the per-block and per-TB arrays and the
     inputs in HBM (d_llr) and in pinned host memory (host_pool)."""
-    from syndrome_amd import DecodePlan, HostPool, traffic_blocks
-    from syndrome_amd.transport import compute_segmentation
+    from syndrome_amd import DecodePlan, HostPool, traffic_blocks_info
+    from syndrome_amd.transport import attach_crc24, compute_segmentation, segment
     rng = np.random.default_rng(args.seed + 5)
-    cb_bg, cb_z, cb_nf, cb_e, cb_kp, tb_first, tb_num, tbs = [], [], [], [], [], [], [], []
+    cb_bg, cb_z, cb_nf, cb_e, cb_kp, tb_first, tb_num, tbs, infos = [], [], [], [], [], [], [], [], []
     for _ in range(n_tb):
         a = int(np.exp(rng.uniform(np.log(1000), np.log(500000))))
         r = float((1 / 3, 1 / 2, 2 / 3, 3 / 4, 5 / 6)[rng.integers(0, 5)])
         sg = compute_segmentation(a, r)
+        blocks = segment(attach_crc24(rng.integers(0, 2, a).astype(np.uint8)), sg)
         tb_first.append(len(cb_bg))
         tb_num.append(sg.c)
         tbs.append(a)
         e = int(round((a + 24) / r / sg.c))
-        for _ in range(sg.c):
+        for i in range(sg.c):
             cb_bg.append(sg.bg)
             cb_z.append(sg.z)
             cb_nf.append(sg.n_filler)
             cb_e.append(max(e, 1))
             cb_kp.append(sg.k_prime)
+            infos.append(blocks[i])
     cb_bg, cb_z, cb_nf, cb_e, cb_kp = map(np.array, (cb_bg, cb_z, cb_nf, cb_e, cb_kp))
     nb = len(cb_bg)
-    e_llr, e_off, _, _ = traffic_blocks(cb_bg, cb_z, cb_nf, cb_e, np.full(nb, 2.0), cb_kp / cb_e, args.seed + 5,
-                                        want_info=False)
+    i_off = np.concatenate([[0], np.cumsum(cb_kp)[:-1]]).astype(np.int64)
+    e_llr, e_off = traffic_blocks_info(cb_bg, cb_z, cb_nf, cb_e, np.full(nb, 2.0), cb_kp / cb_e,
+                                       np.concatenate(infos), i_off, args.seed + 5)
     n = np.where(cb_bg == 0, 68, 52) * cb_z
     offs = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
     plan = DecodePlan(cb_bg, cb_z, cb_nf, offs, args.beta)
@@ -399,9 +403,11 @@ def serving_traffic(torch, args, n_tb=96):
     info_bits = np.zeros(nb, np.int64)
     for t in range(n_tb):
         info_bits[tb_first[t]:tb_first[t] + tb_num[t]] = tbs[t] // tb_num[t]
+    tb_out_off = np.concatenate([[0], np.cumsum(tbs)[:-1]]).astype(np.int64)
     return {"n_tb": n_tb, "nb": nb, "offs": offs, "bg": cb_bg, "z": cb_z, "nf": cb_nf, "info_bits": info_bits,
-            "tb_first": tb_first, "tb_num": tb_num, "mean_tb_bits": float(np.mean(tbs)), "d_llr": d_llr,
-            "host_pool": host_pool}
+            "tb_first": tb_first, "tb_num": tb_num, "tb_size": np.array(tbs), "tb_out_off": tb_out_off,
+            "d_tb_out": torch.zeros(int(sum(tbs)), dtype=torch.uint8, device="cuda"),
+            "mean_tb_bits": float(np.mean(tbs)), "d_llr": d_llr, "host_pool": host_pool}
 
 
 def bench_serving(torch, args, peak_ref_gbps):
@@ -417,7 +423,9 @@ def bench_serving(torch, args, peak_ref_gbps):
     out = {"workload": "Poisson TB arrivals, TBS log-U[1e3,5e5], rate in {1/3..5/6}, BPSK/AWGN 2.0 dB, "
                        f"{n_tb} TBs / {nb} code blocks cycled, HBM budget 1 GiB",
            "modes": {"resident": "one grid stays on the device and pulls code blocks from a queue",
-                     "lanes": "batches of what is pending, launched on 3 streams"}}
+                     "lanes": "batches of what is pending, launched on 3 streams",
+                     "resident_transport_blocks": "whole transport blocks submitted; CRC-24B, CRC-24A and payload "
+                                                  "decided on the device; latency per transport block"}}
 
     def pct(x, q):
         return round(float(np.percentile(x, q)) / 1e6, 4)
@@ -441,6 +449,24 @@ def bench_serving(torch, args, peak_ref_gbps):
                                      "code_blocks": int(len(cbl)), "admission_waits": int(waits)})
             out[f"{mode}_{where}"] = res
         if mode == "resident":
+            # transport blocks end to end: CRC-24B per block, CRC-24A and the
+            # payload decided on the device, latency per transport block
+            from syndrome_amd import serve_loadgen_tb
+            runt = lambda rate, dur: serve_loadgen_tb(srv, d_llr, offs, cb_bg, cb_z, cb_nf, tb_first, tb_num,
+                                                      tr["tb_size"], tr["d_tb_out"], tr["tb_out_off"], rate, dur,
+                                                      seed=args.seed)
+            runt(0, 0.5)
+            tbl, ncb, peak_bps, fails, _ = runt(0, 2.0)
+            peak_tb_rate = peak_bps / mean_tb_bits
+            res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_tb_per_s": round(len(tbl) / 2.0, 1),
+                   "crc_failures_at_peak": int(fails), "loads": []}
+            for load in (0.1, 0.3, 0.5, 0.7, 0.9):
+                tbl, ncb, bps, fails, waits = runt(load * peak_tb_rate, 2.0)
+                res["loads"].append({"offered_fraction_of_peak": load, "achieved_info_gbps": round(bps / 1e9, 4),
+                                     "tb_latency_ms_p50": pct(tbl, 50), "tb_latency_ms_p99": pct(tbl, 99),
+                                     "transport_blocks": int(len(tbl)), "code_blocks": int(ncb),
+                                     "crc_failures": int(fails)})
+            out["resident_transport_blocks_inputs_in_hbm"] = res
             # the same traffic offered at 70% of the batched BG1 Z=384 decoder's peak
             cbl, tbl, bps, waits = serve_loadgen(srv, d_llr, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
                                                  0.7 * peak_ref_gbps * 1e9 / mean_tb_bits, 2.0, seed=args.seed)

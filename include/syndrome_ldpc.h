@@ -283,8 +283,9 @@ int32_t syndrome_server_release(SyndromeServer *server, uint32_t slot);
 
 /* Transport blocks through the resident server (streams == 0): the end of
  * DlSchDecoder::decode (transport_block.rs:963-1044) on device.  The caller
- * acquires num_cb slots of one code and n_filler, fills (or will point)
- * their inputs, then submits them together, in code-block order, with the
+ * acquires num_cb slots of one code and n_filler, fills their inputs (or
+ * passes srcs[i], device-accessible, as submit_from does; srcs or an entry
+ * may be NULL), then submits them together, in code-block order, with the
  * transport block's payload size and a device-accessible destination of
  * tb_size bytes.  Each block, once decoded, checks its CRC-24B (num_cb > 1)
  * and folds its share of the CRC-24A into the transport block's state
@@ -307,8 +308,8 @@ typedef struct {
     double submit_ns, done_ns;
 } SyndromeServedTb;
 
-int32_t syndrome_server_submit_tb(SyndromeServer *server, const uint32_t *slots, size_t num_cb, size_t tb_size,
-                                  uint8_t *tb_out, uint64_t tag);
+int32_t syndrome_server_submit_tb(SyndromeServer *server, const uint32_t *slots, const float *const *srcs,
+                                  size_t num_cb, size_t tb_size, uint8_t *tb_out, uint64_t tag);
 int32_t syndrome_server_try_recv_tb(SyndromeServer *server, SyndromeServedTb *out);
 int32_t syndrome_server_release_tb(SyndromeServer *server, uint32_t tb);
 int32_t syndrome_server_stats(const SyndromeServer *server, int64_t *out_batches, int64_t *out_decoded,
@@ -330,6 +331,20 @@ int32_t syndrome_server_loadgen(SyndromeServer *server, const float *pool, const
                                 const int32_t *tb_first_cb, const int32_t *tb_num_cb, size_t n_tb, double tb_rate,
                                 double duration_s, uint64_t seed, double *cb_lat_ns, size_t cb_cap, size_t *n_cb,
                                 double *tb_lat_ns, size_t tb_cap, size_t *n_tbs, double *info_bps, int64_t *waits);
+/* The same over transport blocks (resident server only): each arrival is
+ * pool transport block p, its blocks acquired together and submitted with
+ * syndrome_server_submit_tb (inputs from the pool, payload to tb_out +
+ * tb_out_off[p], tb_size[p] bits); latency = transport-block completion,
+ * CRCs decided on the device, minus scheduled arrival.  Reports completed
+ * transport blocks and their code blocks, info bits/s (sum of tb_size) and
+ * CRC failures. */
+int32_t syndrome_server_loadgen_tb(SyndromeServer *server, const float *pool, const int64_t *cb_off,
+                                   const int32_t *cb_bg, const int32_t *cb_z, const int32_t *cb_nfill,
+                                   const int32_t *tb_first_cb, const int32_t *tb_num_cb, const int32_t *tb_size,
+                                   uint8_t *tb_out, const int64_t *tb_out_off, size_t n_tb, double tb_rate,
+                                   double duration_s, uint64_t seed, double *tb_lat_ns, size_t lat_cap,
+                                   size_t *n_tbs, size_t *n_cbs, double *info_bps, int64_t *crc_fails,
+                                   int64_t *waits);
 
 /* Synthetic code-block traffic, as the reference's benchmarks build it
  * (channel_sim.rs, QcLdpcEncoder, rate_match): for block i, K'_i = K_i -
@@ -341,6 +356,13 @@ int32_t syndrome_server_loadgen(SyndromeServer *server, const float *pool, const
 int32_t syndrome_traffic_blocks(const int32_t *bg, const int32_t *z, const int32_t *n_filler, const int32_t *e_len,
                                 const float *ebno_db, const float *rate, uint64_t seed, size_t count, float *e_llr,
                                 const int64_t *e_off, uint8_t *info, const int64_t *info_off, int32_t threads);
+/* The same with the info bits given: block i's K'_i bits are read from
+ * info + info_off[i] (one bit per byte), so transport-block payloads with
+ * their CRCs and segmentation can be sent through the same chain. */
+int32_t syndrome_traffic_blocks_info(const int32_t *bg, const int32_t *z, const int32_t *n_filler,
+                                     const int32_t *e_len, const float *ebno_db, const float *rate, uint64_t seed,
+                                     size_t count, const uint8_t *info, const int64_t *info_off, float *e_llr,
+                                     const int64_t *e_off, int32_t threads);
 
 #ifdef __cplusplus
 }

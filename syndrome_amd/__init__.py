@@ -8,4 +8,5 @@ reference's Rust decoder API over that ABI.
 from ._lib import SyndromeError, lib  # noqa: F401
 from .ldpc import (APP_CLAMP_I8, APP_CLAMP_WIDE, BG1, BG2, DEFAULT_SCALE, MSG_MAX, DecodePlan, LdpcWorkspace,  # noqa: F401,E501
                    HostPool, QcLdpcDecoder, QcLdpcEncoder, QuantParams, Server, awgn_transmit,
-                   quantize_llr_i16_device, serve_loadgen, traffic_blocks)
+                   quantize_llr_i16_device, serve_loadgen, serve_loadgen_tb, traffic_blocks,
+                   traffic_blocks_info)

@@ -76,13 +76,14 @@ SIGNATURES = {
     "syndrome_ldpc_plan_set_tbs": (I32, [P, SZ, P, P, P, P]),
     "syndrome_ldpc_plan_check_tbs": (I32, [P, P, P, P, P, P]),
     "syndrome_traffic_blocks": (I32, [P, P, P, P, P, P, C.c_uint64, SZ, P, P, P, P, I32]),
+    "syndrome_traffic_blocks_info": (I32, [P, P, P, P, P, P, C.c_uint64, SZ, P, P, P, P, I32]),
     "syndrome_server_create": (I32, [C.c_float, SZ, SZ, I32, I32, C.POINTER(P)]),
     "syndrome_server_destroy": (None, [P]),
     "syndrome_server_acquire": (I32, [P, I32, SZ, SZ, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32)]),
     "syndrome_server_submit": (I32, [P, C.c_uint32, C.c_uint64]),
     "syndrome_server_try_recv": (I32, [P, P]),
     "syndrome_server_release": (I32, [P, C.c_uint32]),
-    "syndrome_server_submit_tb": (I32, [P, P, SZ, SZ, P, C.c_uint64]),
+    "syndrome_server_submit_tb": (I32, [P, P, P, SZ, SZ, P, C.c_uint64]),
     "syndrome_server_try_recv_tb": (I32, [P, P]),
     "syndrome_server_release_tb": (I32, [P, C.c_uint32]),
     "syndrome_server_stats": (I32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(I32)]),
@@ -92,6 +93,8 @@ SIGNATURES = {
     "syndrome_server_loadgen": (I32, [P, P, P, P, P, P, P, P, P, SZ, C.c_double, C.c_double, C.c_uint64, P, SZ,
                                       C.POINTER(SZ), P, SZ, C.POINTER(SZ), C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]),
+    "syndrome_server_loadgen_tb": (I32, [P, P, P, P, P, P, P, P, P, P, P, SZ, C.c_double, C.c_double, C.c_uint64,
+                                         P, SZ, P, P, P, P, P]),
 }
 
 

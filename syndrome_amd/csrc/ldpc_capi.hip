@@ -494,13 +494,18 @@ int32_t syndrome_ldpc_encode_batch(const SyndromeLdpcDecoder *d, const uint8_t *
     });
 }
 
-int32_t syndrome_traffic_blocks(const int32_t *bg, const int32_t *z, const int32_t *n_filler, const int32_t *e_len,
-                                const float *ebno_db, const float *rate, uint64_t seed, size_t count, float *e_llr,
-                                const int64_t *e_off, uint8_t *info, const int64_t *info_off, int32_t threads)
+namespace {
+
+// Encode -> rate match (rv 0, Qm 1) -> AWGN for every block; info bits
+// either drawn from the seed (written to info when given) or read from
+// info_in.
+int32_t traffic(const int32_t *bg, const int32_t *z, const int32_t *n_filler, const int32_t *e_len,
+                const float *ebno_db, const float *rate, uint64_t seed, size_t count, float *e_llr,
+                const int64_t *e_off, uint8_t *info, const uint8_t *info_in, const int64_t *info_off, int32_t threads)
 {
     return guard([&]() -> int32_t {
         if (count && (!bg || !z || !e_len || !ebno_db || !rate || !e_llr || !e_off)) return SYNDROME_NULL_POINTER;
-        if (info && !info_off) return SYNDROME_NULL_POINTER;
+        if ((info || info_in) && !info_off) return SYNDROME_NULL_POINTER;
         // one encoder per distinct code, built up front
         std::map<std::pair<int, int>, std::unique_ptr<syn::Encoder>> enc;
         std::map<std::pair<int, int>, SynCode> codes;
@@ -529,14 +534,18 @@ int32_t syndrome_traffic_blocks(const int32_t *bg, const int32_t *z, const int32
                     const size_t k = syn::code_k(c), n = syn::code_n(c);
                     const size_t nf = n_filler ? (size_t)n_filler[i] : 0;
                     inf.assign(k, 0);
-                    uint64_t s = (seed + 0x9E3779B97F4A7C15ULL * (i + 1)) | 1;
-                    for (size_t b = 0; b < k - nf; ++b) {  // xorshift64, a middle bit per info bit
-                        s ^= s << 13;
-                        s ^= s >> 7;
-                        s ^= s << 17;
-                        inf[b] = (uint8_t)((s >> 27) & 1);
+                    if (info_in) {
+                        for (size_t b = 0; b < k - nf; ++b) inf[b] = info_in[info_off[i] + (int64_t)b] & 1u;
+                    } else {
+                        uint64_t s = (seed + 0x9E3779B97F4A7C15ULL * (i + 1)) | 1;
+                        for (size_t b = 0; b < k - nf; ++b) {  // xorshift64, a middle bit per info bit
+                            s ^= s << 13;
+                            s ^= s >> 7;
+                            s ^= s << 17;
+                            inf[b] = (uint8_t)((s >> 27) & 1);
+                        }
+                        if (info) memcpy(info + info_off[i], inf.data(), k - nf);
                     }
-                    if (info) memcpy(info + info_off[i], inf.data(), k - nf);
                     cw.resize(n);
                     enc[key]->encode(inf.data(), cw.data());
                     e.resize((size_t)e_len[i]);
@@ -548,6 +557,24 @@ int32_t syndrome_traffic_blocks(const int32_t *bg, const int32_t *z, const int32
         for (auto &th : pool) th.join();
         return SYNDROME_OK;
     });
+}
+
+}  // namespace
+
+int32_t syndrome_traffic_blocks(const int32_t *bg, const int32_t *z, const int32_t *n_filler, const int32_t *e_len,
+                                const float *ebno_db, const float *rate, uint64_t seed, size_t count, float *e_llr,
+                                const int64_t *e_off, uint8_t *info, const int64_t *info_off, int32_t threads)
+{
+    return traffic(bg, z, n_filler, e_len, ebno_db, rate, seed, count, e_llr, e_off, info, nullptr, info_off, threads);
+}
+
+int32_t syndrome_traffic_blocks_info(const int32_t *bg, const int32_t *z, const int32_t *n_filler,
+                                     const int32_t *e_len, const float *ebno_db, const float *rate, uint64_t seed,
+                                     size_t count, const uint8_t *info, const int64_t *info_off, float *e_llr,
+                                     const int64_t *e_off, int32_t threads)
+{
+    if (count && !info) return SYNDROME_NULL_POINTER;
+    return traffic(bg, z, n_filler, e_len, ebno_db, rate, seed, count, e_llr, e_off, nullptr, info, info_off, threads);
 }
 
 }  // extern "C"

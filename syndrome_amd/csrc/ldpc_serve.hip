@@ -794,8 +794,8 @@ int32_t syndrome_server_release(SyndromeServer *s, uint32_t slot)
     });
 }
 
-int32_t syndrome_server_submit_tb(SyndromeServer *s, const uint32_t *slots, size_t num_cb, size_t tb_size,
-                                  uint8_t *tb_out, uint64_t tag)
+int32_t syndrome_server_submit_tb(SyndromeServer *s, const uint32_t *slots, const float *const *srcs, size_t num_cb,
+                                  size_t tb_size, uint8_t *tb_out, uint64_t tag)
 {
     return guard([&]() -> int32_t {
         if (!s || !slots || !tb_out) return SYNDROME_NULL_POINTER;
@@ -827,6 +827,7 @@ int32_t syndrome_server_submit_tb(SyndromeServer *s, const uint32_t *slots, size
         r.state = 2;
         for (size_t i = 0; i < num_cb; ++i) {
             SyndromeServer::Slot &x = s->st[slots[i]];
+            if (srcs && srcs[i]) x.src = srcs[i];
             x.tb = ti;
             x.tb_ci = (int)i;
             x.tag = tag;
@@ -1000,6 +1001,120 @@ int32_t syndrome_server_loadgen(SyndromeServer *s, const float *pool, const int6
         if (n_cb) *n_cb = ncb;
         if (n_tbs) *n_tbs = ntb;
         if (info_bps) *info_bps = (double)info / ((last_done - t0) * 1e-9);
+        if (waits) *waits = wait_count;
+        return SYNDROME_OK;
+    });
+}
+
+// Transport-block load generator over the resident server: as
+// syndrome_server_loadgen, but each arrival is one transport block (its
+// blocks acquired together, submitted with syndrome_server_submit_tb with
+// inputs read from the pool, payloads written to tb_out + tb_out_off[p]),
+// and latency is transport-block completion (CRCs decided on the device)
+// minus scheduled arrival.
+int32_t syndrome_server_loadgen_tb(SyndromeServer *s, const float *pool, const int64_t *cb_off,
+                                   const int32_t *cb_bg, const int32_t *cb_z, const int32_t *cb_nfill,
+                                   const int32_t *tb_first_cb, const int32_t *tb_num_cb, const int32_t *tb_size,
+                                   uint8_t *tb_out, const int64_t *tb_out_off, size_t n_tb, double tb_rate,
+                                   double duration_s, uint64_t seed, double *tb_lat_ns, size_t lat_cap,
+                                   size_t *n_tbs, size_t *n_cbs, double *info_bps, int64_t *crc_fails,
+                                   int64_t *waits)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !pool || !cb_off || !cb_bg || !cb_z || !tb_first_cb || !tb_num_cb || !tb_size || !tb_out ||
+            !tb_out_off || !n_tb)
+            return SYNDROME_NULL_POINTER;
+        if (!s->resident) return SYNDROME_INVALID_PARAM;
+        struct Pending {
+            int pool_tb;
+            double arrival;
+            std::vector<uint32_t> slots;
+        };
+        std::deque<Pending> pending;
+        std::vector<std::pair<double, int>> meta;  // per submission (its tag): arrival, pool transport block
+        uint64_t rs = seed | 1;
+        auto urand = [&]() {
+            rs ^= rs << 13;
+            rs ^= rs >> 7;
+            rs ^= rs << 17;
+            return ((rs >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        };
+        const double t0 = now_ns(), t_end = t0 + duration_s * 1e9;
+        double next = t0, last_done = t0;
+        size_t ntb = 0, ncb = 0, next_tb = 0;
+        long long info = 0, fails = 0, wait_count = 0, outstanding = 0;
+        std::vector<const float *> srcs;
+        for (;;) {
+            const double now = now_ns();
+            const bool open = now < t_end;
+            if (open) {
+                if (tb_rate > 0) {
+                    while (next <= now && next < t_end) {
+                        pending.push_back(Pending{(int)(next_tb++ % n_tb), next, {}});
+                        next += -log(urand()) / tb_rate * 1e9;
+                    }
+                } else {
+                    while (pending.size() < 64) pending.push_back(Pending{(int)(next_tb++ % n_tb), now, {}});
+                }
+            }
+            while (!pending.empty() && (open || tb_rate > 0)) {
+                Pending &pd = pending.front();
+                const int p = pd.pool_tb, c = tb_num_cb[p], f = tb_first_cb[p];
+                bool busy = false;
+                while ((int)pd.slots.size() < c) {  // a transport block holds its slots as it gets them
+                    float *llr = nullptr;
+                    uint32_t slot = 0;
+                    const int cb = f + (int)pd.slots.size();
+                    const int32_t rc = syndrome_server_acquire(s, cb_bg[cb], (size_t)cb_z[cb],
+                                                               cb_nfill ? (size_t)cb_nfill[cb] : 0, &llr, &slot);
+                    if (rc == SYNDROME_BUSY) {
+                        busy = true;
+                        break;
+                    }
+                    if (rc) return rc;
+                    pd.slots.push_back(slot);
+                }
+                if (!busy) {
+                    srcs.resize(c);
+                    for (int i = 0; i < c; ++i) srcs[i] = pool + cb_off[f + i];
+                    const int32_t rc = syndrome_server_submit_tb(s, pd.slots.data(), srcs.data(), (size_t)c,
+                                                                 (size_t)tb_size[p], tb_out + tb_out_off[p],
+                                                                 (uint64_t)meta.size());
+                    if (rc == SYNDROME_BUSY) busy = true;
+                    else if (rc) return rc;
+                }
+                if (busy) {
+                    ++wait_count;
+                    break;
+                }
+                meta.push_back({pd.arrival, p});
+                pending.pop_front();
+                ++outstanding;
+            }
+            if (!open && tb_rate <= 0) {
+                for (Pending &pd : pending)
+                    for (uint32_t sl : pd.slots) (void)syndrome_server_release(s, sl);
+                pending.clear();
+            }
+            SyndromeServedTb r;
+            while (syndrome_server_try_recv_tb(s, &r) == SYNDROME_OK) {
+                const auto &m = meta[r.tag];
+                if (ntb < lat_cap && tb_lat_ns) tb_lat_ns[ntb] = r.done_ns - m.first;
+                ++ntb;
+                ncb += r.num_cb;
+                info += tb_size[m.second];
+                if (!r.crc_ok) ++fails;
+                last_done = std::max(last_done, r.done_ns);
+                --outstanding;
+                (void)syndrome_server_release_tb(s, r.tb);
+            }
+            if (!open && pending.empty() && outstanding == 0) break;
+            if (now > t_end + 30e9) return SYNDROME_DEVICE_ERROR;  // drained for 30 s without finishing
+        }
+        if (n_tbs) *n_tbs = ntb;
+        if (n_cbs) *n_cbs = ncb;
+        if (info_bps) *info_bps = (double)info / ((last_done - t0) * 1e-9);
+        if (crc_fails) *crc_fails = fails;
         if (waits) *waits = wait_count;
         return SYNDROME_OK;
     });

@@ -364,6 +364,25 @@ def traffic_blocks(bg, z, n_filler, e_len, ebno_db, rate, seed, threads=0, want_
     return e_llr, e_off, info, i_off
 
 
+def traffic_blocks_info(bg, z, n_filler, e_len, ebno_db, rate, info, info_off, seed, threads=0):
+    """syndrome_traffic_blocks_info: the given info bits (K'_i at info_off[i])
+    encoded, rate matched and sent over AWGN.  Returns (e_llr, e_offsets)."""
+    bg = np.ascontiguousarray(bg, np.int32)
+    z = np.ascontiguousarray(z, np.int32)
+    nf = np.ascontiguousarray(n_filler, np.int32)
+    el = np.ascontiguousarray(e_len, np.int32)
+    eb = np.ascontiguousarray(ebno_db, np.float32)
+    rt = np.ascontiguousarray(rate, np.float32)
+    inf = np.ascontiguousarray(info, np.uint8)
+    ioff = np.ascontiguousarray(info_off, np.int64)
+    e_off = np.concatenate([[0], np.cumsum(el.astype(np.int64))[:-1]]).astype(np.int64)
+    e_llr = np.zeros(int(el.astype(np.int64).sum()), np.float32)
+    check(lib().syndrome_traffic_blocks_info(_ptr(bg), _ptr(z), _ptr(nf), _ptr(el), _ptr(eb), _ptr(rt), seed, len(bg),
+                                             _ptr(inf), _ptr(ioff), _ptr(e_llr), _ptr(e_off), threads),
+          "traffic_blocks_info")
+    return e_llr, e_off
+
+
 class Server:
     """Serving loop (include/syndrome_ldpc.h Part 5): acquire / submit /
     try_recv / release, as LdpcPipeline (src/ldpc_pipeline.rs:499-576).
@@ -425,7 +444,7 @@ class Server:
             out = tb_out
         else:
             out = tb_out.ptr if isinstance(tb_out, HostPool) else tb_out.data_ptr()
-        rc = lib().syndrome_server_submit_tb(self._h, _ptr(sl), sl.size, tb_size, out, tag)
+        rc = lib().syndrome_server_submit_tb(self._h, _ptr(sl), None, sl.size, tb_size, out, tag)
         if rc == _lib.BUSY:
             return False
         check(rc, "server_submit_tb")
@@ -493,3 +512,26 @@ def serve_loadgen(server, pool, cb_off, cb_bg, cb_z, cb_nfill, cb_info_bits, tb_
                                         duration_s, seed, _ptr(cb_lat), cap, C.byref(n_cb), _ptr(tb_lat), cap // 4,
                                         C.byref(n_tb), C.byref(bps), C.byref(waits)), "server_loadgen")
     return cb_lat[:min(n_cb.value, cap)], tb_lat[:min(n_tb.value, cap // 4)], bps.value, waits.value
+
+
+def serve_loadgen_tb(server, pool, cb_off, cb_bg, cb_z, cb_nfill, tb_first, tb_num, tb_size, tb_out, tb_out_off,
+                     tb_rate, duration_s, seed=1):
+    """Transport-block load generator (syndrome_server_loadgen_tb) over a
+    resident Server.  tb_out: device uint8 tensor or HostPool holding every
+    pool transport block's payload at tb_out_off.  Returns (tb_latency_ns,
+    code_blocks, info_bits_per_s, crc_failures, refused_acquires)."""
+    pool_ptr = pool.ptr if isinstance(pool, HostPool) else pool.data_ptr()
+    out_ptr = tb_out.ptr if isinstance(tb_out, HostPool) else tb_out.data_ptr()
+    arrs = [np.ascontiguousarray(x, t) for x, t in ((cb_off, np.int64), (cb_bg, np.int32), (cb_z, np.int32),
+                                                     (cb_nfill, np.int32), (tb_first, np.int32), (tb_num, np.int32),
+                                                     (tb_size, np.int32), (tb_out_off, np.int64))]
+    cap = 2_000_000
+    lat = np.zeros(cap, np.float64)
+    n_tb, n_cb = C.c_size_t(), C.c_size_t()
+    bps = C.c_double()
+    fails, waits = C.c_int64(), C.c_int64()
+    check(lib().syndrome_server_loadgen_tb(server._h, pool_ptr, *[_ptr(a) for a in arrs[:7]], out_ptr, _ptr(arrs[7]),
+                                           len(arrs[4]), tb_rate, duration_s, seed, _ptr(lat), cap, C.byref(n_tb),
+                                           C.byref(n_cb), C.byref(bps), C.byref(fails), C.byref(waits)),
+          "server_loadgen_tb")
+    return lat[:min(n_tb.value, cap)], n_cb.value, bps.value, fails.value, waits.value

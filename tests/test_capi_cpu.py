@@ -127,3 +127,27 @@ def test_traffic_generator_matches_oracle_pipeline(oracle):
         s = (seed ^ ((0xD1B54A32D192ED03 * (i + 1)) & 0xFFFFFFFFFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
         ref = oracle.awgn(np.float32(ebno), np.float32(rate[i]), s, bits)
         assert np.array_equal(f32_bits(e_llr[e_off[i]:e_off[i] + e]), f32_bits(ref)), i
+
+
+def test_traffic_generator_with_given_info(oracle):
+    """syndrome_traffic_blocks_info: caller's info bits through the same chain."""
+    from syndrome_amd import traffic_blocks_info
+    cases = [(0, 64, 12, 5000, 1.0), (1, 30, 0, 700, 2.5)]
+    rng = np.random.default_rng(2)
+    bg = [c[0] for c in cases]
+    z = [c[1] for c in cases]
+    nf = [c[2] for c in cases]
+    el = [c[3] for c in cases]
+    eb = [c[4] for c in cases]
+    kp = [(22 if b == 0 else 10) * zz - f for b, zz, f in zip(bg, z, nf)]
+    info = rng.integers(0, 2, sum(kp)).astype(np.uint8)
+    i_off = np.concatenate([[0], np.cumsum(kp)[:-1]]).astype(np.int64)
+    rate = [k / e for k, e in zip(kp, el)]
+    e_llr, e_off = traffic_blocks_info(bg, z, nf, el, eb, rate, info, i_off, 99)
+    for i in range(len(cases)):
+        code = oracle.code_3gpp(bg[i], z[i])
+        cw = oracle.encode(code, np.concatenate([info[i_off[i]:i_off[i] + kp[i]], np.zeros(nf[i], np.uint8)]))
+        bits = oracle.rate_match(cw, el[i], 0, 1, bg[i], z[i], nf[i])
+        s = (99 ^ ((0xD1B54A32D192ED03 * (i + 1)) & 0xFFFFFFFFFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
+        ref = oracle.awgn(np.float32(eb[i]), np.float32(rate[i]), s, bits)
+        assert np.array_equal(f32_bits(e_llr[e_off[i]:e_off[i] + el[i]]), f32_bits(ref)), i

@@ -243,3 +243,56 @@ def test_server_transport_blocks_match_oracle(oracle):
         ri, _, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, llr, nf), 8)
         assert r[2] == ri and np.array_equal(r[3], rh), f"loose block {t}"
     srv.close()
+
+
+def test_loadgen_transport_blocks():
+    """The transport-block load generator: every admitted transport block
+    completes, at 5 dB (above every rate's threshold) every CRC passes, and
+    the payloads written on the device are the sent transport blocks."""
+    import syndrome_amd
+    from syndrome_amd import DecodePlan, Server, serve_loadgen_tb, traffic_blocks_info
+    from syndrome_amd.transport import attach_crc24, compute_segmentation, segment
+    syndrome_amd.lib()
+    say("[serve tb loadgen] 5 TBs, saturating 0.3 s then Poisson 150 TB/s for 0.3 s")
+    rng = np.random.default_rng(12)
+    bg, z, nf, e, kp, first, num, sizes, infos, payloads = [], [], [], [], [], [], [], [], [], []
+    for a, r in ((1000, 0.5), (20000, 0.75), (3000, 1 / 3), (45000, 5 / 6), (300, 0.5)):
+        sg = compute_segmentation(a, r)
+        tb = rng.integers(0, 2, a).astype(np.uint8)
+        blocks = segment(attach_crc24(tb), sg)
+        first.append(len(bg))
+        num.append(sg.c)
+        sizes.append(a)
+        payloads.append(tb)
+        for i in range(sg.c):
+            bg.append(sg.bg)
+            z.append(sg.z)
+            nf.append(sg.n_filler)
+            e.append(int(round((a + 24) / r / sg.c)))
+            kp.append(sg.k_prime)
+            infos.append(blocks[i])
+    bg, z, nf, e, kp = map(np.array, (bg, z, nf, e, kp))
+    i_off = np.concatenate([[0], np.cumsum(kp)[:-1]]).astype(np.int64)
+    e_llr, e_off = traffic_blocks_info(bg, z, nf, e, np.full(len(bg), 5.0), kp / e, np.concatenate(infos), i_off, 4)
+    n = np.where(bg == 0, 68, 52) * z
+    offs = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+    plan = DecodePlan(bg, z, nf, offs, 0.5)
+    plan.set_dematch(e_off, e)
+    d = torch.empty(int(n.sum()), dtype=torch.float32, device="cuda")
+    plan.dematch(torch.from_numpy(e_llr).cuda(), d)
+    out_off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    d_out = torch.zeros(int(sum(sizes)), dtype=torch.uint8, device="cuda")
+    srv = Server(0.5, 8, hbm_budget_bytes=64 << 20, max_batch=64, streams=0)
+    lat, ncb, bps, fails, _ = serve_loadgen_tb(srv, d, offs, bg, z, nf, first, num, sizes, d_out, out_off, 0, 0.3)
+    say(f"  saturating: {len(lat)} TBs, {ncb} blocks, {bps / 1e9:.3f} Gbit/s, {fails} CRC failures")
+    assert len(lat) > 0 and fails == 0 and (lat > 0).all()
+    torch.cuda.synchronize()
+    got = d_out.cpu().numpy()
+    for t, tb in enumerate(payloads):
+        assert np.array_equal(got[out_off[t]:out_off[t] + sizes[t]], tb), f"TB {t} payload"
+    lat, ncb, bps, fails, _ = serve_loadgen_tb(srv, d, offs, bg, z, nf, first, num, sizes, d_out, out_off, 150.0, 0.3,
+                                               seed=2)
+    say(f"  poisson: {len(lat)} TBs, p50 {np.median(lat) / 1e6:.3f} ms")
+    assert len(lat) > 10 and fails == 0
+    assert ncb == sum(num[i % 5] for i in range(len(lat)))
+    srv.close()
