@@ -100,7 +100,8 @@ int build_code_raw(const int *rows, const int *cols, const int *shifts, size_t n
 
 int build_code_wifi(long long z, long long rate_num, long long rate_den, SynCode *out)
 {
-    for (const syn_wifi_matrix_t &m : SYN_WIFI_MATRICES) {
+    for (int i = 0; i < SYN_WIFI_COUNT; ++i) {
+        const syn_wifi_matrix_t &m = SYN_WIFI_MATRICES[i];
         if (m.z != z || m.rate_num != rate_num || m.rate_den != rate_den) continue;
         int rows[SYN_MAX_EDGES], cols[SYN_MAX_EDGES], sh[SYN_MAX_EDGES];
         for (int e = 0; e < m.n_edges; ++e) {
@@ -108,7 +109,9 @@ int build_code_wifi(long long z, long long rate_num, long long rate_den, SynCode
             cols[e] = m.edges[e][1];
             sh[e] = m.edges[e][2];
         }
-        return build_code_raw(rows, cols, sh, (size_t)m.n_edges, m.row_blocks, m.col_blocks, m.z, out);
+        const int rc = build_code_raw(rows, cols, sh, (size_t)m.n_edges, m.row_blocks, m.col_blocks, m.z, out);
+        if (rc == SYN_OK) out->kind = SYN_KIND_WIFI0 + i;  // the register-resident kernels know this graph
+        return rc;
     }
     return SYN_INVALID_PARAM;
 }

@@ -92,11 +92,11 @@ def main(ref_root):
         for r, c, s in ents:
             rows[r].append((c, s))
         flat = [(r, c, s) for r in range(mb) for c, s in rows[r]]
-        out.append(f"static const short SYN_WIFI_{i}_EDGES[{len(flat)}][3] = {{\n  "
+        out.append(f"SYN_TABLE_QUAL short SYN_WIFI_{i}_EDGES[{len(flat)}][3] = {{\n  "
                    + ", ".join("{%d,%d,%d}" % t for t in flat) + "\n};\n")
     out.append("typedef struct { int z, rate_num, rate_den, row_blocks, col_blocks, n_edges;"
                " const short (*edges)[3]; } syn_wifi_matrix_t;\n")
-    out.append("static const syn_wifi_matrix_t SYN_WIFI_MATRICES[SYN_WIFI_COUNT] = {\n")
+    out.append("SYN_TABLE_QUAL syn_wifi_matrix_t SYN_WIFI_MATRICES[SYN_WIFI_COUNT] = {\n")
     for i, (z, rn, rd, mb, nb, ents) in enumerate(mats):
         out.append(f"  {{{z}, {rn}, {rd}, {mb}, {nb}, {len(ents)}, SYN_WIFI_{i}_EDGES}},\n")
     out.append("};\n")

@@ -97,7 +97,10 @@ struct Shape {
     size_t lds = 0;
 };
 
-bool use_fast(const SynCode &c, bool plain) { return !plain && (c.kind == SYN_KIND_BG1 || c.kind == SYN_KIND_BG2); }
+bool use_fast(const SynCode &c, bool plain)
+{
+    return !plain && (c.kind == SYN_KIND_BG1 || c.kind == SYN_KIND_BG2 || c.kind >= SYN_KIND_WIFI0);
+}
 
 // Codewords per workgroup: as many whole codewords as fit the lane budget
 // (512 lanes for the register-resident kernels, whose state takes up to 256

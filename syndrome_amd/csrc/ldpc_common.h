@@ -31,6 +31,7 @@
 #define SYN_KIND_BG1 0
 #define SYN_KIND_BG2 1
 #define SYN_KIND_RAW 2
+#define SYN_KIND_WIFI0 16  // + matrix index: the 12 IEEE 802.11 matrices (src/wifi_ldpc_tables.rs)
 
 // Runtime description of one lifted code, QcLdpcParams (src/qc_ldpc.rs:233-253).
 // Lives in device memory; every field is read with wave-uniform (scalar) loads.

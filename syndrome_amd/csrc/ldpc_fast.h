@@ -18,8 +18,25 @@
 
 namespace syn {
 
+// The IEEE 802.11 matrices (KIND = SYN_KIND_WIFI0 + index): edges in the
+// table's row-grouped order, which is from_raw_edges' order
+// (src/qc_ldpc.rs:412-455); shifts are given for the matrix's own Z.
 template <int KIND>
-struct Graph;
+struct Graph {
+    static constexpr int M = KIND - SYN_KIND_WIFI0;
+    static_assert(M >= 0 && M < SYN_WIFI_COUNT, "graph kind");
+    static constexpr int ROWS = SYN_WIFI_MATRICES[M].row_blocks, COLS = SYN_WIFI_MATRICES[M].col_blocks;
+    static constexpr int EDGES = SYN_WIFI_MATRICES[M].n_edges;
+    static constexpr int Z = SYN_WIFI_MATRICES[M].z;
+    static constexpr int deg(int l)
+    {
+        int d = 0;
+        for (int e = 0; e < EDGES; ++e) d += SYN_WIFI_MATRICES[M].edges[e][0] == l;
+        return d;
+    }
+    static constexpr int col(int e) { return SYN_WIFI_MATRICES[M].edges[e][1]; }
+    static constexpr int v(int e, int) { return SYN_WIFI_MATRICES[M].edges[e][2]; }
+};
 
 template <>
 struct Graph<SYN_KIND_BG1> {
@@ -229,7 +246,7 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo,
                                           bool act)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
-    static_assert(D >= 2 && D <= SYN_MAX_DEG, "row degree");
+    static_assert(D >= 2 && D <= 27, "row degree: its signs and argmin share one 32-bit word");
     float *adr[D];
     float Lv[D], ro[D];
     SFor<0, D>::run([&](auto ec) {
@@ -283,7 +300,7 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC> &geo, 
                                          bool act)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
-    static_assert(D >= 2 && D <= SYN_MAX_DEG, "row degree");
+    static_assert(D >= 2 && D <= 27, "row degree: its signs and argmin share one 32-bit word");
     int16_t *adr[D];
     int Av[D], ro[D];
     SFor<0, D>::run([&](auto ec) {
@@ -353,7 +370,9 @@ __device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballo
 template <class G, int ZC, class T>
 __device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC> &geo, bool going, volatile int *cw_fail)
 {
-    constexpr int S0 = 1, S1 = 4, S2 = 10, S3 = 20, S4 = 32, S5 = G::ROWS;
+    constexpr int R = G::ROWS;
+    constexpr int S0 = R < 1 ? R : 1, S1 = R < 4 ? R : 4, S2 = R < 10 ? R : 10, S3 = R < 20 ? R : 20,
+                  S4 = R < 32 ? R : 32, S5 = R;
     bool fail = false;
     auto segment = [&](auto lo, auto hi) {
         constexpr int A = decltype(lo)::value, B = decltype(hi)::value;

@@ -91,7 +91,8 @@ def test_checksum_golden_on_gpu(dev, oracle):
         assert oracle.fnv1a(hard[0]) == 0xA2448E8CF5272825
 
 
-def test_wifi_convergence_fixture_on_gpu(dev, oracle):
+@pytest.mark.parametrize("plain", [False, True], ids=["fast", "plain"])
+def test_wifi_convergence_fixture_on_gpu(dev, oracle, plain):
     """ldpc_convergence.json: Wi-Fi Z=27 R=1/2, beta 0.25, seed 139, 13 iterations."""
     import json
     import os
@@ -103,13 +104,51 @@ def test_wifi_convergence_fixture_on_gpu(dev, oracle):
     llr = oracle.awgn(d["channel"]["eb_n0_db"], 0.5, d["channel"]["seed"], cw)
     dec = QcLdpcDecoder.wifi(27, 1, 2, 0.25)
     for it_budget in (d["max_iterations"],):
-        post, hard, edge, it = _run_batch_f32(dec, llr[None, :], it_budget, init5g=False, plain=True)
+        post, hard, edge, it = _run_batch_f32(dec, llr[None, :], it_budget, init5g=False, plain=plain)
         assert it[0] == d["iterations_used"]
         assert np.array_equal(hard[0], np.array(d["frames"][-1]["hard_bits"], np.uint8))
     # every intermediate frame: decode with a budget of k iterations
     for k in range(1, d["iterations_used"] + 1):
-        post, hard, edge, it = _run_batch_f32(dec, llr[None, :], k, init5g=False, plain=True)
+        post, hard, edge, it = _run_batch_f32(dec, llr[None, :], k, init5g=False, plain=plain)
         assert np.array_equal((post[0] < 0).astype(np.uint8), np.array(d["frames"][k]["hard_bits"], np.uint8)), k
+
+
+WIFI = [(z, rn, rd) for z in (27, 54, 81) for rn, rd in ((1, 2), (2, 3), (3, 4), (5, 6))]
+
+
+@pytest.mark.parametrize("plain", [False, True], ids=["fast", "plain"])
+@pytest.mark.parametrize("z,rn,rd", WIFI)
+def test_wifi_matrices_match_oracle(dev, oracle, z, rn, rd, plain):
+    """All 12 IEEE 802.11 matrices (wifi_ldpc_tables.rs): f32 and int8,
+    bit-exact in iterations, posteriors / app, R and hard decisions."""
+    import syndrome_amd._lib as L
+    from syndrome_amd import QcLdpcDecoder
+    say(f"[wifi] Z={z} R={rn}/{rd} plain={plain}")
+    code = oracle.code_wifi(z, rn, rd)
+    dec = QcLdpcDecoder.wifi(z, rn, rd, 0.5)
+    B = 5
+    cws = codewords(oracle, code, B, seed=31 * z + rn)
+    llr = channel(oracle, code, cws, [0.5, 1.5, 2.5, 3.5, 5.0], seed=7 * z + rd)
+    post, hard, edge, it = _run_batch_f32(dec, llr, 10, init5g=False, plain=plain)
+    for b in range(B):
+        ri, rp, re, rh = oracle.decode_f32(code, 0.5, llr[b], 10)
+        assert it[b] == ri and np.array_equal(hard[b], rh), f"f32 codeword {b}"
+        assert np.array_equal(f32_bits(post[b]), f32_bits(rp)), f"f32 codeword {b}: posterior"
+        assert np.array_equal(f32_bits(edge[b][: code.edge_buf]), f32_bits(re)), f"f32 codeword {b}: edge_r"
+    app = np.stack([oracle.quantize_i16(x, 8.0) for x in llr])
+    d_in = torch.from_numpy(app).cuda()
+    d_post = torch.empty_like(d_in)
+    d_hard = torch.empty(app.shape, dtype=torch.uint8, device="cuda")
+    d_edge = torch.empty((B, dec.required_edge_buffer()), dtype=torch.int8, device="cuda")
+    d_it = torch.empty(B, dtype=torch.int32, device="cuda")
+    dec.decode_batch_i8(d_in, d_post, d_hard, d_it, 10, flags=L.WRITE_POST | L.WRITE_HARD | L.WRITE_EDGE |
+                        (L.PLAIN if plain else 0), d_edge=d_edge)
+    torch.cuda.synchronize()
+    pa, ha, ea, ia = d_post.cpu().numpy(), d_hard.cpu().numpy(), d_edge.cpu().numpy(), d_it.cpu().numpy()
+    for b in range(B):
+        ri, ra, re, rh = oracle.decode_i8(code, 0.5, app[b], 10)
+        assert ia[b] == ri and np.array_equal(pa[b], ra) and np.array_equal(ha[b], rh), f"i8 codeword {b}"
+        assert np.array_equal(ea[b][: code.edge_buf], re), f"i8 codeword {b}: edge_r"
 
 
 def test_pipeline_codeword_five_iterations(dev, oracle):
