@@ -149,9 +149,11 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
             v = float(r["Counter_Value"])
             if kernel_substr in r["Kernel_Name"] and int(r["Grid_Size"]) == max(by_grid):
                 ker.append(v)
-            elif "copyBuffer" in r["Kernel_Name"] and v > 1000:
-                cpy.append(v)
-        counters[nm] = (float(np.median(ker)), float(np.median(cpy)))
+            elif "copyBuffer" in r["Kernel_Name"]:
+                cpy.append((int(r["Grid_Size"]), v))
+        # the calibration copy is the largest device copy of the run
+        big = max(g for g, _ in cpy)
+        counters[nm] = (float(np.median(ker)), float(np.median([v for g, v in cpy if g == big])))
     calib = json.loads(open(os.path.join(fetch_dir, "..", "bench_copycal.json")).read())["copy_calibration_bytes"] \
         if os.path.exists(os.path.join(fetch_dir, "..", "bench_copycal.json")) else None
     lines = [f"# rocprofv3 summary: {kernel_substr}", "", "Kernel trace command:", "", f"    {cmd_trace}", "",
@@ -159,7 +161,7 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
              "| kernel | calls | average ns | min ns | max ns |", "|---|---|---|---|---|"]
     for r in stats[:6]:
         lines.append(f"| {r['Name'][:70]} | {r['Calls']} | {float(r['AverageNs']):.0f} | {r['MinNs']} | {r['MaxNs']} |")
-    lines += ["", "## decode kernel by grid size (trace CSV)", "", "| grid (work-items) | launches | average ms |",
+    lines += ["", "## profiled kernel by grid size (trace CSV)", "", "| grid (work-items) | launches | average ms |",
               "|---|---|---|"]
     for g, v in sorted(by_grid.items()):
         lines.append(f"| {g} | {len(v)} | {np.mean(v):.4f} |")
@@ -177,8 +179,8 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
                     "hbm_fetch_bytes_per_launch": fetch_b, "hbm_write_bytes_per_launch": write_b,
                     "hbm_bytes_per_launch": fetch_b + write_b})
         lines += ["", "## HBM traffic per full-batch launch (--pmc)", "",
-                  f"FETCH_SIZE {fk:.0f} KiB raw; the input copy ({calib} B) reads {fc:.0f} KiB raw -> scale {f_scale:.4f}",
-                  f"WRITE_SIZE {wk:.0f} KiB raw; the input copy writes {wc:.0f} KiB raw -> scale {w_scale:.4f}",
+                  f"FETCH_SIZE {fk:.0f} KiB raw; the calibration copy ({calib} B) reads {fc:.0f} KiB raw -> scale {f_scale:.4f}",
+                  f"WRITE_SIZE {wk:.0f} KiB raw; the calibration copy writes {wc:.0f} KiB raw -> scale {w_scale:.4f}",
                   f"=> {fetch_b / 1e6:.1f} MB fetched + {write_b / 1e6:.1f} MB written = "
                   f"{(fetch_b + write_b) / 1e6:.1f} MB per launch"]
     os.makedirs(os.path.dirname(out_prefix), exist_ok=True)
@@ -467,7 +469,9 @@ def bench_serving(torch, args, peak_ref_gbps):
                                      "transport_blocks": int(len(tbl)), "code_blocks": int(ncb),
                                      "crc_failures": int(fails)})
             out["resident_transport_blocks_inputs_in_hbm"] = res
-            # the same traffic offered at 70% of the batched BG1 Z=384 decoder's peak
+        if mode == "resident" and peak_ref_gbps is not None:
+            # the same traffic offered at 70% of the batched BG1 Z=384
+            # decoder's peak (measured by the headline run)
             cbl, tbl, bps, waits = serve_loadgen(srv, d_llr, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
                                                  0.7 * peak_ref_gbps * 1e9 / mean_tb_bits, 2.0, seed=args.seed)
             out["resident_inputs_in_hbm_at_70pct_of_batched_bg1_peak"] = {
@@ -513,6 +517,70 @@ def bench_wifi(torch, args):
     return out
 
 
+def bench_encode(torch, args):
+    """Device encoder: 5G NR BG1 Z=384 (K=8448, N=26,112), 4,096 seeded
+    random info blocks in HBM -> codewords in HBM, one launch.  Algorithmic
+    HBM bytes per block: K read + N written (one byte per bit)."""
+    from syndrome_amd import QcLdpcDecoder
+    B = 4096
+    dec = QcLdpcDecoder.with_lifting_size(0, 384, 0.5)
+    k, n = dec.info_bit_count_5g(), dec.variable_node_count()
+    info = np.random.default_rng(args.seed + 2).integers(0, 2, (B, k), dtype=np.uint8)
+    d_info = torch.from_numpy(info).cuda()
+    d_cw = torch.empty((B, n), dtype=torch.uint8, device="cuda")
+    run = lambda: dec.encode_batch_device(d_info, d_cw)
+    if getattr(args, "copy_calibrate", False):  # known bytes to scale HBM counters
+        d_copy = torch.empty_like(d_cw)
+        for _ in range(3):
+            d_copy.copy_(d_cw)
+        torch.cuda.synchronize()
+        del d_copy
+        cal = os.environ.get("SYNDROME_COPYCAL_OUT")
+        if cal:
+            open(cal, "w").write(json.dumps({"copy_calibration_bytes": int(d_cw.numel())}))
+    _, per = time_launches(run, 50, 5, torch)
+    t = float(np.mean(per)) / 1e3
+    sample = 256
+    if not np.array_equal(d_cw[:sample].cpu().numpy(), dec.encode_batch(info[:sample])):
+        raise RuntimeError("device codewords differ from the host encoder")
+    gbs = B * (k + n) / t / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "encode_stream_bg1_z384_b4096_traffic.json")
+    if os.path.exists(prof):
+        traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+    out = {"workload": "BG1 Z=384 systematic encode, 4,096 blocks per launch, u8 bits in HBM",
+           "value": round(B * k / t / 1e9, 3), "unit": "info Gbit/s", "ms_per_launch": round(t * 1e3, 4),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "kernel": "syn::encode_stream", "algorithmic_bytes_per_launch": B * (k + n),
+                        "profile": "profiles/encode_stream_bg1_z384_b4096.md"}}
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    dec.encode_batch(info[:1024], threads=threads)
+    th = time.perf_counter() - t0
+    out["host_encoder"] = {"value": round(1024 * k / th / 1e9, 4), "unit": "info Gbit/s", "cores": threads,
+                           "sample": "1,024 blocks"}
+    if not args.no_cpu:
+        from oracle import orc
+        code = orc.code_3gpp(0, 384)
+        t0 = time.perf_counter()
+        for i in range(64):
+            orc.encode(code, info[i])
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(64 * k / tc / 1e9, 5), "unit": "info Gbit/s", "cores": 1,
+                               "kind": "port", "sample": "64 blocks through the oracle encoder"}
+    return out
+
+
+def extra_benches(torch, args, peak_gbps):
+    """(name, thunk) of every configuration reported beside the headline."""
+    return (("int8_bg2_z208", lambda: bench_int8(torch, args)),
+            ("uneven_persistent", lambda: bench_uneven(torch, args)),
+            ("serving", lambda: bench_serving(torch, args, peak_gbps)),
+            ("wifi_z27", lambda: bench_wifi(torch, args)),
+            ("encode_bg1_z384", lambda: bench_encode(torch, args)))
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-profiles":
         # bench.py --summarize-profiles TRACE_DIR FETCH_DIR WRITE_DIR OUT_PREFIX KERNEL CMD_TRACE CMD_PMC
@@ -537,6 +605,7 @@ def main():
                     help="single-codeword launches timed for latency (0 = skip)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run device copies of the input batch (known bytes) to scale HBM counters")
+    ap.add_argument("--only", default=None, help="run one extra by name and print its JSON (no headline line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU. The decode shards by codeword with no exchange between GPUs, "
@@ -551,6 +620,12 @@ def main():
     syndrome_amd.lib()
     if args.calibrate_ebno:
         calibrate_ebno(torch, args)
+        return
+    if args.only:
+        fn = dict(extra_benches(torch, args, None)).get(args.only)
+        if fn is None:
+            sys.exit(f"unknown extra {args.only!r}")
+        print(json.dumps({args.only: fn()}), flush=True)
         return
     dev = QcLdpcDecoder.with_lifting_size(syndrome_amd.BG1, 384, args.beta)
     n, k = dev.variable_node_count(), dev.info_bit_count_5g()
@@ -654,10 +729,7 @@ def main():
     line.update(out)
     if not args.quick:
         extras = {}
-        for name, fn in (("int8_bg2_z208", lambda: bench_int8(torch, args)),
-                         ("uneven_persistent", lambda: bench_uneven(torch, args)),
-                         ("serving", lambda: bench_serving(torch, args, value)),
-                         ("wifi_z27", lambda: bench_wifi(torch, args))):
+        for name, fn in extra_benches(torch, args, value):
             log(f"[bench] extra: {name}")
             t0 = time.perf_counter()
             try:

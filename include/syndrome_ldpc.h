@@ -162,6 +162,16 @@ int32_t syndrome_awgn_transmit(float ebno_db, float code_rate, uint64_t seed, co
  * threads (0 = all). */
 int32_t syndrome_ldpc_encode_batch(const SyndromeLdpcDecoder *decoder, const uint8_t *info, uint8_t *codewords,
                                    size_t count, int32_t threads);
+/* The same encode on device (QcLdpcEncoder::encode, qc_ldpc.rs:2592-2614,
+ * whose 3GPP path is encode_sparse, qc_ldpc.rs:2620-2727): d_info
+ * [count][K] and d_codewords [count][N] are device buffers of one byte per
+ * bit; info bytes are copied through, parity bits are 0/1 computed from bit
+ * 0 of each info byte.  Asynchronous on `stream` (NULL = default stream);
+ * the first call on a handle builds the device tables (one allocation).
+ * INVALID_PARAM for a raw code whose core inverse has more than 1024
+ * circulant terms (every 3GPP and 802.11 code has at most 180). */
+int32_t syndrome_ldpc_encode_batch_device(const SyndromeLdpcDecoder *decoder, const uint8_t *d_info,
+                                          uint8_t *d_codewords, size_t count, void *stream);
 
 /* ======================================================================
  * Part 4: work-queue batches of mixed codes (one persistent launch)

@@ -65,6 +65,7 @@ SIGNATURES = {
     "syndrome_ldpc_quantize_i16_device": (I32, [P, P, SZ, C.c_float, P]),
     "syndrome_awgn_transmit": (I32, [C.c_float, C.c_float, C.c_uint64, P, SZ, P]),
     "syndrome_ldpc_encode_batch": (I32, [P, P, P, SZ, I32]),
+    "syndrome_ldpc_encode_batch_device": (I32, [P, P, P, SZ, P]),
     "syndrome_ldpc_plan_create": (I32, [P, P, P, P, SZ, C.c_float, I32, I32, C.POINTER(P)]),
     "syndrome_ldpc_plan_destroy": (None, [P]),
     "syndrome_ldpc_plan_run": (I32, [P, P, P, P, P, SZ, C.c_float, C.c_int16, C.c_uint32, P]),

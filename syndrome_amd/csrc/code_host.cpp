@@ -226,6 +226,34 @@ void Encoder::encode(const uint8_t *info, uint8_t *cw) const
         }
 }
 
+bool Encoder::core_terms(std::vector<int> &off, std::vector<int> &blk, std::vector<int> &shift) const
+{
+    if (!ok_) return false;
+    const int z = c_.z;
+    auto bit = [&](int row, int col) { return (inv_[(size_t)row * words_ + col / 64] >> (col & 63)) & 1; };
+    off.assign(1, 0);
+    blk.clear();
+    shift.clear();
+    for (int i = 0; i < core_; ++i) {
+        for (int j = 0; j < core_; ++j)
+            for (int t = 0; t < z; ++t)
+                if (bit(i * z, j * z + t)) {
+                    blk.push_back(j);
+                    shift.push_back(t);
+                }
+        off.push_back((int)blk.size());
+        // every row of the block row must be the first one rotated
+        for (int zi = 1; zi < z; ++zi)
+            for (int j = 0; j < core_; ++j)
+                for (int t = 0; t < z; ++t) {
+                    int col = zi + t;
+                    if (col >= z) col -= z;
+                    if (bit(i * z + zi, j * z + col) != bit(i * z, j * z + t)) return false;
+                }
+    }
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // rate matching
 // ---------------------------------------------------------------------------

@@ -36,6 +36,16 @@ public:
     explicit Encoder(const SynCode &c);
     bool ok() const { return ok_; }
     void encode(const uint8_t *info, uint8_t *codeword) const;
+    // Block rows solved jointly; rows core() .. m_b-1 close on their own
+    // identity block.
+    int core() const { return core_; }
+    // The core inverse as circulant terms.  The core commutes with the cyclic
+    // shift inside every Z-block, so its inverse does too: each Z x Z block
+    // (i, j) of the inverse is a sum of permutations P_t, row zi holding a 1
+    // at column (zi + t) mod Z.  Terms of block row i are
+    // [off[i], off[i+1]) of (blk, shift).  False when the encoder is not ok
+    // or the inverse fails the circulant check.
+    bool core_terms(std::vector<int> &off, std::vector<int> &blk, std::vector<int> &shift) const;
 
 private:
     SynCode c_;

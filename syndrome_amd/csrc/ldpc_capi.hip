@@ -25,6 +25,9 @@ namespace syn {
 int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, const SynLaunch &a, int block,
                   size_t lds, hipStream_t stream);
 int launch_quantize(const float *in, int16_t *out, size_t n, float scale, hipStream_t stream);
+size_t encode_lds_bytes(const SynCode &c, const SynEncCode &e, const void *info, const void *cw);
+int launch_encode(const SynCode *d_code, const SynEncCode *d_enc, int z, int k, int n, size_t lds,
+                  const uint8_t *info, uint8_t *cw, long long count, hipStream_t stream);
 }
 
 struct SyndromeLdpcDecoder {
@@ -41,6 +44,12 @@ struct SyndromeLdpcDecoder {
     // lazily built encoder for traffic generation
     std::once_flag enc_once;
     std::unique_ptr<syn::Encoder> enc;
+    // its device form (circulant terms of the core inverse), built on the
+    // first device encode
+    std::once_flag denc_once;
+    SynEncCode *d_enc = nullptr;
+    std::unique_ptr<SynEncCode> h_enc;
+    int32_t denc_status = SYNDROME_OK;
 };
 
 struct SyndromeLdpcEncoder {
@@ -254,6 +263,7 @@ void syndrome_ldpc_decoder_destroy(SyndromeLdpcDecoder *d)
         DeviceScope ds(d->device);
         if (d->d_code) (void)hipFree(d->d_code);
         if (d->d_stage) (void)hipFree(d->d_stage);
+        if (d->d_enc) (void)hipFree(d->d_enc);
         if (d->stream) (void)hipStreamDestroy(d->stream);
     } catch (...) {
     }
@@ -494,6 +504,53 @@ int32_t syndrome_ldpc_encode_batch(const SyndromeLdpcDecoder *d, const uint8_t *
             });
         for (auto &th : pool) th.join();
         return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_ldpc_encode_batch_device(const SyndromeLdpcDecoder *d, const uint8_t *d_info, uint8_t *d_codewords,
+                                          size_t count, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if (!d || ((!d_info || !d_codewords) && count)) return SYNDROME_NULL_POINTER;
+        if (count > (size_t)LLONG_MAX / syn::code_n(d->code)) return SYNDROME_INVALID_PARAM;
+        SyndromeLdpcDecoder *md = const_cast<SyndromeLdpcDecoder *>(d);
+        DeviceScope ds(d->device);
+        std::call_once(md->denc_once, [&] {
+            std::call_once(md->enc_once, [&] { md->enc.reset(new syn::Encoder(md->code)); });
+            std::vector<int> off, blk, sh;
+            if (!md->enc || !md->enc->core_terms(off, blk, sh) || blk.size() > SYN_ENC_MAX_TERMS) {
+                md->denc_status = SYNDROME_INVALID_PARAM;
+                return;
+            }
+            std::unique_ptr<SynEncCode> h(new SynEncCode());
+            h->core = md->enc->core();
+            h->n_terms = (int)blk.size();
+            for (size_t i = 0; i < off.size(); ++i) h->term_off[i] = off[i];
+            for (size_t t = 0; t < blk.size(); ++t) {
+                h->term_blk[t] = (short)blk[t];
+                h->term_shift[t] = (short)sh[t];
+            }
+            SynEncCode *p = nullptr;
+            if (hipMalloc((void **)&p, sizeof(SynEncCode)) != hipSuccess) {
+                md->denc_status = SYNDROME_DEVICE_ERROR;
+                return;
+            }
+            if (hipMemcpy(p, h.get(), sizeof(SynEncCode), hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipFree(p);
+                md->denc_status = SYNDROME_DEVICE_ERROR;
+                return;
+            }
+            md->d_enc = p;
+            md->h_enc = std::move(h);
+        });
+        if (md->denc_status != SYNDROME_OK) return md->denc_status;
+        if (!count) return SYNDROME_OK;
+        const SynCode &c = d->code;
+        const size_t lds = syn::encode_lds_bytes(c, *md->h_enc, d_info, d_codewords);
+        return syn::launch_encode(d->d_code, md->d_enc, c.z, (int)syn::code_k(c), (int)syn::code_n(c), lds, d_info,
+                                  d_codewords, (long long)count, (hipStream_t)stream)
+                   ? SYNDROME_DEVICE_ERROR
+                   : SYNDROME_OK;
     });
 }
 

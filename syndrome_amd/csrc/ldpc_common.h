@@ -155,3 +155,17 @@ struct SynDematchCb {
     int rv;             // redundancy version 0..3
     int n_filler;
 };
+
+// Parity solve of the device encoder for one code: block rows [0, core)
+// are solved jointly through the core inverse, given as circulant terms
+// (block row i: XOR over terms [term_off[i], term_off[i+1]) of the core
+// syndrome block term_blk rotated by term_shift); rows [core, m_b) each
+// close on the identity block in their own parity column.
+#define SYN_ENC_MAX_TERMS 1024
+struct SynEncCode {
+    int core;
+    int n_terms;
+    int term_off[SYN_MAX_ROWS + 1];
+    short term_blk[SYN_ENC_MAX_TERMS];
+    short term_shift[SYN_ENC_MAX_TERMS];
+};

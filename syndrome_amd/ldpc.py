@@ -194,6 +194,21 @@ class QcLdpcDecoder:
         check(lib().syndrome_ldpc_encode_batch(self._h, _ptr(info), _ptr(out), count, threads), "encode_batch")
         return out
 
+    def encode_batch_device(self, d_info, d_cw, stream=None):
+        """Encode on device: d_info u8 [count, K] -> d_cw u8 [count, N] (torch tensors in HBM)."""
+        import torch
+        k, n = self.info_bit_count_5g(), self.variable_node_count()
+        if d_info.dtype != torch.uint8 or d_cw.dtype != torch.uint8:
+            raise ValueError("encode_batch_device: u8 tensors expected")
+        count = d_info.numel() // k if k else 0
+        if d_info.numel() != count * k or d_cw.numel() != count * n:
+            raise ValueError("encode_batch_device: shapes must be [count, K] and [count, N]")
+        if not (d_info.is_contiguous() and d_cw.is_contiguous()):
+            raise ValueError("encode_batch_device: contiguous tensors expected")
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().syndrome_ldpc_encode_batch_device(self._h, d_info.data_ptr(), d_cw.data_ptr(), count, s),
+              "encode_batch_device")
+
 
 class LdpcWorkspace:
     """LdpcWorkspace (src/qc_ldpc.rs:1919-1963)."""

@@ -52,6 +52,7 @@ def test_null_decoder_is_reported_not_dereferenced(L):
     p = buf.ctypes.data_as(C.c_void_p)
     assert L.syndrome_ldpc_decode_5g(None, p, 16, 0, p, 16, p, 16, p, 16, 5, None) == _lib.NULL_POINTER
     assert L.syndrome_ldpc_decode(None, p, 16, p, 16, p, 16, p, 16, 5, None) == _lib.NULL_POINTER
+    assert L.syndrome_ldpc_encode_batch_device(None, p, p, 1, None) == _lib.NULL_POINTER
     L.syndrome_ldpc_decoder_destroy(None)  # no-op
     L.syndrome_ldpc_encoder_destroy(None)
 
