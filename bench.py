@@ -581,7 +581,51 @@ def extra_benches(torch, args, peak_gbps):
             ("encode_bg1_z384", lambda: bench_encode(torch, args)))
 
 
+def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, waves_per_cw, edges, iters, clock_ghz=2.4, cus=256):
+    """Append the SQ counters of the profiled kernel's full-batch dispatches
+    (mean over them) to a profile summary, with the VALU issue share they
+    imply: VALU instructions x 4 cycles over cus x 4 SIMDs x kernel time."""
+    import csv
+    from collections import defaultdict
+    vals = defaultdict(list)
+    for d in sq_dirs:
+        rows = list(csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
+        ker = [r for r in rows if kernel_substr in r["Kernel_Name"]]
+        big = max(int(r["Grid_Size"]) for r in ker)
+        for r in ker:
+            if int(r["Grid_Size"]) == big:
+                vals[r["Counter_Name"]].append(float(r["Counter_Value"]))
+    mean = {k: float(np.mean(v)) for k, v in vals.items()}
+    traffic = json.load(open(out_md[:-3] + "_traffic.json"))
+    kern_s = traffic["average_ms_full_batch"] / 1e3
+    waves = mean.get("SQ_WAVES", 0.0)
+    valu = mean.get("SQ_INSTS_VALU", 0.0)
+    issue = valu * 4 / (cus * 4 * kern_s * clock_ghz * 1e9)
+    lines = ["", f"## SQ counters of the same kernel (full-batch dispatches, mean of {len(vals.get('SQ_WAVES', []))})",
+             "", "Commands (counters within the per-block limits, one pass each):", ""]
+    lines += [f"    {c}" for c in cmds]
+    lines += ["", "| counter | value |", "|---|---|"]
+    for k in sorted(mean):
+        note = ""
+        if k == "SQ_WAVES" and waves_per_cw:
+            note = f" ({waves / waves_per_cw:,.0f} codewords x {waves_per_cw} waves)"
+        if k == "SQ_INSTS_VALU" and waves:
+            note = (f" ({valu / waves:,.0f} per wave: {valu / waves / iters:,.0f} per wave per iteration at {iters} "
+                    f"mean iterations, {valu / waves / iters / edges:,.1f} per edge)")
+        lines.append(f"| {k} | {mean[k]:,.0f}{note} |")
+    lines += ["", f"VALU issue share: {valu:.3e} VALU instructions x 4 cycles over {cus} CUs x 4 SIMDs and the "
+                  f"{kern_s * 1e3:.3f} ms kernel ({clock_ghz} GHz) = {issue * 100:.0f} % of the device's VALU issue slots."]
+    open(out_md, "a").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-sq":
+        # bench.py --summarize-sq OUT_MD KERNEL WAVES_PER_CW EDGES MEAN_ITERS SQ_DIR1 CMD1 [SQ_DIR2 CMD2 ...]
+        a = sys.argv[2:]
+        rest = a[5:]
+        summarize_sq(rest[0::2], a[0], a[1], rest[1::2], int(a[2]), int(a[3]), float(a[4]))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-profiles":
         # bench.py --summarize-profiles TRACE_DIR FETCH_DIR WRITE_DIR OUT_PREFIX KERNEL CMD_TRACE CMD_PMC
         summarize_profiles(*sys.argv[2:9])

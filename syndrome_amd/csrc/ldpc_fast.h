@@ -240,8 +240,10 @@ __device__ __forceinline__ int r_i8(int g1, int g2, uint32_t s, uint32_t am, int
     return (m ^ neg) - neg;
 }
 
-// One layer of the f32 decode (src/qc_ldpc.rs:1072-1204).
-template <class G, int ZC, int L>
+// One layer of the f32 decode (src/qc_ldpc.rs:1072-1204).  KEEPQ keeps q's
+// bits in registers from pass 1 to pass 2, which saves one operation per
+// edge in pass 2 for D more live registers.
+template <class G, int ZC, int L, bool KEEPQ>
 __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo, StateF32<G> &st, float beta,
                                           bool act)
 {
@@ -263,10 +265,12 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo,
     // min1/min2 (a NaN is ignored there, and here it can never be selected).
     uint32_t m1 = 0x7f7fffffu, m2 = 0x7f7fffffu;  // FLT_MAX
     uint32_t qs = 0, am = 31;
+    uint32_t qsg[D];  // q's bits, kept for the sign of R_new (KEEPQ)
     SFor<0, D>::run([&](auto ec) {
         constexpr int e = decltype(ec)::value;
         ro[e] = r_f32<D>(o1, o2, os, oam, e);
         const uint32_t qb = __float_as_uint(Lv[e] - ro[e]);  // Q-build
+        if constexpr (KEEPQ) qsg[e] = qb;
         qs = __builtin_amdgcn_alignbit(qs, qb, 31);           // shift in q's sign
         const uint32_t a = qb & 0x7fffffffu;
         am = (a <= m1) ? (uint32_t)e : am;
@@ -281,11 +285,21 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo,
     float g2 = f2 - beta;
     g2 = g2 < 0.f ? 0.f : g2;
     // sign of R_e = XOR of all q signs but e's own
-    const uint32_t ns = qs ^ ((__popc(qs) & 1u) ? ((1u << D) - 1u) : 0u);
+    const uint32_t par = __popc(qs) & 1u;
+    const uint32_t ns = qs ^ (par ? ((1u << D) - 1u) : 0u);
     if (act) {
+        // R_new's sign is q's own sign XOR the parity of all of them: carry
+        // the parity in bit 31 of both magnitudes, XOR q's sign bit in
+        const uint32_t g1p = __float_as_uint(g1) | (par << 31), g2p = __float_as_uint(g2) | (par << 31);
         SFor<0, D>::run([&](auto ec) {
             constexpr int e = decltype(ec)::value;
-            const float rn = r_f32<D>(g1, g2, ns, am, e);
+            float rn;
+            if constexpr (KEEPQ) {
+                const uint32_t mp = (am == (uint32_t)e) ? g2p : g1p;
+                rn = __uint_as_float((qsg[e] & 0x80000000u) ^ mp);
+            } else {
+                rn = r_f32<D>(g1, g2, ns, am, e);
+            }
             *adr[e] = Lv[e] + (rn - ro[e]);  // pass 2: posterior += R_new - R_old
         });
         st.g1[L] = g1;
@@ -395,7 +409,7 @@ __device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC> &geo
 // and over the lanes, `io` saying where the live ones come from and go to.
 // I8 selects the int8 path.  Waves without a live lane keep only the
 // barriers.
-template <int KIND, int ZC, bool I8>
+template <int KIND, int ZC, bool I8, bool KEEPQ = true>
 __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int Z, int cws, const GroupIO &io,
                                            const SynLaunch &a)
 {
@@ -448,7 +462,7 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
             constexpr int L = decltype(lc)::value;
             if (wave_live) {
                 if constexpr (I8) layer_i8<G, ZC, L>(ln, geo, st, bq, C, act);
-                else layer_f32<G, ZC, L>(ln, geo, st, beta, act);
+                else layer_f32<G, ZC, L, KEEPQ>(ln, geo, st, beta, act);
             }
             if constexpr (barrier_after<G>(L)) __syncthreads();
         });
