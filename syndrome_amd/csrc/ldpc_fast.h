@@ -273,8 +273,11 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo,
         if constexpr (KEEPQ) qsg[e] = qb;
         qs = __builtin_amdgcn_alignbit(qs, qb, 31);           // shift in q's sign
         const uint32_t a = qb & 0x7fffffffu;
-        am = (a <= m1) ? (uint32_t)e : am;
-        m2 = clampu(a, m1, m2);  // min(m2, max(m1, a)) as m1 <= m2
+        if constexpr (!KEEPQ) am = (a <= m1) ? (uint32_t)e : am;
+        // min(m2, max(m1, a)) as m1 <= m2: one v_med3 (the register-lean
+        // build keeps the compiler's max/min form, which needs fewer VGPRs)
+        if constexpr (KEEPQ) m2 = med3u(a, m1, m2);
+        else m2 = clampu(a, m1, m2);
         m1 = a < m1 ? a : m1;
     });
     float f1 = __uint_as_float(m1), f2 = __uint_as_float(m2);
@@ -291,12 +294,18 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo,
         // R_new's sign is q's own sign XOR the parity of all of them: carry
         // the parity in bit 31 of both magnitudes, XOR q's sign bit in
         const uint32_t g1p = __float_as_uint(g1) | (par << 31), g2p = __float_as_uint(g2) | (par << 31);
+        const float m1f = __uint_as_float(m1);
         SFor<0, D>::run([&](auto ec) {
             constexpr int e = decltype(ec)::value;
             float rn;
             if constexpr (KEEPQ) {
-                const uint32_t mp = (am == (uint32_t)e) ? g2p : g1p;
-                rn = __uint_as_float((qsg[e] & 0x80000000u) ^ mp);
+                // the argmin is the last edge whose |q| equals min1 (the
+                // running "a <= m1" of pass 1 ends there); |q| compares as a
+                // float, which for non-negative values is bit equality and
+                // never matches a NaN
+                const bool at = fabsf(__uint_as_float(qsg[e])) == m1f;
+                am = at ? (uint32_t)e : am;
+                rn = __uint_as_float((qsg[e] & 0x80000000u) ^ (at ? g2p : g1p));
             } else {
                 rn = r_f32<D>(g1, g2, ns, am, e);
             }
