@@ -291,6 +291,27 @@ int32_t syndrome_server_submit_from(SyndromeServer *server, uint32_t slot, const
 int32_t syndrome_server_try_recv(SyndromeServer *server, SyndromeServed *out);
 int32_t syndrome_server_release(SyndromeServer *server, uint32_t slot);
 
+/* Batched forms of acquire / submit_from / try_recv / release: one server
+ * lock per call instead of one per code block, for front ends that move
+ * many blocks at a time (LdpcPipeline::submit / try_recv,
+ * ldpc_pipeline.rs:536-589, called in a loop).
+ * acquire_many: slots for blocks i < n of (base_graph[i], z[i],
+ * n_filler[i] or 0), in order, until the budget runs out; *out_acquired =
+ * how many (SYNDROME_BUSY when none); out_llr may be NULL.
+ * submit_many: srcs NULL, or srcs[i] NULL, keeps slot i's pinned staging;
+ * tags may be NULL (0).  All slots are checked first: a slot out of range,
+ * not acquired or listed twice submits nothing (INVALID_PARAM).
+ * try_recv_many: up to cap completed blocks; *out_n = how many
+ * (SYNDROME_BUSY when none; DEVICE_ERROR if any carries iterations < 0).
+ * release_many: all-or-nothing like submit_many. */
+int32_t syndrome_server_acquire_many(SyndromeServer *server, const int32_t *base_graph, const int32_t *z,
+                                     const int32_t *n_filler, size_t n, float **out_llr, uint32_t *out_slots,
+                                     size_t *out_acquired);
+int32_t syndrome_server_submit_many(SyndromeServer *server, const uint32_t *slots, const float *const *srcs,
+                                    const uint64_t *tags, size_t n);
+int32_t syndrome_server_try_recv_many(SyndromeServer *server, SyndromeServed *out, size_t cap, size_t *out_n);
+int32_t syndrome_server_release_many(SyndromeServer *server, const uint32_t *slots, size_t n);
+
 /* Transport blocks through the resident server (streams == 0): the end of
  * DlSchDecoder::decode (transport_block.rs:963-1044) on device.  The caller
  * acquires num_cb slots of one code and n_filler, fills their inputs (or

@@ -84,6 +84,10 @@ SIGNATURES = {
     "syndrome_server_submit": (I32, [P, C.c_uint32, C.c_uint64]),
     "syndrome_server_try_recv": (I32, [P, P]),
     "syndrome_server_release": (I32, [P, C.c_uint32]),
+    "syndrome_server_acquire_many": (I32, [P, P, P, P, SZ, P, P, C.POINTER(SZ)]),
+    "syndrome_server_submit_many": (I32, [P, P, P, P, SZ]),
+    "syndrome_server_try_recv_many": (I32, [P, P, SZ, C.POINTER(SZ)]),
+    "syndrome_server_release_many": (I32, [P, P, SZ]),
     "syndrome_server_submit_tb": (I32, [P, P, P, SZ, SZ, P, C.c_uint64]),
     "syndrome_server_try_recv_tb": (I32, [P, P]),
     "syndrome_server_release_tb": (I32, [P, C.c_uint32]),

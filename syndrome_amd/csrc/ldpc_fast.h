@@ -317,8 +317,10 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo,
     }
 }
 
-// One layer of the int8 decode (src/qc_ldpc.rs:1619-1745).
-template <class G, int ZC, int L>
+// One layer of the int8 decode (src/qc_ldpc.rs:1619-1745).  KEEPA keeps
+// |q| in registers for pass 2, which then finds the argmin instead of
+// pass 1 (one operation less per edge, D more live registers).
+template <class G, int ZC, int L, bool KEEPA>
 __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC> &geo, StateI8<G> &st, int bq, int C,
                                          bool act)
 {
@@ -336,13 +338,15 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC> &geo, 
     const uint32_t oam = (os >> D) & 31u;
     int m1 = 127, m2 = 127;
     uint32_t qs = 0, am = 31;
+    int av[D];  // |q| per edge, kept for pass 2's argmin
     SFor<0, D>::run([&](auto ec) {
         constexpr int e = decltype(ec)::value;
         ro[e] = r_i8<D>(o1, o2, os, oam, e);
         const int q = clampi(Av[e] - ro[e], -127, 127);  // Q-build, clamped to the message range
         qs = __builtin_amdgcn_alignbit(qs, (uint32_t)q, 31);
         const int a = q < 0 ? -q : q;
-        am = (a <= m1) ? (uint32_t)e : am;  // pass 1: sorted insertion
+        if constexpr (KEEPA) av[e] = a;
+        else am = (a <= m1) ? (uint32_t)e : am;  // pass 1: sorted insertion
         m2 = med3i(a, m1, m2);  // min(m2, max(m1, a)), as m1 <= m2 always
         m1 = a < m1 ? a : m1;
     });
@@ -354,7 +358,19 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC> &geo, 
     if (act) {
         SFor<0, D>::run([&](auto ec) {
             constexpr int e = decltype(ec)::value;
-            const int nv = Av[e] + (r_i8<D>(g1, g2, ns, am, e) - ro[e]);  // saturating add + clamp == clamp
+            // the argmin is the last edge whose |q| equals min1 (where pass
+            // 1's sorted insertion "a <= m1" ends); R_new takes g2 there
+            int rn;
+            if constexpr (KEEPA) {
+                const bool at = av[e] == m1;
+                am = at ? (uint32_t)e : am;
+                const int mag = at ? g2 : g1;
+                const int neg = -(int)((ns >> (D - 1 - e)) & 1u);
+                rn = (mag ^ neg) - neg;
+            } else {
+                rn = r_i8<D>(g1, g2, ns, am, e);
+            }
+            const int nv = Av[e] + (rn - ro[e]);  // saturating add + clamp == clamp
             *adr[e] = (int16_t)clampi(nv, -C, C);
         });
         mg_put<G, L>(st, (uint32_t)g1 | ((uint32_t)g2 << 8));
@@ -470,7 +486,7 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
         SFor<0, G::ROWS>::run([&](auto lc) {
             constexpr int L = decltype(lc)::value;
             if (wave_live) {
-                if constexpr (I8) layer_i8<G, ZC, L>(ln, geo, st, bq, C, act);
+                if constexpr (I8) layer_i8<G, ZC, L, KEEPQ && KIND != SYN_KIND_BG1>(ln, geo, st, bq, C, act);
                 else layer_f32<G, ZC, L, KEEPQ>(ln, geo, st, beta, act);
             }
             if constexpr (barrier_after<G>(L)) __syncthreads();
@@ -549,8 +565,10 @@ __device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, 
         const SynItem it = items[w];
         const SynCode *code = codes + it.code;
         const GroupIO io{it.count, 0, ids + it.first, offs, nfill, nullptr, nullptr};
-        if (it.kind == SYN_KIND_BG1) fast_group<SYN_KIND_BG1, 0, I8>(code, code->z, it.count, io, a);
-        else fast_group<SYN_KIND_BG2, 0, I8>(code, code->z, it.count, io, a);
+        // the int8 body keeps its register-lean form here: both base
+        // graphs share one kernel, whose VGPR budget the BG1 body sets
+        if (it.kind == SYN_KIND_BG1) fast_group<SYN_KIND_BG1, 0, I8, !I8>(code, code->z, it.count, io, a);
+        else fast_group<SYN_KIND_BG2, 0, I8, !I8>(code, code->z, it.count, io, a);
     }
 }
 

@@ -27,6 +27,7 @@
 #include <atomic>
 #include <chrono>
 #include <deque>
+#include <initializer_list>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -171,6 +172,7 @@ struct SyndromeServer {
         unsigned id;
         int n;
         int slot[SYN_WORK_MAX];
+        bool fin;  // seen complete (popped once everything before it is)
     };
     struct Resident {
         unsigned cap = 0;
@@ -183,7 +185,7 @@ struct SyndromeServer {
         hipEvent_t ev = nullptr;
         bool running = false;
         unsigned published = 0;
-        std::vector<Outstanding> outstanding;
+        std::deque<Outstanding> outstanding;  // in publish (= claim) order
         std::vector<int> cws;  // code blocks per item, per code
         int grid = 0;
         size_t lds = 0;
@@ -238,6 +240,30 @@ struct DeviceScope {
         if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
     }
 };
+
+// Marks slots[0..n) with state `mark` if each is in range, in one of the
+// states `want` (and, with no_tb, not part of a transport block) and listed
+// once (a repeat finds its own mark); on any failure every mark is undone
+// and false returned.  Called with the server lock held.
+bool claim_slots(SyndromeServer *s, const uint32_t *slots, size_t n, std::initializer_list<int> want, int mark,
+                 bool no_tb)
+{
+    std::vector<int> prev(n);
+    size_t i = 0;
+    for (; i < n; ++i) {
+        const uint32_t sl = slots[i];
+        if (sl >= (uint32_t)s->slots) break;
+        const int stt = s->st[sl].state;
+        bool ok = false;
+        for (int w : want) ok = ok || stt == w;
+        if (!ok || (no_tb && s->st[sl].tb >= 0)) break;
+        prev[i] = stt;
+        s->st[sl].state = mark;
+    }
+    if (i == n) return true;
+    while (i-- > 0) s->st[slots[i]].state = prev[i];
+    return false;
+}
 
 size_t group_bytes(const SynCode &c, int cws)
 {
@@ -488,23 +514,34 @@ void SyndromeServer::run_resident()
         bool progressed = false;
         const bool grid_done = rs.running && hipEventQuery(rs.ev) == hipSuccess;
         if (grid_done) rs.running = false;
-        for (size_t k = 0; k < rs.outstanding.size();) {
-            const Outstanding &o = rs.outstanding[k];
-            if (__atomic_load_n(rs.done + o.id % rs.cap, __ATOMIC_ACQUIRE) == o.id + 1u) {
-                const double t = now_ns();
-                std::lock_guard<std::mutex> lk(mu);
+        // The grid claims items in publish order and holds at most `grid` of
+        // them at once, so past the (grid + 1)-th unfinished item nothing
+        // has been claimed yet: the scan stops there.
+        {
+            int open_seen = 0;
+            double t = 0;
+            std::unique_lock<std::mutex> lk(mu, std::defer_lock);
+            for (size_t k = 0; k < rs.outstanding.size() && open_seen <= rs.grid; ++k) {
+                Outstanding &o = rs.outstanding[k];
+                if (o.fin) continue;
+                if (__atomic_load_n(rs.done + o.id % rs.cap, __ATOMIC_ACQUIRE) != o.id + 1u) {
+                    ++open_seen;
+                    continue;
+                }
+                if (!lk.owns_lock()) {
+                    t = now_ns();
+                    lk.lock();
+                }
                 for (int j = 0; j < o.n; ++j) {
                     st[o.slot[j]].state = 4;
                     st[o.slot[j]].t_done = t;
                     if (st[o.slot[j]].tb < 0) done.push_back(o.slot[j]);  // TB blocks are delivered per TB
                 }
                 decoded += o.n;
-                rs.outstanding[k] = rs.outstanding.back();
-                rs.outstanding.pop_back();
+                o.fin = true;
                 progressed = true;
-            } else {
-                ++k;
             }
+            while (!rs.outstanding.empty() && rs.outstanding.front().fin) rs.outstanding.pop_front();
         }
         {
             std::lock_guard<std::mutex> lk(mu);
@@ -544,7 +581,7 @@ void SyndromeServer::run_resident()
                     if (rs.outstanding.size() + 1 >= rs.cap) break;
                     const int cnt = (int)std::min(v.size() - j, (size_t)per);
                     SynWork &wk = rs.ring[rs.published % rs.cap];
-                    Outstanding o{rs.published, cnt, {}};
+                    Outstanding o{rs.published, cnt, {}, false};
                     int aligned = 1;
                     for (int q = 0; q < cnt; ++q) {
                         const Slot &x = st[v[j + q]];
@@ -581,7 +618,7 @@ void SyndromeServer::run_resident()
                 std::lock_guard<std::mutex> lk(mu);
                 error = rc;
                 for (const Outstanding &o : rs.outstanding)
-                    for (int j = 0; j < o.n; ++j) {
+                    for (int j = 0; j < (o.fin ? 0 : o.n); ++j) {
                         st[o.slot[j]].state = 4;
                         st[o.slot[j]].t_done = now_ns();
                         h_it[o.slot[j]] = -1;
@@ -794,6 +831,102 @@ int32_t syndrome_server_release(SyndromeServer *s, uint32_t slot)
     });
 }
 
+int32_t syndrome_server_acquire_many(SyndromeServer *s, const int32_t *base_graph, const int32_t *z,
+                                     const int32_t *n_filler, size_t n, float **out_llr, uint32_t *out_slots,
+                                     size_t *out_acquired)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !out_acquired || (n && (!base_graph || !z || !out_slots))) return SYNDROME_NULL_POINTER;
+        *out_acquired = 0;
+        std::vector<int> code(n), zi(n);
+        for (size_t i = 0; i < n; ++i) {
+            zi[i] = z[i] > 0 && z[i] <= 384 ? z_index(z[i]) : -1;
+            if ((base_graph[i] != 0 && base_graph[i] != 1) || zi[i] < 0 || (n_filler && n_filler[i] < 0))
+                return SYNDROME_INVALID_PARAM;
+            code[i] = base_graph[i] * 51 + zi[i];
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        size_t k = 0;
+        for (; k < n && !s->free_list.empty(); ++k) {
+            const int slot = s->free_list.back();
+            s->free_list.pop_back();
+            SyndromeServer::Slot &sl = s->st[slot];
+            sl.code = code[k];
+            sl.n = (base_graph[k] == 0 ? 68 : 52) * z[k];
+            sl.nfill = n_filler ? n_filler[k] : 0;
+            sl.state = 1;
+            sl.src = s->h_llr_dev + (size_t)slot * NMAX;
+            if (out_llr) out_llr[k] = s->h_llr + (size_t)slot * NMAX;
+            out_slots[k] = (uint32_t)slot;
+        }
+        *out_acquired = k;
+        return k == 0 && n ? SYNDROME_BUSY : SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_submit_many(SyndromeServer *s, const uint32_t *slots, const float *const *srcs,
+                                    const uint64_t *tags, size_t n)
+{
+    return guard([&]() -> int32_t {
+        if (!s || (n && !slots)) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!claim_slots(s, slots, n, {1}, -2, false)) return SYNDROME_INVALID_PARAM;
+        const double t = now_ns();
+        for (size_t i = 0; i < n; ++i) {
+            SyndromeServer::Slot &sl = s->st[slots[i]];
+            if (srcs && srcs[i]) sl.src = srcs[i];
+            sl.tag = tags ? tags[i] : 0;
+            sl.t_submit = t;
+            sl.state = 2;
+            s->submitted.push_back((int)slots[i]);
+        }
+        return SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_try_recv_many(SyndromeServer *s, SyndromeServed *out, size_t cap, size_t *out_n)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !out_n || (cap && !out)) return SYNDROME_NULL_POINTER;
+        *out_n = 0;
+        std::lock_guard<std::mutex> lk(s->mu);
+        size_t k = 0;
+        bool failed = false;
+        for (; k < cap && !s->done.empty(); ++k) {
+            const int slot = s->done.front();
+            s->done.pop_front();
+            SyndromeServer::Slot &sl = s->st[slot];
+            sl.state = 5;
+            SyndromeServed &o = out[k];
+            o.slot = (uint32_t)slot;
+            o.tag = sl.tag;
+            o.iterations = s->h_it[slot];
+            o.hard = s->h_hard + (size_t)slot * NMAX;
+            o.n = (size_t)sl.n;
+            o.submit_ns = sl.t_submit;
+            o.done_ns = sl.t_done;
+            failed = failed || o.iterations < 0;
+        }
+        *out_n = k;
+        if (!k) return SYNDROME_BUSY;
+        return failed ? SYNDROME_DEVICE_ERROR : SYNDROME_OK;
+    });
+}
+
+int32_t syndrome_server_release_many(SyndromeServer *s, const uint32_t *slots, size_t n)
+{
+    return guard([&]() -> int32_t {
+        if (!s || (n && !slots)) return SYNDROME_NULL_POINTER;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!claim_slots(s, slots, n, {1, 5}, -3, true)) return SYNDROME_INVALID_PARAM;
+        for (size_t i = 0; i < n; ++i) {
+            s->st[slots[i]].state = 0;
+            s->free_list.push_back((int)slots[i]);
+        }
+        return SYNDROME_OK;
+    });
+}
+
 int32_t syndrome_server_submit_tb(SyndromeServer *s, const uint32_t *slots, const float *const *srcs, size_t num_cb,
                                   size_t tb_size, uint8_t *tb_out, uint64_t tag)
 {
@@ -930,6 +1063,12 @@ int32_t syndrome_server_loadgen(SyndromeServer *s, const float *pool, const int6
         std::vector<Tb> tbs;
         std::deque<Cb> pending;
         std::vector<Cb> inflight(s->slots);
+        constexpr int BATCH = 256;
+        std::vector<int32_t> a_bg(BATCH), a_z(BATCH), a_nf(BATCH);
+        std::vector<uint32_t> a_slot(BATCH), a_rel(BATCH);
+        std::vector<const float *> a_src(BATCH);
+        std::vector<uint64_t> a_tag(BATCH);
+        std::vector<SyndromeServed> served(BATCH);
         uint64_t rs = seed | 1;
         auto urand = [&]() {
             rs ^= rs << 13;
@@ -962,38 +1101,58 @@ int32_t syndrome_server_loadgen(SyndromeServer *s, const float *pool, const int6
                     }
                 }
             }
+            // what is pending goes in in batches: one server lock per batch
             while (!pending.empty() && (open || tb_rate > 0)) {
-                const Cb cb = pending.front();
-                float *llr = nullptr;
-                uint32_t slot = 0;
-                const int32_t rc = syndrome_server_acquire(s, cb_bg[cb.pool_cb], (size_t)cb_z[cb.pool_cb],
-                                                           cb_nfill ? (size_t)cb_nfill[cb.pool_cb] : 0, &llr, &slot);
-                if (rc == SYNDROME_BUSY) {
+                const size_t m = std::min(pending.size(), (size_t)BATCH);
+                for (size_t i = 0; i < m; ++i) {
+                    const int pc = pending[i].pool_cb;
+                    a_bg[i] = cb_bg[pc];
+                    a_z[i] = cb_z[pc];
+                    a_nf[i] = cb_nfill ? cb_nfill[pc] : 0;
+                }
+                size_t k = 0;
+                const int32_t rc =
+                    syndrome_server_acquire_many(s, a_bg.data(), a_z.data(), a_nf.data(), m, nullptr, a_slot.data(), &k);
+                if (rc && rc != SYNDROME_BUSY) return rc;
+                for (size_t i = 0; i < k; ++i) {
+                    inflight[a_slot[i]] = pending[i];
+                    a_src[i] = pool + cb_off[pending[i].pool_cb];
+                    a_tag[i] = a_slot[i];
+                }
+                if (k) {
+                    if (int32_t r2 = syndrome_server_submit_many(s, a_slot.data(), a_src.data(), a_tag.data(), k))
+                        return r2;
+                }
+                pending.erase(pending.begin(), pending.begin() + (long)k);
+                outstanding += (long long)k;
+                if (k < m) {
                     ++wait_count;
                     break;
                 }
-                if (rc) return rc;
-                inflight[slot] = cb;
-                if (int32_t r2 = syndrome_server_submit_from(s, slot, pool + cb_off[cb.pool_cb], (uint64_t)slot)) return r2;
-                pending.pop_front();
-                ++outstanding;
             }
             if (!open && tb_rate <= 0) pending.clear();
-            SyndromeServed r;
-            while (syndrome_server_try_recv(s, &r) == SYNDROME_OK) {
-                const Cb &cb = inflight[r.slot];
-                if (ncb < cb_cap && cb_lat_ns) cb_lat_ns[ncb] = r.done_ns - cb.arrival;
-                ++ncb;
-                if (cb_info_bits) info += cb_info_bits[cb.pool_cb];
-                Tb &tb = tbs[cb.tb_slot];
-                tb.last_done = std::max(tb.last_done, r.done_ns);
-                last_done = std::max(last_done, r.done_ns);
-                if (--tb.left == 0) {
-                    if (ntb < tb_cap && tb_lat_ns) tb_lat_ns[ntb] = tb.last_done - tb.arrival;
-                    ++ntb;
+            for (;;) {
+                size_t k = 0;
+                const int32_t rc = syndrome_server_try_recv_many(s, served.data(), served.size(), &k);
+                if (rc == SYNDROME_BUSY) break;
+                if (rc) return rc;
+                for (size_t i = 0; i < k; ++i) {
+                    const SyndromeServed &r = served[i];
+                    const Cb &cb = inflight[r.slot];
+                    if (ncb < cb_cap && cb_lat_ns) cb_lat_ns[ncb] = r.done_ns - cb.arrival;
+                    ++ncb;
+                    if (cb_info_bits) info += cb_info_bits[cb.pool_cb];
+                    Tb &tb = tbs[cb.tb_slot];
+                    tb.last_done = std::max(tb.last_done, r.done_ns);
+                    last_done = std::max(last_done, r.done_ns);
+                    if (--tb.left == 0) {
+                        if (ntb < tb_cap && tb_lat_ns) tb_lat_ns[ntb] = tb.last_done - tb.arrival;
+                        ++ntb;
+                    }
+                    a_rel[i] = r.slot;
                 }
-                (void)syndrome_server_release(s, r.slot);
-                --outstanding;
+                if (int32_t r2 = syndrome_server_release_many(s, a_rel.data(), k)) return r2;
+                outstanding -= (long long)k;
             }
             if (!open && pending.empty() && outstanding == 0) break;
             if (now > t_end + 30e9) return SYNDROME_DEVICE_ERROR;  // drained for 30 s without finishing

@@ -449,6 +449,51 @@ class Server:
     def release(self, slot):
         check(lib().syndrome_server_release(self._h, slot), "server_release")
 
+    # -- batched forms (one server lock per call) --
+    def acquire_many(self, bg, z, n_filler=None):
+        """Slots for blocks (bg[i], z[i], n_filler[i]) in order until the
+        budget runs out: (uint32 slots, numpy views of their pinned input
+        staging), possibly shorter than bg (empty when busy)."""
+        bg = np.ascontiguousarray(bg, np.int32)
+        z = np.ascontiguousarray(z, np.int32)
+        nf = None if n_filler is None else np.ascontiguousarray(n_filler, np.int32)
+        slots = np.zeros(bg.size, np.uint32)
+        ptrs = (C.POINTER(C.c_float) * max(bg.size, 1))()
+        k = C.c_size_t(0)
+        rc = lib().syndrome_server_acquire_many(self._h, _ptr(bg), _ptr(z), None if nf is None else _ptr(nf),
+                                                bg.size, ptrs, _ptr(slots), C.byref(k))
+        if rc != _lib.BUSY:
+            check(rc, "server_acquire_many")
+        views = [np.ctypeslib.as_array(ptrs[i], shape=((68 if bg[i] == 0 else 52) * int(z[i]),))
+                 for i in range(k.value)]
+        return slots[:k.value], views
+
+    def submit_many(self, slots, srcs=None, tags=None):
+        """srcs: None (each slot's pinned staging) or device-accessible addresses (ints)."""
+        sl = np.ascontiguousarray(slots, np.uint32)
+        sp = None if srcs is None else np.ascontiguousarray(srcs, np.uint64)
+        tg = None if tags is None else np.ascontiguousarray(tags, np.uint64)
+        check(lib().syndrome_server_submit_many(self._h, _ptr(sl), None if sp is None else _ptr(sp),
+                                                None if tg is None else _ptr(tg), sl.size), "server_submit_many")
+
+    def try_recv_many(self, cap=256):
+        """Up to cap completed blocks as (slot, tag, iterations, hard copy, submit_ns, done_ns) tuples."""
+        out = (_lib.Served * cap)()
+        k = C.c_size_t(0)
+        rc = lib().syndrome_server_try_recv_many(self._h, out, cap, C.byref(k))
+        if rc == _lib.BUSY:
+            return []
+        check(rc, "server_try_recv_many")
+        res = []
+        for r in out[:k.value]:
+            hard = np.ctypeslib.as_array(r.hard, shape=(r.n,)).copy()
+            res.append((r.slot, r.tag, r.iterations, hard, r.submit_ns, r.done_ns))
+        return res
+
+    def release_many(self, slots):
+        sl = np.ascontiguousarray(slots, np.uint32)
+        check(lib().syndrome_server_release_many(self._h, _ptr(sl), sl.size), "server_release_many")
+
     def submit_tb(self, slots, tb_size, tb_out, tag):
         """Submit acquired, filled slots of one code as one transport block
         (syndrome_server_submit_tb).  tb_out: a HostPool of uint8, a device

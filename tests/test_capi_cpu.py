@@ -53,6 +53,11 @@ def test_null_decoder_is_reported_not_dereferenced(L):
     assert L.syndrome_ldpc_decode_5g(None, p, 16, 0, p, 16, p, 16, p, 16, 5, None) == _lib.NULL_POINTER
     assert L.syndrome_ldpc_decode(None, p, 16, p, 16, p, 16, p, 16, 5, None) == _lib.NULL_POINTER
     assert L.syndrome_ldpc_encode_batch_device(None, p, p, 1, None) == _lib.NULL_POINTER
+    n = C.c_size_t(0)
+    assert L.syndrome_server_acquire_many(None, p, p, None, 1, None, p, C.byref(n)) == _lib.NULL_POINTER
+    assert L.syndrome_server_submit_many(None, p, None, None, 1) == _lib.NULL_POINTER
+    assert L.syndrome_server_try_recv_many(None, p, 1, C.byref(n)) == _lib.NULL_POINTER
+    assert L.syndrome_server_release_many(None, p, 1) == _lib.NULL_POINTER
     L.syndrome_ldpc_decoder_destroy(None)  # no-op
     L.syndrome_ldpc_encoder_destroy(None)
 

@@ -296,3 +296,60 @@ def test_loadgen_transport_blocks():
     assert len(lat) > 10 and fails == 0
     assert ncb == sum(num[i % 5] for i in range(len(lat)))
     srv.close()
+
+
+@pytest.mark.parametrize("streams", [3, 0], ids=["lanes", "resident"])
+def test_server_batched_calls(oracle, streams):
+    """acquire_many / submit_many / try_recv_many / release_many: partial
+    acquisition at the budget, inputs from pinned staging and from HBM, the
+    same results as the oracle alone, all-or-nothing validation."""
+    import syndrome_amd
+    from syndrome_amd import Server, SyndromeError
+    syndrome_amd.lib()
+    say(f"[serve] batched calls, streams={streams}")
+    blocks = _blocks(oracle, 96, 23)
+    slot_bytes = 68 * 384 * 4 + 64
+    srv = Server(0.5, 8, hbm_budget_bytes=40 * slot_bytes, max_batch=32, streams=streams)
+    # validation first: a repeated slot submits nothing, an unacquired slot releases nothing
+    sl, views = srv.acquire_many([0, 1], [8, 8])
+    assert len(sl) == 2 and views[0].size == 68 * 8 and views[1].size == 52 * 8
+    with pytest.raises(SyndromeError):
+        srv.submit_many([sl[0], sl[0]])
+    with pytest.raises(SyndromeError):
+        srv.release_many([sl[0], next(x for x in range(40) if x not in sl)])
+    srv.release_many(sl)  # both still only acquired (the failed calls rolled back): released now
+    d_pool = torch.from_numpy(np.concatenate([llr for _, _, _, llr in blocks])).cuda()
+    offs = np.concatenate([[0], np.cumsum([llr.size for _, _, _, llr in blocks])[:-1]])
+    results, i, partial = {}, 0, False
+    t_end = time.time() + 60
+    while len(results) < len(blocks) and time.time() < t_end:
+        want = blocks[i:i + 24]
+        if want:
+            got, views = srv.acquire_many([b for b, _, _, _ in want], [z for _, z, _, _ in want],
+                                          [nf for _, _, nf, _ in want])
+            partial = partial or 0 < len(got) < len(want)
+            srcs = []
+            for j, slot in enumerate(got):
+                k = i + j
+                if k % 2:
+                    views[j][:] = blocks[k][3]
+                    srcs.append(0)  # pinned staging
+                else:
+                    srcs.append(d_pool.data_ptr() + 4 * int(offs[k]))
+            if len(got):
+                srv.submit_many(got, srcs, [i + j for j in range(len(got))])
+            i += len(got)
+        done = srv.try_recv_many(16)
+        for slot, tag, its, hard, t0, t1 in done:
+            results[tag] = (its, hard)
+        if done:
+            srv.release_many([d[0] for d in done])
+    assert len(results) == len(blocks), f"only {len(results)} of {len(blocks)} completed"
+    assert partial, "the 40-slot budget never cut an acquire_many short"
+    for tag, (b, z, nf, llr) in enumerate(blocks):
+        code = oracle.code_3gpp(b, z)
+        ri, _, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, llr, nf), 8)
+        its, hard = results[tag]
+        assert its == ri, f"block {tag} (BG{b + 1} Z={z}): iterations {its} != {ri}"
+        assert np.array_equal(hard, rh), f"block {tag}: hard decisions differ"
+    srv.close()
