@@ -619,7 +619,39 @@ def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, waves_per_cw, edges, iter
     print("\n".join(lines))
 
 
+def summarize_trace(trace_dir, out_md, title, cmd, extra_json=None):
+    """A kernel-trace-only profile summary: the stats table and, per kernel,
+    launches by grid size with their average duration; with extra_json (the
+    `--only` output of the same run) the measured figures beside it."""
+    import csv
+    from collections import defaultdict
+    stats = list(csv.DictReader(open(os.path.join(trace_dir, "run_kernel_stats.csv"))))
+    by = defaultdict(list)
+    for r in csv.DictReader(open(os.path.join(trace_dir, "run_kernel_trace.csv"))):
+        g = int(r["Grid_Size_X"]) if "Grid_Size_X" in r else int(r.get("Grid_Size", 0))
+        by[(r["Kernel_Name"][:70], g)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+    lines = [f"# rocprofv3 summary: {title}", "", "Command:", "", f"    {cmd}", "",
+             "## kernel trace (stats CSV, all dispatches)", "",
+             "| kernel | calls | average ns | min ns | max ns |", "|---|---|---|---|---|"]
+    for r in stats[:8]:
+        lines.append(f"| {r['Name'][:70]} | {r['Calls']} | {float(r['AverageNs']):.0f} | {r['MinNs']} | {r['MaxNs']} |")
+    lines += ["", "## by kernel and grid size (trace CSV, the 8 longest in total)", "",
+              "| kernel | grid (work-items) | launches | average ms |", "|---|---|---|---|"]
+    for (k, g), v in sorted(by.items(), key=lambda kv: -sum(kv[1]))[:8]:
+        lines.append(f"| {k} | {g} | {len(v)} | {np.mean(v):.4f} |")
+    if extra_json and os.path.exists(extra_json):
+        lines += ["", "## the same run's figures (`--only` JSON line)", "", "```",
+                  open(extra_json).read().strip()[:1500], "```"]
+    os.makedirs(os.path.dirname(out_md), exist_ok=True)
+    open(out_md, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-trace":
+        # bench.py --summarize-trace TRACE_DIR OUT_MD TITLE CMD [ONLY_JSON]
+        summarize_trace(*sys.argv[2:7])
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-sq":
         # bench.py --summarize-sq OUT_MD KERNEL WAVES_PER_CW EDGES MEAN_ITERS SQ_DIR1 CMD1 [SQ_DIR2 CMD2 ...]
         a = sys.argv[2:]

@@ -169,3 +169,11 @@ struct SynEncCode {
     short term_blk[SYN_ENC_MAX_TERMS];
     short term_shift[SYN_ENC_MAX_TERMS];
 };
+
+// One unit of device rate de-matching: output elements
+// [p0, p0 + SYN_DEMATCH_CHUNK) of code block `cb` (clipped to its N).
+#define SYN_DEMATCH_CHUNK 1024
+struct SynDematchChunk {
+    int cb;
+    int p0;
+};

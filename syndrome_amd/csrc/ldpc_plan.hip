@@ -21,8 +21,9 @@ const void *persist_kernel_f32();
 const void *persist_kernel_i8();
 int launch_reset(int *head, hipStream_t stream);
 int allow_max_dynamic_lds(const void *fn);
-int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, int count, int max_n, bool combine,
-                   int rv, hipStream_t stream);
+void dematch_chunks(const SynDematchCb *cbs, int count, std::vector<SynDematchChunk> &out);
+int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, const SynDematchChunk *chunks,
+                   int n_chunks, bool combine, int rv, hipStream_t stream);
 int launch_tb_check(const uint8_t *hard, const long long *offs, const SynTb *tbs, int n_tb, uint8_t *tb_out,
                     uint8_t *tb_ok, uint8_t *cb_ok, hipStream_t stream);
 }  // namespace syn
@@ -50,7 +51,8 @@ struct SyndromeLdpcPlan {
     int *d_ids = nullptr, *d_nfill = nullptr, *d_head = nullptr;
     long long *d_offs = nullptr;
     SynDematchCb *d_dm = nullptr;
-    int dm_max_n = 0;
+    SynDematchChunk *d_dm_chunks = nullptr;
+    int n_dm_chunks = 0;
     SynTb *d_tbs = nullptr;
     int n_tbs = 0;
     std::mutex mu;
@@ -254,7 +256,8 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *p)
     try {
         DeviceScope ds(p->device);
         for (void *q : {(void *)p->d_codes, (void *)p->d_items, (void *)p->d_ids, (void *)p->d_nfill,
-                        (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm, (void *)p->d_tbs})
+                        (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm, (void *)p->d_dm_chunks,
+                        (void *)p->d_tbs})
             if (q) (void)hipFree(q);
     } catch (...) {
     }
@@ -340,18 +343,20 @@ int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *p, const int64_t *e_off
     return guard([&]() -> int32_t {
         if (!p || !e_offsets || !e_lengths) return SYNDROME_NULL_POINTER;
         std::vector<SynDematchCb> v(p->bg.size());
-        int max_n = 0;
         for (size_t i = 0; i < v.size(); ++i) {
             const int q = qm ? qm[i] : 1, r = rv ? rv[i] : 0;
             if (q <= 0 || e_lengths[i] < 0 || e_lengths[i] % q || r < 0 || r > 3 || e_offsets[i] < 0)
                 return SYNDROME_INVALID_PARAM;
             v[i] = SynDematchCb{e_offsets[i], p->offs[i], e_lengths[i], q, p->z[i], p->bg[i], r, p->nfill[i]};
-            max_n = std::max(max_n, (p->bg[i] == 0 ? 68 : 52) * p->z[i]);
         }
+        std::vector<SynDematchChunk> ch;
+        syn::dematch_chunks(v.data(), (int)v.size(), ch);
+        if (ch.size() > (size_t)INT_MAX) return SYNDROME_INVALID_PARAM;
         std::lock_guard<std::mutex> lk(p->mu);
         DeviceScope ds(p->device);
         if (int32_t rc = upload(&p->d_dm, v)) return rc;
-        p->dm_max_n = max_n;
+        if (int32_t rc = upload(&p->d_dm_chunks, ch)) return rc;
+        p->n_dm_chunks = (int)ch.size();
         return SYNDROME_OK;
     });
 }
@@ -362,7 +367,7 @@ int32_t syndrome_ldpc_plan_dematch(SyndromeLdpcPlan *p, const float *d_e_llr, fl
         if (!p || !d_e_llr || !d_llr_out) return SYNDROME_NULL_POINTER;
         if (!p->d_dm) return SYNDROME_INVALID_PARAM;
         DeviceScope ds(p->device);
-        return syn::launch_dematch(d_e_llr, d_llr_out, p->d_dm, (int)p->bg.size(), p->dm_max_n, false, -1,
+        return syn::launch_dematch(d_e_llr, d_llr_out, p->d_dm, p->d_dm_chunks, p->n_dm_chunks, false, -1,
                                    (hipStream_t)stream)
                    ? SYNDROME_DEVICE_ERROR
                    : SYNDROME_OK;
@@ -420,7 +425,7 @@ int32_t syndrome_ldpc_plan_harq_combine(SyndromeLdpcPlan *p, const float *d_e_ll
         if (!p || !d_e_llr || !d_acc) return SYNDROME_NULL_POINTER;
         if (!p->d_dm || rv > 3) return SYNDROME_INVALID_PARAM;  // rv < 0: each block's own
         DeviceScope ds(p->device);
-        return syn::launch_dematch(d_e_llr, d_acc, p->d_dm, (int)p->bg.size(), p->dm_max_n, true, rv < 0 ? -1 : rv,
+        return syn::launch_dematch(d_e_llr, d_acc, p->d_dm, p->d_dm_chunks, p->n_dm_chunks, true, rv < 0 ? -1 : rv,
                                    (hipStream_t)stream)
                    ? SYNDROME_DEVICE_ERROR
                    : SYNDROME_OK;
