@@ -183,11 +183,11 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         const SynCode *code = codes + s_code;
         const GroupIO io{s_count, 0, s_slots, offs, nullptr, s_src, s_nf};
         if (code->kind == SYN_KIND_BG1) {
-            if (code->z == 384) fast_group<SYN_KIND_BG1, 384, false, false>(code, 384, s_count, io, la);
-            else fast_group<SYN_KIND_BG1, 0, false, false>(code, code->z, s_count, io, la);
+            if (code->z == 384) fast_group<SYN_KIND_BG1, 384, false, true>(code, 384, s_count, io, la);
+            else fast_group<SYN_KIND_BG1, 0, false, true>(code, code->z, s_count, io, la);
         } else {
-            if (code->z == 208) fast_group<SYN_KIND_BG2, 208, false, false>(code, 208, s_count, io, la);
-            else fast_group<SYN_KIND_BG2, 0, false, false>(code, code->z, s_count, io, la);
+            if (code->z == 208) fast_group<SYN_KIND_BG2, 208, false, true>(code, 208, s_count, io, la);
+            else fast_group<SYN_KIND_BG2, 0, false, true>(code, code->z, s_count, io, la);
         }
         if (s_any_tb) {
             __syncthreads();  // the decode is finished with the posterior in LDS
