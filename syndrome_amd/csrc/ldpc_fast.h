@@ -382,11 +382,13 @@ template <class G, int ZC, int L, class T>
 __device__ __forceinline__ bool row_parity(const Lane &ln, const Geo<G, ZC> &geo)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
-    bool p = false;
+    T v[D];  // every load of the row issued before the first compare
     SFor<0, D>::run([&](auto ec) {
         constexpr int e = decltype(ec)::value;
-        p ^= *(const T *)geo.template addr<B + e, sizeof(T)>(ln) < (T)0;
+        v[e] = *(const T *)geo.template addr<B + e, sizeof(T)>(ln);
     });
+    bool p = false;
+    SFor<0, D>::run([&](auto ec) { p ^= v[decltype(ec)::value] < (T)0; });
     return p;
 }
 
@@ -394,7 +396,8 @@ template <class G, int ZC, int L0, int L1, class T>
 __device__ __forceinline__ bool rows_parity(const Lane &ln, const Geo<G, ZC> &geo)
 {
     bool p = false;
-    SFor<L0, L1>::run([&](auto lc) { p = p || row_parity<G, ZC, decltype(lc)::value, T>(ln, geo); });
+    // every row of the segment, without a branch between rows
+    SFor<L0, L1>::run([&](auto lc) { p = p | row_parity<G, ZC, decltype(lc)::value, T>(ln, geo); });
     return p;
 }
 
