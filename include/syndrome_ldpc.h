@@ -147,6 +147,16 @@ int32_t syndrome_ldpc_decode_batch_i8(const SyndromeLdpcDecoder *decoder, const 
 int32_t syndrome_ldpc_launch_shape(const SyndromeLdpcDecoder *decoder, int32_t int8, int32_t plain,
                                    int32_t *out_codewords_per_group, int32_t *out_threads, int32_t *out_lds_bytes);
 
+/* QcLdpcDecoder::decode_layered_offset_min_sum_traced (qc_ldpc.rs:853-874):
+ * as syndrome_ldpc_decode, and row i of trace [iterations][N] receives the
+ * posterior after layered pass i + 1, before that pass's syndrome check (the
+ * reference's on_iteration(i + 1, llr), qc_ldpc.rs:1208).  Rows
+ * 0 .. *out_iterations_used - 1 are written; a binding calls its callback
+ * once per row.  trace_len < iterations * N -> BUFFER_TOO_SMALL. */
+int32_t syndrome_ldpc_decode_traced(SyndromeLdpcDecoder *decoder, float *llr, size_t llr_len, float *edge_r,
+                                    size_t edge_r_len, float *layer_scratch, size_t layer_scratch_len,
+                                    uint8_t *hard_output, size_t hard_output_len, size_t iterations, float *trace,
+                                    size_t trace_len, size_t *out_iterations_used);
 /* quantize_llr_i16 (quantize.rs:331-341) on device: f32 [n] -> i16 [n]. */
 int32_t syndrome_ldpc_quantize_i16_device(const float *d_llr, int16_t *d_out, size_t n, float scale, void *stream);
 

@@ -56,6 +56,7 @@ SIGNATURES = {
     "syndrome_ldpc_decoder_check_node_count": (SZ, [P]),
     "syndrome_ldpc_decoder_offset_beta": (C.c_float, [P]),
     "syndrome_ldpc_decode": (I32, [P, P, SZ, P, SZ, P, SZ, P, SZ, SZ, C.POINTER(SZ)]),
+    "syndrome_ldpc_decode_traced": (I32, [P, P, SZ, P, SZ, P, SZ, P, SZ, SZ, P, SZ, C.POINTER(SZ)]),
     "syndrome_ldpc_decode_i8": (I32, [P, P, SZ, P, SZ, P, SZ, P, SZ, SZ, C.c_float, C.c_int16, C.POINTER(SZ)]),
     "syndrome_ldpc_decode_5g_i8": (I32, [P, P, SZ, SZ, P, SZ, P, SZ, P, SZ, SZ, C.c_float, C.c_int16,
                                          C.POINTER(SZ)]),

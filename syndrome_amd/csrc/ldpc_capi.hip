@@ -133,11 +133,13 @@ int32_t launch_shape(const SyndromeLdpcDecoder *d, bool i8, bool plain, Shape *s
 
 int32_t run_batch(const SyndromeLdpcDecoder *d, bool i8, const void *in, void *post, void *edge, uint8_t *hard,
                   int32_t *iters, size_t batch, size_t iterations, size_t n_filler, float scale, int16_t app_clamp,
-                  uint32_t flags, hipStream_t stream)
+                  uint32_t flags, hipStream_t stream, float *trace = nullptr)
 {
     if (batch == 0) return SYNDROME_OK;
     if (batch > (size_t)INT_MAX / 2) return SYNDROME_INVALID_PARAM;
-    const bool plain = (flags & SYNDROME_BATCH_PLAIN) != 0;
+    // a traced decode runs on the plain kernel (same bits; the trace store
+    // stays out of the register-resident kernels' register budget)
+    const bool plain = (flags & SYNDROME_BATCH_PLAIN) != 0 || (trace && !i8);
     if (!in || !iters) return SYNDROME_NULL_POINTER;
     if ((flags & SYNDROME_BATCH_WRITE_POST) && !post) return SYNDROME_NULL_POINTER;
     if ((flags & SYNDROME_BATCH_WRITE_HARD) && !hard) return SYNDROME_NULL_POINTER;
@@ -160,6 +162,10 @@ int32_t run_batch(const SyndromeLdpcDecoder *d, bool i8, const void *in, void *p
         if (syn::code_n(d->code) % 4 == 0 && al(in) && (!(flags & SYNDROME_BATCH_WRITE_POST) || al(post)) &&
             (!(flags & SYNDROME_BATCH_WRITE_HARD) || al(hard)))
             a.flags |= SYN_F_ALIGNED;  // contiguous codewords: offsets b * N
+    }
+    if (trace && !i8) {
+        a.trace = trace;
+        a.flags |= SYN_F_TRACE;
     }
     a.n_filler = (long long)std::min(n_filler, (size_t)LLONG_MAX);
     a.beta = d->beta;
@@ -193,7 +199,7 @@ int32_t ensure_stage(SyndromeLdpcDecoder *d, size_t bytes)
 // or 2 (i16 posterior, i8 messages).
 int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, size_t edge_elems, uint8_t *hard,
                     size_t iterations, size_t n_filler, bool init5g, float scale, int16_t app_clamp,
-                    size_t *out_used)
+                    size_t *out_used, float *trace = nullptr)
 {
     const SynCode &c = d->code;
     const size_t n = syn::code_n(c), ez = syn::code_ez(c);
@@ -201,7 +207,8 @@ int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, si
     const size_t off_edge = (n * pe + 255) & ~(size_t)255;
     const size_t off_hard = off_edge + ((ez * me + 255) & ~(size_t)255);
     const size_t off_iter = off_hard + ((n + 255) & ~(size_t)255);
-    const size_t total = off_iter + 256;
+    const size_t off_trace = off_iter + 256;
+    const size_t total = off_trace + (trace ? iterations * n * 4 : 0);
     std::lock_guard<std::mutex> lk(d->mu);
     DeviceScope ds(d->device);
     if (int32_t rc = ensure_stage(d, total)) return rc;
@@ -210,7 +217,8 @@ int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, si
                            SYNDROME_BATCH_WRITE_HARD | SYNDROME_BATCH_WRITE_EDGE;
     if (hipMemcpyAsync(base, llr, n * pe, hipMemcpyHostToDevice, d->stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
     int32_t rc = run_batch(d, i8, base, base, base + off_edge, (uint8_t *)(base + off_hard), (int32_t *)(base + off_iter),
-                           1, iterations, n_filler, scale, app_clamp, flags, d->stream);
+                           1, iterations, n_filler, scale, app_clamp, flags, d->stream,
+                           trace ? (float *)(base + off_trace) : nullptr);
     if (rc) return rc;
     int32_t used = 0;
     if (hipMemcpyAsync(llr, base, n * pe, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
@@ -218,6 +226,11 @@ int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, si
         hipMemcpyAsync(hard, base + off_hard, n, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
         hipMemcpyAsync(&used, base + off_iter, 4, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
         hipStreamSynchronize(d->stream) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    if (trace && used > 0 &&
+        (hipMemcpyAsync(trace, base + off_trace, (size_t)used * n * 4, hipMemcpyDeviceToHost, d->stream) !=
+             hipSuccess ||
+         hipStreamSynchronize(d->stream) != hipSuccess))
         return SYNDROME_DEVICE_ERROR;
     (void)edge_elems;
     if (out_used) *out_used = (size_t)used;
@@ -391,6 +404,24 @@ int32_t syndrome_ldpc_decode(SyndromeLdpcDecoder *d, float *llr, size_t llr_len,
         if (llr_len != n || hard_len != n) return SYNDROME_INVALID_PARAM;  // qc_ldpc.rs:980-1004
         if (edge_r_len < syn::code_ez(c) || layer_scratch_len < syn::code_layer_buf(c)) return SYNDROME_BUFFER_TOO_SMALL;
         return host_decode(d, false, llr, edge_r, edge_r_len, hard, iterations, 0, false, 0.f, 0, out_used);
+    });
+}
+
+int32_t syndrome_ldpc_decode_traced(SyndromeLdpcDecoder *d, float *llr, size_t llr_len, float *edge_r,
+                                    size_t edge_r_len, float *layer_scratch, size_t layer_scratch_len, uint8_t *hard,
+                                    size_t hard_len, size_t iterations, float *trace, size_t trace_len,
+                                    size_t *out_used)
+{
+    return guard([&]() -> int32_t {
+        if (!d || !llr || !edge_r || !layer_scratch || !hard || (!trace && iterations)) return SYNDROME_NULL_POINTER;
+        const SynCode &c = d->code;
+        const size_t n = syn::code_n(c);
+        if (llr_len != n || hard_len != n) return SYNDROME_INVALID_PARAM;  // qc_ldpc.rs:980-1004
+        if (edge_r_len < syn::code_ez(c) || layer_scratch_len < syn::code_layer_buf(c)) return SYNDROME_BUFFER_TOO_SMALL;
+        if (iterations > (size_t)INT_MAX || (iterations && trace_len / iterations < n))
+            return SYNDROME_BUFFER_TOO_SMALL;
+        return host_decode(d, false, llr, edge_r, edge_r_len, hard, iterations, 0, false, 0.f, 0, out_used,
+                           iterations ? trace : nullptr);
     });
 }
 

@@ -55,6 +55,7 @@ struct SynCode {
 #define SYN_F_QUANTIZE_IN 16u   // i8 path: input is f32 LLR, quantize_llr_i16 on load
 #define SYN_F_PUBLIC 31u        // the flags a caller may pass; the rest are set by the launchers
 #define SYN_F_ALIGNED 32u       // element offsets and N are multiples of 4, in/post/hard 16-byte aligned
+#define SYN_F_TRACE 64u         // f32 path: posterior after every pass to `trace`
 
 // Per-launch arguments shared by every decode kernel.
 struct SynLaunch {
@@ -72,6 +73,7 @@ struct SynLaunch {
     float scale;           // i8 path: quantizer scale
     int beta_q;            // i8 path: quantized offset
     int app_clamp;         // i8 path: max(app_clamp, 0)
+    float *trace;          // SYN_F_TRACE: f32 [B][iterations][N], row it = posterior after pass it+1
 };
 
 // One work item of a work-queue launch: `count` codewords (ids

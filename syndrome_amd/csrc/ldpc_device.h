@@ -350,6 +350,19 @@ __device__ __forceinline__ void flags_init(CwFlags f, int cws, const GroupIO &io
     }
 }
 
+// decode_layered_offset_min_sum_traced (qc_ldpc.rs:853-874, the
+// on_iteration call at :1208): after pass it + 1, before its syndrome check,
+// the posterior of every codeword still decoding goes to its trace row.
+// P = the lane's codeword posterior (f32, column-major as in HBM).  Used by
+// the plain kernel, which every traced decode runs on.
+__device__ __forceinline__ void trace_store(const SynLaunch &a, const float *P, int N, int Z, int z, bool act,
+                                            const GroupIO &io, int cl, int it)
+{
+    if (!(a.flags & SYN_F_TRACE) || !act || cl >= io.count) return;
+    float *dst = a.trace + ((size_t)io.id(cl) * (size_t)a.iterations + (size_t)it) * (size_t)N;
+    for (int v = z; v < N; v += Z) dst[v] = P[v];
+}
+
 __device__ __forceinline__ void store_iters(const SynLaunch &a, CwFlags f, const GroupIO &io)
 {
     if ((int)threadIdx.x < io.count) a.iters[io.id(threadIdx.x)] = f.used[threadIdx.x];

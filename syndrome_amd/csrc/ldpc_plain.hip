@@ -75,6 +75,7 @@ __global__ void __launch_bounds__(1024) plain_f32(const SynCode *__restrict__ co
             }
             __syncthreads();
         }
+        trace_store(a, P, N, Z, z, act, io, cl, it);
         // end-of-iteration syndrome (check_syndrome_f32, qc_ldpc.rs:1231-1252)
         if (act) {
             int fail = 0;

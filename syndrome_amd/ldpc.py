@@ -123,6 +123,22 @@ class QcLdpcDecoder:
                                          hard_output.size, iterations, C.byref(used)), "decode")
         return used.value
 
+    def decode_layered_offset_min_sum_traced(self, llr, edge_r, layer_scratch, hard_output, iterations,
+                                             on_iteration):
+        """QcLdpcDecoder::decode_layered_offset_min_sum_traced (qc_ldpc.rs:853-874):
+        on_iteration(i, posterior) once per completed pass i = 1, 2, ..., with
+        the posterior before that pass's syndrome check.  Returns iterations used."""
+        n = self.variable_node_count()
+        trace = np.zeros((max(iterations, 1), n), np.float32)
+        used = C.c_size_t(0)
+        check(lib().syndrome_ldpc_decode_traced(self._h, _ptr(llr), llr.size, _ptr(edge_r), edge_r.size,
+                                                _ptr(layer_scratch), layer_scratch.size, _ptr(hard_output),
+                                                hard_output.size, iterations, _ptr(trace), trace.size,
+                                                C.byref(used)), "decode_traced")
+        for i in range(used.value):
+            on_iteration(i + 1, trace[i])
+        return used.value
+
     def decode_layered_offset_min_sum_i8(self, app, edge_r, layer_scratch, hard_output, iterations, quant=None):
         """QcLdpcDecoder::decode_layered_offset_min_sum_i8 (qc_ldpc.rs:1345-1363)."""
         q = quant or QuantParams()
