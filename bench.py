@@ -325,6 +325,11 @@ def bench_uneven(torch, args):
     full = lambda: (plan.dematch(d_e, d_llr), plan.run(d_llr, d_post, d_hard, d_it, args.iters, flags))
     t = _timed(full, 5, torch)
     t_dec = _timed(lambda: plan.run(d_llr, d_post, d_hard, d_it, args.iters, flags), 5, torch)
+    # the de-match kernel alone (HIP events on the launch stream); its
+    # algorithmic bytes: E f32 read + N f32 written per block
+    _, per_dm = time_launches(lambda: plan.dematch(d_e, d_llr), 20, 2, torch)
+    t_dm = float(np.mean(per_dm)) / 1e3
+    dm_bytes = 4.0 * float(e_len.sum() + n.sum())
     hard = d_hard.cpu().numpy()
     its = d_it.cpu().numpy()
     errs = sum(bool((hard[offs[i]:offs[i] + k[i]] != info[i_off[i]:i_off[i] + k[i]]).any()) for i in range(B))
@@ -357,7 +362,10 @@ def bench_uneven(torch, args):
             "value": round(bits / t / 1e9, 4), "unit": "Gbit/s", "ms_dematch_plus_decode": round(t * 1e3, 3),
             "decode_only_value": round(bits / t_dec / 1e9, 4), "per_config_launches_value": round(bits / t_per / 1e9, 4),
             "per_config_launches": len(groups), "int8_value": round(bits / t8 / 1e9, 4), "tuned_block": block,
-            "shape": plan.shape(), "mean_iterations": round(float(its.mean()), 3), "bler": round(errs / B, 4)}
+            "shape": plan.shape(), "mean_iterations": round(float(its.mean()), 3), "bler": round(errs / B, 4),
+            "dematch": {"ms_per_launch": round(t_dm * 1e3, 4), "algorithmic_bytes": dm_bytes,
+                        "roofline": {"bound": "hbm", "achieved": round(dm_bytes / t_dm / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": round(dm_bytes / t_dm / 1e9 / HBM_PEAK_GBS, 4)}}}
 
 
 def serving_traffic(torch, args, n_tb=96):

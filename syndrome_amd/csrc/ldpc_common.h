@@ -156,6 +156,16 @@ struct SynDematchCb {
     int bg;             // 0 = BG1, 1 = BG2
     int rv;             // redundancy version 0..3
     int n_filler;
+    // derived once on the host (dematch_derive), read by every workgroup
+    int n;              // decoder input length: the circular buffer + 2Z
+    int fs, fe;         // filler range in the circular buffer
+    int m;              // selectable (non-filler) positions per cycle
+    int rows;           // e / qm
+    int kmax;           // selections that follow the walk: min(e, 2m + 1)
+    int bncb;           // non-filler positions in the whole buffer
+    int qsh;            // log2(qm) when qm is a power of two, else -1
+    int k0[4];          // walk start per redundancy version
+    int bk0[4];         // non-filler positions before k0[rv]
 };
 
 // Parity solve of the device encoder for one code: block rows [0, core)

@@ -37,6 +37,9 @@ namespace syn {
 namespace {
 
 constexpr int DM_THREADS = 256;
+// Workgroups of a launch at most (one per chunk below that; they stride
+// over the chunk table above it).
+constexpr int DM_GRID = 1 << 20;
 
 __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ e_llr, float *__restrict__ out,
                                                        const SynDematchCb *__restrict__ cbs,
@@ -45,30 +48,25 @@ __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ 
 {
     for (int ci = blockIdx.x; ci < n_chunks; ci += gridDim.x) {
         const SynDematchChunk ch = chunks[ci];
-        const SynDematchCb cb = cbs[ch.cb];
-        const int z = cb.z;
-        const int nb = cb.bg == 0 ? 66 : 50, kb = cb.bg == 0 ? 22 : 10;
-        const int ncb = nb * z, n = ncb + 2 * z, k = kb * z;
-        const int kp = cb.n_filler >= k ? 0 : k - cb.n_filler;
-        const int fs = kp >= 2 * z ? kp - 2 * z : 0;  // filler range in the circular buffer
-        const int fe = k >= 2 * z ? k - 2 * z : 0;
-        const int m = ncb - (fe - fs);                 // selectable positions per cycle
-        const int j0tab[2][4] = {{0, 17, 33, 56}, {0, 13, 25, 43}};
-        const int k0 = j0tab[cb.bg][(rv >= 0 ? rv : cb.rv) & 3] * z;
-        const int qm = cb.qm, e = cb.e;
-        const int rows = qm > 0 ? e / qm : 0;
-        const int kmax = e < 2 * m + 1 ? e : 2 * m + 1;  // selections that follow the walk
+        const SynDematchCb &cb = cbs[ch.cb];
+        const int z = cb.z, n = cb.n, fs = cb.fs, fe = cb.fe, m = cb.m, qm = cb.qm, e = cb.e;
+        const int rows = cb.rows, kmax = cb.kmax, bncb = cb.bncb, qsh = cb.qsh;
+        const int rvi = (rv >= 0 ? rv : cb.rv) & 3;
+        const int k0 = cb.k0[rvi], bk0 = cb.bk0[rvi];
+        const long long out_off = cb.out_off;
         const float *el = e_llr + cb.e_off;
         // non-filler positions in [0, x)
         auto below = [&](int x) {
             const int f = x <= fs ? 0 : (x >= fe ? fe - fs : x - fs);
             return x - f;
         };
-        const int bk0 = below(k0), bncb = below(ncb);
+        // selection k -> received index (k % qm) * rows + k / qm, without a
+        // division when qm is a power of two
+        auto q_of = [&](int kk) { return qsh >= 0 ? (kk & (qm - 1)) * rows + (kk >> qsh) : (kk % qm) * rows + kk / qm; };
         const int p_end = ch.p0 + SYN_DEMATCH_CHUNK < n ? ch.p0 + SYN_DEMATCH_CHUNK : n;
         for (int p = ch.p0 + (int)threadIdx.x; p < p_end; p += DM_THREADS) {
             const int c = p - 2 * z;
-            float v = (combine && c >= 0) ? out[cb.out_off + p] : 0.f;
+            float v = (combine && c >= 0) ? out[out_off + p] : 0.f;
             if (c >= 0 && m > 0 && qm > 0) {
                 const bool sel = !(c >= fs && c < fe);
                 int ks[3], nk = 0, r = -1;
@@ -85,22 +83,49 @@ __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ 
                     }
                 } else {
                     int qs[3];
-                    for (int i = 0; i < nk; ++i) qs[i] = (ks[i] % qm) * rows + ks[i] / qm;
-                    for (int i = 1; i < nk; ++i)  // insertion sort of at most three
-                        for (int j = i; j > 0 && qs[j - 1] > qs[j]; --j) {
-                            const int t = qs[j];
-                            qs[j] = qs[j - 1];
-                            qs[j - 1] = t;
-                        }
+                    for (int i = 0; i < nk; ++i) qs[i] = q_of(ks[i]);
+                    if (qm > 1)  // with qm == 1, q = k is already increasing
+                        for (int i = 1; i < nk; ++i)  // insertion sort of at most three
+                            for (int j = i; j > 0 && qs[j - 1] > qs[j]; --j) {
+                                const int t = qs[j];
+                                qs[j] = qs[j - 1];
+                                qs[j - 1] = t;
+                            }
                     for (int i = 0; i < nk; ++i) v += el[qs[i]];
                 }
             }
-            out[cb.out_off + p] = v;
+            out[out_off + p] = v;
         }
     }
 }
 
 }  // namespace
+
+// Fills the derived fields of a code block (its circular-buffer geometry,
+// TS 38.212 5.4.2.1 as rate_matching.rs:181-260 walks it).
+void dematch_derive(SynDematchCb &cb)
+{
+    const int z = cb.z;
+    const int nb = cb.bg == 0 ? 66 : 50, kb = cb.bg == 0 ? 22 : 10;
+    const int ncb = nb * z, k = kb * z;
+    const int kp = cb.n_filler >= k ? 0 : k - cb.n_filler;
+    cb.n = ncb + 2 * z;
+    cb.fs = kp >= 2 * z ? kp - 2 * z : 0;
+    cb.fe = k >= 2 * z ? k - 2 * z : 0;
+    cb.m = ncb - (cb.fe - cb.fs);
+    cb.rows = cb.qm > 0 ? cb.e / cb.qm : 0;
+    cb.kmax = cb.e < 2 * cb.m + 1 ? cb.e : 2 * cb.m + 1;
+    const auto below = [&](int x) { return x - (x <= cb.fs ? 0 : (x >= cb.fe ? cb.fe - cb.fs : x - cb.fs)); };
+    cb.bncb = below(ncb);
+    cb.qsh = -1;
+    for (int b = 0; b < 31; ++b)
+        if (cb.qm == (1 << b)) cb.qsh = b;
+    static const int j0[2][4] = {{0, 17, 33, 56}, {0, 13, 25, 43}};  // TS 38.212 Table 5.4.2.1-2
+    for (int r = 0; r < 4; ++r) {
+        cb.k0[r] = j0[cb.bg][r] * z;
+        cb.bk0[r] = below(cb.k0[r]);
+    }
+}
 
 // The chunk table of `count` code blocks with N_i = (bg ? 52 : 68) * z_i.
 void dematch_chunks(const SynDematchCb *cbs, int count, std::vector<SynDematchChunk> &out)
@@ -116,7 +141,7 @@ int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, cons
                    int n_chunks, bool combine, int rv, hipStream_t stream)
 {
     if (n_chunks <= 0) return SYN_OK;
-    const unsigned grid = (unsigned)(n_chunks < (1 << 20) ? n_chunks : (1 << 20));
+    const unsigned grid = (unsigned)(n_chunks < DM_GRID ? n_chunks : DM_GRID);
     hipLaunchKernelGGL(dematch, dim3(grid), dim3(DM_THREADS), 0, stream, e_llr, out, cbs, chunks, n_chunks,
                        combine ? 1 : 0, rv);
     return hipGetLastError() == hipSuccess ? SYN_OK : SYN_DEVICE_ERROR;

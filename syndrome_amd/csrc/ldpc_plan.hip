@@ -21,6 +21,7 @@ const void *persist_kernel_f32();
 const void *persist_kernel_i8();
 int launch_reset(int *head, hipStream_t stream);
 int allow_max_dynamic_lds(const void *fn);
+void dematch_derive(SynDematchCb &cb);
 void dematch_chunks(const SynDematchCb *cbs, int count, std::vector<SynDematchChunk> &out);
 int launch_dematch(const float *e_llr, float *out, const SynDematchCb *cbs, const SynDematchChunk *chunks,
                    int n_chunks, bool combine, int rv, hipStream_t stream);
@@ -348,6 +349,7 @@ int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *p, const int64_t *e_off
             if (q <= 0 || e_lengths[i] < 0 || e_lengths[i] % q || r < 0 || r > 3 || e_offsets[i] < 0)
                 return SYNDROME_INVALID_PARAM;
             v[i] = SynDematchCb{e_offsets[i], p->offs[i], e_lengths[i], q, p->z[i], p->bg[i], r, p->nfill[i]};
+            syn::dematch_derive(v[i]);
         }
         std::vector<SynDematchChunk> ch;
         syn::dematch_chunks(v.data(), (int)v.size(), ch);
