@@ -184,7 +184,7 @@ struct SynEncCode {
 
 // One unit of device rate de-matching: output elements
 // [p0, p0 + SYN_DEMATCH_CHUNK) of code block `cb` (clipped to its N).
-#define SYN_DEMATCH_CHUNK 1024
+#define SYN_DEMATCH_CHUNK 4096
 struct SynDematchChunk {
     int cb;
     int p0;

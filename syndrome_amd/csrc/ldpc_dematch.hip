@@ -24,8 +24,9 @@
 //
 // Work decomposition: a table of chunks of SYN_DEMATCH_CHUNK output
 // elements of one code block each (built once per plan); one workgroup per
-// chunk, its lanes on consecutive elements, so every launch has as many
-// workgroups as the output needs and no empty ones.
+// chunk, so every launch has as many workgroups as the output needs and no
+// empty ones.  A lane owns 4 consecutive elements and writes them with one
+// 16-byte store when the block's offset allows.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -64,7 +65,8 @@ __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ 
         // division when qm is a power of two
         auto q_of = [&](int kk) { return qsh >= 0 ? (kk & (qm - 1)) * rows + (kk >> qsh) : (kk % qm) * rows + kk / qm; };
         const int p_end = ch.p0 + SYN_DEMATCH_CHUNK < n ? ch.p0 + SYN_DEMATCH_CHUNK : n;
-        for (int p = ch.p0 + (int)threadIdx.x; p < p_end; p += DM_THREADS) {
+        // one element: its value before the LLRs, plus its LLRs in q order
+        auto elem = [&](int p) {
             const int c = p - 2 * z;
             float v = (combine && c >= 0) ? out[out_off + p] : 0.f;
             if (c >= 0 && m > 0 && qm > 0) {
@@ -94,7 +96,17 @@ __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ 
                     for (int i = 0; i < nk; ++i) v += el[qs[i]];
                 }
             }
-            out[out_off + p] = v;
+            return v;
+        };
+        // 4 consecutive elements per lane, one 16-byte store when aligned
+        const bool vec = (out_off & 3) == 0;
+        for (int p = ch.p0 + 4 * (int)threadIdx.x; p < p_end; p += 4 * DM_THREADS) {
+            if (vec && p + 3 < p_end) {
+                const float v0 = elem(p), v1 = elem(p + 1), v2 = elem(p + 2), v3 = elem(p + 3);
+                *reinterpret_cast<float4 *>(out + out_off + p) = make_float4(v0, v1, v2, v3);
+            } else {
+                for (int j = 0; j < 4 && p + j < p_end; ++j) out[out_off + p + j] = elem(p + j);
+            }
         }
     }
 }
