@@ -623,6 +623,12 @@ def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, waves_per_cw, edges, iter
         lines.append(f"| {k} | {mean[k]:,.0f}{note} |")
     lines += ["", f"VALU issue share: {valu:.3e} VALU instructions x 4 cycles over {cus} CUs x 4 SIMDs and the "
                   f"{kern_s * 1e3:.3f} ms kernel ({clock_ghz} GHz) = {issue * 100:.0f} % of the device's VALU issue slots."]
+    lds = mean.get("SQ_INSTS_LDS", 0.0)
+    if lds:
+        lds_tbs = lds * 64 * 4 / kern_s / 1e12  # every LDS instruction of this kernel moves 4 bytes per lane
+        lds_peak = cus * 128 * clock_ghz * 1e9 / 1e12
+        lines += [f"LDS throughput: {lds:.3e} LDS instructions x 64 lanes x 4 B over the kernel = {lds_tbs:.1f} TB/s, "
+                  f"{lds_tbs / lds_peak * 100:.0f} % of the {lds_peak:.1f} TB/s the CUs' LDS can serve (128 B per CU per cycle)."]
     open(out_md, "a").write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
