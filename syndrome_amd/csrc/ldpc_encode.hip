@@ -41,7 +41,7 @@ namespace syn {
 namespace {
 
 constexpr int ENC_THREADS = 256;
-constexpr int ENC_WAVES = ENC_THREADS / 64;
+constexpr int ENC_SMALL_N = 8192;  // codeword bits up to which a workgroup is one wave
 
 // Sizes of one code, shared by every phase.
 struct Shape {
@@ -81,6 +81,7 @@ __host__ __device__ inline size_t lds_words_end(int n_edges, int n_terms, int mb
            ((size_t)(kb + 2 * core) * len + (size_t)core * nw) * sizeof(uint32_t);
 }
 
+template <int T>
 __device__ __forceinline__ Lds carve(uint8_t *base, const Shape &s, const SynCode *code, const SynEncCode *enc)
 {
     Lds l;
@@ -94,16 +95,16 @@ __device__ __forceinline__ Lds carve(uint8_t *base, const Shape &s, const SynCod
     l.wsyn = l.wr + (s.kb + s.core) * s.len;                       // core blocks
     l.syn = l.wsyn + s.core * s.len;                               // core * nw words
     l.bytes = base + ((lds_words_end(s.n_edges, s.n_terms, s.mb, s.core, s.kb, s.len, s.nw) + 15) & ~(size_t)15);
-    for (int i = (int)threadIdx.x; i < s.n_edges; i += ENC_THREADS) {
+    for (int i = (int)threadIdx.x; i < s.n_edges; i += T) {
         l.e_col[i] = (short)code->cols[i];
         l.e_sh[i] = (short)code->shifts[i];
     }
-    for (int i = (int)threadIdx.x; i < s.n_terms; i += ENC_THREADS) {
+    for (int i = (int)threadIdx.x; i < s.n_terms; i += T) {
         l.t_blk[i] = enc->term_blk[i];
         l.t_sh[i] = enc->term_shift[i];
     }
-    for (int i = (int)threadIdx.x; i <= s.mb; i += ENC_THREADS) l.row_off[i] = code->layer_off[i];
-    for (int i = (int)threadIdx.x; i <= s.core; i += ENC_THREADS) l.term_off[i] = enc->term_off[i];
+    for (int i = (int)threadIdx.x; i <= s.mb; i += T) l.row_off[i] = code->layer_off[i];
+    for (int i = (int)threadIdx.x; i <= s.core; i += T) l.term_off[i] = enc->term_off[i];
     return l;
 }
 
@@ -186,7 +187,7 @@ __global__ void __launch_bounds__(ENC_THREADS) encode_stream(const SynCode *__re
 {
     extern __shared__ __align__(16) uint8_t lds[];
     const Shape s = shape_of(code, enc);
-    const Lds l = carve(lds, s, code, enc);
+    const Lds l = carve<ENC_THREADS>(lds, s, code, enc);
     const int tid = (int)threadIdx.x;
     for (long long b = blockIdx.x; b < count; b += gridDim.x) {
         const uint8_t *src = info + b * s.k;
@@ -239,12 +240,12 @@ __device__ __forceinline__ void put_bits(uint8_t *dst, uint32_t w, int cnt, bool
 // A wave takes whole blocks; lane l holds j = (64c + l) mod Z for chunk c,
 // advanced by 64 mod Z per chunk.  Four chunks' bits are fetched before
 // their ballots.
-template <class Src>
+template <int T, class Src>
 __device__ __forceinline__ void wrap_blocks(uint32_t *dst, int nblk, const Shape &s, int j0, int step, Src src)
 {
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
     const int chunks = (s.len + 1) >> 1;  // 64-bit chunks per block
-    for (int b = wave; b < nblk; b += ENC_WAVES) {
+    for (int b = wave; b < nblk; b += T / 64) {
         uint32_t *d = dst + b * s.len;
         int j = j0;
         for (int c0 = 0; c0 < chunks; c0 += 4) {
@@ -267,14 +268,15 @@ __device__ __forceinline__ void wrap_blocks(uint32_t *dst, int nblk, const Shape
 
 // Any Z.  One workgroup per codeword (grid-stride); the codeword is
 // assembled in LDS bytes and written out at the end.
-__global__ void __launch_bounds__(ENC_THREADS) encode_staged(const SynCode *__restrict__ code,
+template <int T>
+__global__ void __launch_bounds__(T) encode_staged(const SynCode *__restrict__ code,
                                                              const SynEncCode *__restrict__ enc,
                                                              const uint8_t *__restrict__ info,
                                                              uint8_t *__restrict__ cw, long long count, int vec)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     const Shape s = shape_of(code, enc);
-    const Lds l = carve(lds, s, code, enc);
+    const Lds l = carve<T>(lds, s, code, enc);
     uint8_t *bytes = l.bytes;
     const bool words = (s.z & 3) == 0;
     const int tid = (int)threadIdx.x;
@@ -283,40 +285,40 @@ __global__ void __launch_bounds__(ENC_THREADS) encode_staged(const SynCode *__re
         const uint8_t *src = info + b * s.k;
         uint8_t *dst = cw + b * s.n;
         if (vec) {
-            for (int i = tid; i < s.k / 16; i += ENC_THREADS)
+            for (int i = tid; i < s.k / 16; i += T)
                 reinterpret_cast<uint4 *>(bytes)[i] = reinterpret_cast<const uint4 *>(src)[i];
         } else {
-            for (int i = tid; i < s.k; i += ENC_THREADS) bytes[i] = src[i];
+            for (int i = tid; i < s.k; i += T) bytes[i] = src[i];
         }
         __syncthreads();
-        wrap_blocks(l.wr, s.kb, s, j0, step, [&](int blk, int j) { return bytes[blk * s.z + j] & 1u; });
+        wrap_blocks<T>(l.wr, s.kb, s, j0, step, [&](int blk, int j) { return bytes[blk * s.z + j] & 1u; });
         __syncthreads();
-        for (int it = tid; it < s.core * s.nw; it += ENC_THREADS) {
+        for (int it = tid; it < s.core * s.nw; it += T) {
             const int r = it / s.nw, j = it - r * s.nw;
             l.syn[it] = syndrome_word(l, s, r, j);
         }
         __syncthreads();
-        wrap_blocks(l.wsyn, s.core, s, j0, step,
+        wrap_blocks<T>(l.wsyn, s.core, s, j0, step,
                     [&](int blk, int j) { return (l.syn[blk * s.nw + (j >> 5)] >> (j & 31)) & 1u; });
         __syncthreads();
-        for (int it = tid; it < s.core * s.nw; it += ENC_THREADS) {
+        for (int it = tid; it < s.core * s.nw; it += T) {
             const int i = it / s.nw, j = it - i * s.nw;
             put_bits(bytes + (s.kb + i) * s.z + 32 * j, core_word(l, s, i, j), min(32, s.z - 32 * j), words);
         }
         __syncthreads();
-        wrap_blocks(l.wr + s.kb * s.len, s.core, s, j0, step,
+        wrap_blocks<T>(l.wr + s.kb * s.len, s.core, s, j0, step,
                     [&](int blk, int j) { return (uint32_t)bytes[(s.kb + blk) * s.z + j]; });
         __syncthreads();
-        for (int it = tid; it < (s.mb - s.core) * s.nw; it += ENC_THREADS) {
+        for (int it = tid; it < (s.mb - s.core) * s.nw; it += T) {
             const int rr = it / s.nw, j = it - rr * s.nw, r = s.core + rr;
             put_bits(bytes + (s.kb + r) * s.z + 32 * j, ext_word(l, s, r, j), min(32, s.z - 32 * j), words);
         }
         __syncthreads();
         if (vec) {
-            for (int i = tid; i < s.n / 16; i += ENC_THREADS)
+            for (int i = tid; i < s.n / 16; i += T)
                 reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(bytes)[i];
         } else {
-            for (int i = tid; i < s.n; i += ENC_THREADS) dst[i] = bytes[i];
+            for (int i = tid; i < s.n; i += T) dst[i] = bytes[i];
         }
         __syncthreads();
     }
@@ -349,8 +351,14 @@ int launch_encode(const SynCode *d_code, const SynEncCode *d_enc, int z, int k, 
         hipLaunchKernelGGL(encode_stream, dim3(grid), dim3(ENC_THREADS), lds, stream, d_code, d_enc, info, cw, count);
     } else {
         const bool vec = k % 16 == 0 && n % 16 == 0 && ((uintptr_t)info & 15) == 0 && ((uintptr_t)cw & 15) == 0;
-        hipLaunchKernelGGL(encode_staged, dim3(grid), dim3(ENC_THREADS), lds, stream, d_code, d_enc, info, cw, count,
-                           vec ? 1 : 0);
+        // a small codeword gets a one-wave workgroup: its phases are too short
+        // to keep four waves busy between barriers
+        if (n <= ENC_SMALL_N)
+            hipLaunchKernelGGL(encode_staged<64>, dim3(grid), dim3(64), lds, stream, d_code, d_enc, info, cw, count,
+                               vec ? 1 : 0);
+        else
+            hipLaunchKernelGGL(encode_staged<ENC_THREADS>, dim3(grid), dim3(ENC_THREADS), lds, stream, d_code, d_enc,
+                               info, cw, count, vec ? 1 : 0);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
