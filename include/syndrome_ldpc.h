@@ -335,7 +335,8 @@ int32_t syndrome_server_release_many(SyndromeServer *server, const uint32_t *slo
  * holds the payload whatever the outcome: it is the transport block only
  * when crc_ok (the reference leaves its output untouched otherwise).
  * Block slots of a submitted transport block are not delivered by
- * try_recv; try_recv_tb reports the transport block once it is decided and
+ * try_recv, and their hard decisions are not copied to host memory;
+ * try_recv_tb reports the transport block once it is decided and
  * all its blocks are complete, and release_tb frees its slots.  K' = K -
  * n_filler; each block carries P = K' - 24 payload bits (K' when alone);
  * num_cb * P >= tb_size + 24 is required.  SYNDROME_BUSY when 4096

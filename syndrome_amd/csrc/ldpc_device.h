@@ -129,6 +129,7 @@ struct GroupIO {
     const int *nfill;
     const void *const *srcs;
     const int *nfill_pos;
+    uint32_t no_hard = 0;  // bit j: codeword j's hard decisions are not written (its consumer reads LDS)
 
     __device__ __forceinline__ int id(int j) const { return ids ? ids[j] : base + j; }
     __device__ __forceinline__ long long off(int j, int N) const
@@ -309,7 +310,7 @@ __device__ __forceinline__ void store_output(const SynLaunch &a, int N, const Gr
                 h = ((v.x >> 15) & 1u) | (((v.x >> 31) & 1u) << 8) | (((v.y >> 15) & 1u) << 16) |
                     (((v.y >> 31) & 1u) << 24);
             }
-            if (hard) *(uint32_t *)(a.hard + g) = h;
+            if (hard && !((io.no_hard >> cl) & 1u)) *(uint32_t *)(a.hard + g) = h;
         }
         return;
     }
@@ -320,11 +321,11 @@ __device__ __forceinline__ void store_output(const SynLaunch &a, int N, const Gr
         if constexpr (!I8) {
             const float v = ((const float *)lds)[i];
             if (post) ((float *)a.post)[g] = v;
-            if (hard) a.hard[g] = v < 0.f ? 1 : 0;
+            if (hard && !((io.no_hard >> cl) & 1u)) a.hard[g] = v < 0.f ? 1 : 0;
         } else {
             const int16_t v = ((const int16_t *)lds)[i];
             if (post) ((int16_t *)a.post)[g] = v;
-            if (hard) a.hard[g] = v < 0 ? 1 : 0;
+            if (hard && !((io.no_hard >> cl) & 1u)) a.hard[g] = v < 0 ? 1 : 0;
         }
     }
 }

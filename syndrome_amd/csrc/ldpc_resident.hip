@@ -181,7 +181,13 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         SynLaunch la = a;
         if (s_aligned) la.flags |= SYN_F_ALIGNED;
         const SynCode *code = codes + s_code;
-        const GroupIO io{s_count, 0, s_slots, offs, nullptr, s_src, s_nf};
+        GroupIO io{s_count, 0, s_slots, offs, nullptr, s_src, s_nf};
+        // transport-block code blocks are folded from LDS (tb_fold) and
+        // delivered per transport block: their hard decisions need not cross
+        // to host memory
+        if (s_any_tb)
+            for (int j = 0; j < s_count; ++j)
+                if (s_tb[j] >= 0) io.no_hard |= 1u << j;
         if (code->kind == SYN_KIND_BG1) {
             if (code->z == 384) fast_group<SYN_KIND_BG1, 384, false, true>(code, 384, s_count, io, la);
             else fast_group<SYN_KIND_BG1, 0, false, true>(code, code->z, s_count, io, la);
