@@ -186,10 +186,38 @@ struct Lane {
     int z;
 };
 
+// Which lanes wrap on each edge of a compile-time (graph, Z): bit l of
+// m[w][E] is set when lane 64w + l, whose lifting index is
+// (64w + l) mod ZC, has z + shift(E) >= ZC.  A wave reads its own row with
+// scalar loads, so choosing a lane's LDS base costs one select on a lane
+// mask already in SGPRs instead of a per-lane compare and a select.
+constexpr int WRAP_WAVES = 8;  // 512-lane workgroups
+
 template <class G, int ZC>
+struct WrapMasks {
+    unsigned long long m[WRAP_WAVES][G::EDGES];
+};
+
+template <class G, int ZC>
+constexpr WrapMasks<G, ZC> make_wrap_masks()
+{
+    WrapMasks<G, ZC> w{};
+    for (int e = 0; e < G::EDGES; ++e) {
+        const int T = ZC - G::v(e, lifting_set(ZC)) % ZC;  // lanes z >= T wrap
+        for (int t = 0; t < 64 * WRAP_WAVES; ++t)
+            if (t % ZC >= T) w.m[t / 64][e] |= 1ull << (t % 64);
+    }
+    return w;
+}
+
+template <class G, int ZC>
+__constant__ const WrapMasks<G, ZC> kWrap = make_wrap_masks<G, ZC>();
+
+template <class G, int ZC, bool WM = true>
 struct Geo {
     int Z;                         // run-time lifting size (ZC == 0)
     const int *__restrict__ sh;    // run-time shifts (ZC == 0)
+    const unsigned long long *__restrict__ wm;  // this wave's row of kWrap (ZC > 0 and WM)
 
     __device__ __forceinline__ int z_size() const
     {
@@ -208,10 +236,13 @@ struct Geo {
             constexpr int A = (col * ZC + s) * ESZ;
             if constexpr (s == 0) {
                 return ((A >= 65536) ? ln.pbh : ln.pb) + (A & 0xffff);
-            } else if constexpr (A >= 65536) {
-                return ((ln.z >= T) ? ln.pbwh : ln.pbh) + (A & 0xffff);
             } else {
-                return ((ln.z >= T) ? ln.pbw : ln.pb) + A;
+                static_assert(T > 0);
+                bool wr;
+                if constexpr (WM) wr = __builtin_amdgcn_inverse_ballot_w64(wm[E]);
+                else wr = ln.z >= T;
+                if constexpr (A >= 65536) return (wr ? ln.pbwh : ln.pbh) + (A & 0xffff);
+                else return (wr ? ln.pbw : ln.pb) + A;
             }
         } else {
             const int s = sh[E];
@@ -243,8 +274,8 @@ __device__ __forceinline__ int r_i8(int g1, int g2, uint32_t s, uint32_t am, int
 // One layer of the f32 decode (src/qc_ldpc.rs:1072-1204).  KEEPQ keeps q's
 // bits in registers from pass 1 to pass 2, which saves one operation per
 // edge in pass 2 for D more live registers.
-template <class G, int ZC, int L, bool KEEPQ>
-__device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo, StateF32<G> &st, float beta,
+template <class G, int ZC, int L, bool KEEPQ, bool WM>
+__device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC, WM> &geo, StateF32<G> &st, float beta,
                                           bool act)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
@@ -320,8 +351,8 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC> &geo,
 // One layer of the int8 decode (src/qc_ldpc.rs:1619-1745).  KEEPA keeps
 // |q| in registers for pass 2, which then finds the argmin instead of
 // pass 1 (one operation less per edge, D more live registers).
-template <class G, int ZC, int L, bool KEEPA>
-__device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC> &geo, StateI8<G> &st, int bq, int C,
+template <class G, int ZC, int L, bool KEEPA, bool WM>
+__device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &geo, StateI8<G> &st, int bq, int C,
                                          bool act)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
@@ -378,8 +409,8 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC> &geo, 
     }
 }
 
-template <class G, int ZC, int L, class T>
-__device__ __forceinline__ bool row_parity(const Lane &ln, const Geo<G, ZC> &geo)
+template <class G, int ZC, int L, class T, bool WM>
+__device__ __forceinline__ bool row_parity(const Lane &ln, const Geo<G, ZC, WM> &geo)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
     T v[D];  // every load of the row issued before the first compare
@@ -392,8 +423,8 @@ __device__ __forceinline__ bool row_parity(const Lane &ln, const Geo<G, ZC> &geo
     return p;
 }
 
-template <class G, int ZC, int L0, int L1, class T>
-__device__ __forceinline__ bool rows_parity(const Lane &ln, const Geo<G, ZC> &geo)
+template <class G, int ZC, int L0, int L1, class T, bool WM>
+__device__ __forceinline__ bool rows_parity(const Lane &ln, const Geo<G, ZC, WM> &geo)
 {
     bool p = false;
     // every row of the segment, without a branch between rows
@@ -409,8 +440,8 @@ __device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballo
 // skips the remaining segments once none of its lanes is still checking
 // (wave-uniform branches, so no exec-mask bookkeeping).  Returns whether
 // this lane found a failed check.
-template <class G, int ZC, class T>
-__device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC> &geo, bool going, volatile int *cw_fail)
+template <class G, int ZC, class T, bool WM>
+__device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC, WM> &geo, bool going, volatile int *cw_fail)
 {
     constexpr int R = G::ROWS;
     constexpr int S0 = R < 1 ? R : 1, S1 = R < 4 ? R : 4, S2 = R < 10 ? R : 10, S3 = R < 20 ? R : 20,
@@ -437,7 +468,7 @@ __device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC> &geo
 // and over the lanes, `io` saying where the live ones come from and go to.
 // I8 selects the int8 path.  Waves without a live lane keep only the
 // barriers.
-template <int KIND, int ZC, bool I8, bool KEEPQ = true>
+template <int KIND, int ZC, bool I8, bool KEEPQ = true, bool WM = true>
 __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int Z, int cws, const GroupIO &io,
                                            const SynLaunch &a)
 {
@@ -461,6 +492,8 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
     const int cl = lane_ok ? t / Z : 0;
     const int z = lane_ok ? t - cl * Z : 0;
     const int zb0 = pad + (cl * N + z) * ESZ;
+    int wave = 0;  // this lane's wave, for its row of kWrap
+    if constexpr (ZC > 0 && WM) wave = __builtin_amdgcn_readfirstlane(t) >> 6;
 
     State st;  // R = 0 everywhere (qc_ldpc.rs:1006-1009 / 1579-1581)
     if constexpr (I8) {
@@ -485,7 +518,9 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
         // re-establish that the bases are small and non-negative.
         unsigned char *pb = smem + (opaque_v(zb0) & 0x3ffff);
         const Lane ln{pb, pb - Z * ESZ, pb + 65536, pb - Z * ESZ + 65536, opaque_v(z) & 0x3ff};
-        const Geo<G, ZC> geo{Z, shifts + opaque_s(0)};
+        const unsigned long long *wm = nullptr;
+        if constexpr (ZC > 0 && WM) wm = kWrap<G, ZC>.m[opaque_s(wave)];
+        const Geo<G, ZC, WM> geo{Z, shifts + opaque_s(0), wm};
         SFor<0, G::ROWS>::run([&](auto lc) {
             constexpr int L = decltype(lc)::value;
             if (wave_live) {

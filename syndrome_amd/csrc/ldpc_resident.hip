@@ -189,10 +189,10 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
             for (int j = 0; j < s_count; ++j)
                 if (s_tb[j] >= 0) io.no_hard |= 1u << j;
         if (code->kind == SYN_KIND_BG1) {
-            if (code->z == 384) fast_group<SYN_KIND_BG1, 384, false, true>(code, 384, s_count, io, la);
+            if (code->z == 384) fast_group<SYN_KIND_BG1, 384, false, true, false>(code, 384, s_count, io, la);
             else fast_group<SYN_KIND_BG1, 0, false, true>(code, code->z, s_count, io, la);
         } else {
-            if (code->z == 208) fast_group<SYN_KIND_BG2, 208, false, true>(code, 208, s_count, io, la);
+            if (code->z == 208) fast_group<SYN_KIND_BG2, 208, false, true, false>(code, 208, s_count, io, la);
             else fast_group<SYN_KIND_BG2, 0, false, true>(code, code->z, s_count, io, la);
         }
         if (s_any_tb) {
