@@ -352,8 +352,8 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC, WM> &
 // |q| in registers for pass 2, which then finds the argmin instead of
 // pass 1 (one operation less per edge, D more live registers).
 template <class G, int ZC, int L, bool KEEPA, bool WM>
-__device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &geo, StateI8<G> &st, int bq, int C,
-                                         bool act)
+__device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &geo, StateI8<G> &st, int bq,
+                                         int lo, int hi, bool act)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
     static_assert(D >= 2 && D <= 27, "row degree: its signs and argmin share one 32-bit word");
@@ -402,7 +402,10 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
                 rn = r_i8<D>(g1, g2, ns, am, e);
             }
             const int nv = Av[e] + (rn - ro[e]);  // saturating add + clamp == clamp
-            *adr[e] = (int16_t)clampi(nv, -C, C);
+            // clamp to [lo, hi] = [-C, C] (C >= 0) as one v_med3 on bounds
+            // held in VGPRs: the compiler's max/min pair reads them from
+            // SGPRs, and one VALU instruction cannot read two SGPRs
+            *adr[e] = (int16_t)med3i(nv, lo, hi);
         });
         mg_put<G, L>(st, (uint32_t)g1 | ((uint32_t)g2 << 8));
         sg_put<G, L>(st.sg, ns | (am << D));
@@ -521,10 +524,11 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
         const unsigned long long *wm = nullptr;
         if constexpr (ZC > 0 && WM) wm = kWrap<G, ZC>.m[opaque_s(wave)];
         const Geo<G, ZC, WM> geo{Z, shifts + opaque_s(0), wm};
+        const int clo = opaque_v(-C), chi = opaque_v(C);  // the int8 posterior clamp's bounds, in VGPRs
         SFor<0, G::ROWS>::run([&](auto lc) {
             constexpr int L = decltype(lc)::value;
             if (wave_live) {
-                if constexpr (I8) layer_i8<G, ZC, L, KEEPQ && KIND != SYN_KIND_BG1>(ln, geo, st, bq, C, act);
+                if constexpr (I8) layer_i8<G, ZC, L, KEEPQ && KIND != SYN_KIND_BG1>(ln, geo, st, bq, clo, chi, act);
                 else layer_f32<G, ZC, L, KEEPQ>(ln, geo, st, beta, act);
             }
             if constexpr (barrier_after<G>(L)) __syncthreads();
