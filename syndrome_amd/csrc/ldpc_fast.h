@@ -373,7 +373,12 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
     SFor<0, D>::run([&](auto ec) {
         constexpr int e = decltype(ec)::value;
         ro[e] = r_i8<D>(o1, o2, os, oam, e);
-        const int q = clampi(Av[e] - ro[e], -127, 127);  // Q-build, clamped to the message range
+        // Q-build.  The reference clamps q to [-127, 127]; the clamp changes
+        // no output here, so it is left out: the sign is the same, and with
+        // m1 = m2 = 127 at the start an |q| above 127 moves neither min1 nor
+        // min2.  It could only change "a == m1" / "a <= m1" when min1 ends
+        // at 127, and then min2 == min1, so the argmin does not matter.
+        const int q = Av[e] - ro[e];
         qs = __builtin_amdgcn_alignbit(qs, (uint32_t)q, 31);
         const int a = q < 0 ? -q : q;
         if constexpr (KEEPA) av[e] = a;

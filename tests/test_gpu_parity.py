@@ -203,6 +203,48 @@ def test_i8_batch_matches_oracle(dev, oracle, bg, z, plain):
             assert np.array_equal(hard[b], rh), f"clamp {clamp} codeword {b}: hard differs"
 
 
+@pytest.mark.parametrize("plain", [False, True], ids=["fast", "plain"])
+@pytest.mark.parametrize("bg,z", [(0, 384), (1, 208), (0, 20)])
+def test_i8_saturated_rows_match_oracle(dev, oracle, bg, z, plain):
+    """Inputs far beyond the message range: every |q| of most rows exceeds
+    127, so min1 = min2 = 127 there and the clamp of q (qc_ldpc.rs Q-build)
+    is what the kernels' unclamped form must reproduce; a few flipped signs
+    keep the decode going for several iterations."""
+    import syndrome_amd._lib as L
+    from syndrome_amd import QcLdpcDecoder, QuantParams
+    say(f"[i8 saturated] BG{bg + 1} Z={z} plain={plain}")
+    code = oracle.code_3gpp(bg, z)
+    dec = QcLdpcDecoder.with_lifting_size(bg, z, 0.5)
+    B = 4
+    cws = codewords(oracle, code, B, seed=77 * z + bg)
+    rng = np.random.default_rng(5 * z + bg)
+    mag = rng.integers(160, 480, size=cws.shape)  # int16 app, beyond the message range
+    app_in = np.where(cws == 0, mag, -mag).astype(np.int16)
+    for b in range(B):
+        flip = rng.choice(code.n, size=(b + 1) * 3, replace=False)
+        app_in[b, flip] = -(app_in[b, flip] // (2 if b % 2 else 8))  # strong or weak wrong signs
+    iters = 6
+    for clamp in (32767, 400):
+        d_in = torch.from_numpy(app_in).cuda()
+        d_post = torch.empty((B, code.n), dtype=torch.int16, device="cuda")
+        d_hard = torch.empty((B, code.n), dtype=torch.uint8, device="cuda")
+        d_edge = torch.empty((B, dec.required_edge_buffer()), dtype=torch.int8, device="cuda")
+        d_it = torch.empty(B, dtype=torch.int32, device="cuda")
+        flags = L.WRITE_POST | L.WRITE_HARD | L.WRITE_EDGE | (L.PLAIN if plain else 0)
+        dec.decode_batch_i8(d_in, d_post, d_hard, d_it, iters, flags=flags, d_edge=d_edge,
+                            quant=QuantParams(8.0, clamp))
+        torch.cuda.synchronize()
+        post, hard, edge, it = (d_post.cpu().numpy(), d_hard.cpu().numpy(), d_edge.cpu().numpy(),
+                                d_it.cpu().numpy())
+        for b in range(B):
+            app = app_in[b]
+            ri, ra, re, rh = oracle.decode_i8(code, 0.5, app, iters, 8.0, clamp)
+            assert it[b] == ri, f"clamp {clamp} codeword {b}: iterations {it[b]} != {ri}"
+            assert np.array_equal(post[b], ra), f"clamp {clamp} codeword {b}: app differs"
+            assert np.array_equal(edge[b], re), f"clamp {clamp} codeword {b}: edge_r differs"
+            assert np.array_equal(hard[b], rh), f"clamp {clamp} codeword {b}: hard differs"
+
+
 def test_capi_decode_5g_host_buffers(dev, oracle):
     """The reference-shaped entry point (capi/src/lib.rs:290-343) on host buffers."""
     from syndrome_amd import QcLdpcDecoder
