@@ -98,16 +98,16 @@ __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ 
             }
             return v;
         };
-        // Without HARQ combining, with Qm = 1 and E <= m, a position has at
-        // most one selection, k = its rank r (received index q = k), and only
-        // when r < E: the general element below reduces to 0 + el[r] (the
-        // add keeps the reference's "+=" into zeros, which turns -0 into +0).
-        const bool simple = !combine && qm == 1 && m > 0 && e <= m;
+        // Without HARQ combining and with E <= m, a position has at most one
+        // selection, k = its rank r, and only when r < E: the general element
+        // below reduces to 0 + el[q(r)] (the add keeps the reference's "+="
+        // into zeros, which turns -0 into +0).
+        const bool simple = !combine && qm > 0 && m > 0 && e <= m;
         auto elem1 = [&](int p) {
             const int c = p - 2 * z;
             const bool sel = c >= 0 && !(c >= fs && c < fe);
             const int r = c >= k0 ? below(c) - bk0 : (bncb - bk0) + below(c);
-            return (sel && r < e) ? 0.f + el[r] : 0.f;
+            return (sel && r < e) ? 0.f + el[q_of(r)] : 0.f;
         };
         // 4 consecutive elements per lane, one 16-byte store when aligned
         const bool vec = (out_off & 3) == 0;

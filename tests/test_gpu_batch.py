@@ -141,22 +141,24 @@ def test_device_dematch_matches_oracle(dev_ok, oracle):
         assert np.array_equal(f32_bits(out[offs[i]:offs[i] + n[i]]), f32_bits(ref)), f"block {i}"
 
 
-def test_device_dematch_qm1_no_repetition_matches_oracle(dev_ok, oracle):
-    """The de-match kernel's direct path (Qm 1, E <= m, no combining): every
-    rv, fillers, signed zeros in the received LLRs (the reference's += turns
-    -0 into +0), and blocks at unaligned output offsets, which take the
-    general path beside it."""
+def test_device_dematch_no_repetition_matches_oracle(dev_ok, oracle):
+    """The de-match kernel's direct path (E <= m, no combining): Qm 1, 2, 4,
+    6 and 8, every rv, fillers, signed zeros in the received LLRs (the
+    reference's += turns -0 into +0), and blocks at unaligned output offsets,
+    which take the general path beside it."""
     from syndrome_amd import DecodePlan
-    say("[dematch] 24 code blocks, Qm 1, E <= m, rv 0..3, -0.0 inputs")
+    say("[dematch] 24 code blocks, Qm 1..8, E <= m, rv 0..3, -0.0 inputs")
     rng = np.random.default_rng(11)
-    bg, z, nf, e_len, rv = [], [], [], [], []
+    bg, z, nf, e_len, rv, qm = [], [], [], [], [], []
     for i in range(24):
         b, zz = MIX[i % len(MIX)]
         k = (22 if b == 0 else 10) * zz
         f = int(rng.integers(0, min(40, k // 4) + 1)) if i % 3 else 0
         ncb = (66 if b == 0 else 50) * zz
         m = ncb - f  # non-filler positions of the circular buffer
-        e = int(rng.integers(max(1, ncb // 4), m + 1)) if i % 5 else m
+        q = [1, 2, 4, 6, 8, 1][i % 6]
+        e = (int(rng.integers(max(q, ncb // 4), m + 1)) if i % 5 else m) // q * q
+        qm.append(q)
         bg.append(b)
         z.append(zz)
         nf.append(f)
@@ -169,14 +171,14 @@ def test_device_dematch_qm1_no_repetition_matches_oracle(dev_ok, oracle):
     e_llr = rng.normal(0, 3, int(sum(e_len))).astype(np.float32)
     e_llr[rng.choice(e_llr.size, e_llr.size // 50, replace=False)] = np.float32(-0.0)
     plan = DecodePlan(bg, z, nf, offs, 0.5)
-    plan.set_dematch(e_offs, e_len, [1] * 24, rv)
+    plan.set_dematch(e_offs, e_len, qm, rv)
     d_e = torch.from_numpy(e_llr).cuda()
     d_out = torch.full((int(offs[-1] + n[-1]),), 7.0, dtype=torch.float32, device="cuda")
     plan.dematch(d_e, d_out)
     torch.cuda.synchronize()
     out = d_out.cpu().numpy()
     for i in range(24):
-        cb = oracle.rate_dematch(e_llr[e_offs[i]:e_offs[i] + e_len[i]], rv[i], 1, bg[i], z[i], nf[i], cached=True)
+        cb = oracle.rate_dematch(e_llr[e_offs[i]:e_offs[i] + e_len[i]], rv[i], qm[i], bg[i], z[i], nf[i], cached=True)
         ref = oracle.decoder_input_from_cb(cb, z[i])
         assert np.array_equal(f32_bits(out[offs[i]:offs[i] + n[i]]), f32_bits(ref)), f"block {i}"
 
