@@ -102,15 +102,37 @@ __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ 
         // selection, k = its rank r, and only when r < E: the general element
         // below reduces to 0 + el[q(r)] (the add keeps the reference's "+="
         // into zeros, which turns -0 into +0).
-        const bool simple = !combine && qm > 0 && m > 0 && e <= m;
+        // The load is unconditional (index 0 when nothing is selected; E is a
+        // multiple of Qm, so q(r) < E for r < E) so that a lane's loads go
+        // out together instead of one branch and wait each.
+        const bool simple = !combine && qm > 0 && m > 0 && e > 0 && e <= m;
         auto elem1 = [&](int p) {
             const int c = p - 2 * z;
-            const bool sel = c >= 0 && !(c >= fs && c < fe);
             const int r = c >= k0 ? below(c) - bk0 : (bncb - bk0) + below(c);
-            return (sel && r < e) ? 0.f + el[q_of(r)] : 0.f;
+            const bool take = c >= 0 && !(c >= fs && c < fe) && r < e;
+            const float x = el[take ? q_of(r) : 0];
+            return take ? 0.f + x : 0.f;
         };
         // 4 consecutive elements per lane, one 16-byte store when aligned
         const bool vec = (out_off & 3) == 0;
+        if (simple && vec && p_end - ch.p0 == SYN_DEMATCH_CHUNK) {
+            // a full chunk: every load of the lane's four groups is issued
+            // before the first store
+            constexpr int G = SYN_DEMATCH_CHUNK / (4 * DM_THREADS);
+            float v[G][4];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const int p = ch.p0 + 4 * ((int)threadIdx.x + j * DM_THREADS);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[j][u] = elem1(p + u);
+            }
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const int p = ch.p0 + 4 * ((int)threadIdx.x + j * DM_THREADS);
+                *reinterpret_cast<float4 *>(out + out_off + p) = make_float4(v[j][0], v[j][1], v[j][2], v[j][3]);
+            }
+            continue;
+        }
         for (int p = ch.p0 + 4 * (int)threadIdx.x; p < p_end; p += 4 * DM_THREADS) {
             if (simple && vec && p + 3 < p_end) {
                 const float v0 = elem1(p), v1 = elem1(p + 1), v2 = elem1(p + 2), v3 = elem1(p + 3);
