@@ -72,6 +72,39 @@ def test_special_float_llrs_decode_like_the_oracle(oracle, plain):
             assert its8[b] == ri and np.array_equal(post8[b], ra) and np.array_equal(hard8[b], rh)
 
 
+@pytest.mark.parametrize("plain", [False, True], ids=["fast", "plain"])
+def test_special_offsets_decode_like_the_oracle(oracle, plain):
+    """The offset floor mag = max(me - beta, 0) at zero, negative, huge,
+    infinite and NaN beta.  For a NaN beta the oracle follows the scalar
+    reference (qc_ldpc.rs:1195-1196, where "mag < 0" keeps the NaN); the
+    AVX2 path's max(0.0) (simd_avx2.rs:276) would give 0 instead."""
+    import syndrome_amd
+    from syndrome_amd import QcLdpcDecoder, _lib as L
+    syndrome_amd.lib()
+    bg, z = 0, 384
+    code = oracle.code_3gpp(bg, z)
+    cws = codewords(oracle, code, 3, seed=41)
+    llr = channel(oracle, code, cws, [1.0, 2.0, 3.0], seed=43)
+    for beta in (0.0, -0.5, 1e30, float("inf"), float("nan")):
+        say(f"[offset] BG1 Z=384 beta={beta} plain={plain}")
+        dec = QcLdpcDecoder.with_lifting_size(bg, z, beta)
+        B = llr.shape[0]
+        d_in = torch.from_numpy(llr).cuda()
+        d_post = torch.empty_like(d_in)
+        d_hard = torch.empty(llr.shape, dtype=torch.uint8, device="cuda")
+        d_edge = torch.empty((B, dec.required_edge_buffer()), dtype=torch.float32, device="cuda")
+        d_it = torch.empty(B, dtype=torch.int32, device="cuda")
+        flags = L.WRITE_POST | L.WRITE_HARD | L.WRITE_EDGE | (L.PLAIN if plain else 0)
+        dec.decode_batch_f32(d_in, d_post, d_hard, d_it, 5, flags=flags, d_edge=d_edge)
+        torch.cuda.synchronize()
+        post, hard, edge, its = (d_post.cpu().numpy(), d_hard.cpu().numpy(), d_edge.cpu().numpy(),
+                                 d_it.cpu().numpy())
+        for b in range(B):
+            ri, rp, re, rh = oracle.decode_f32(code, beta, llr[b], 5)
+            assert its[b] == ri and np.array_equal(hard[b], rh), f"beta {beta} codeword {b}"
+            assert _same_f32(post[b], rp) and _same_f32(edge[b], re), f"beta {beta} codeword {b}: values"
+
+
 def test_workspace_path_equals_raw_path(oracle):
     from syndrome_amd import QcLdpcDecoder
     say("[workspace] decode_with_workspace == decode_layered_offset_min_sum (BG1 Z=96)")
