@@ -276,7 +276,7 @@ __device__ __forceinline__ int r_i8(int g1, int g2, uint32_t s, uint32_t am, int
 // edge in pass 2 for D more live registers.
 template <class G, int ZC, int L, bool KEEPQ, bool WM>
 __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC, WM> &geo, StateF32<G> &st, float beta,
-                                          int glo, bool act)
+                                          int glo, int g0, bool act)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
     static_assert(D >= 2 && D <= 27, "row degree: its signs and argmin share one 32-bit word");
@@ -311,16 +311,19 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC, WM> &
         else m2 = clampu(a, m1, m2);
         m1 = a < m1 ? a : m1;
     });
-    float f1 = __uint_as_float(m1), f2 = __uint_as_float(m2);
-    if (m1 == 0x7f7fffffu) f1 = 0.f;
-    if (m2 == 0x7f7fffffu) f2 = f1;
     // mag = me - beta, then (mag < 0) ? 0 : mag, as one signed-integer max
     // of mag's bits against glo: negative floats have negative bit patterns
     // and mag is never -0 (me >= +0), so max(bits, 0) is that select; with
     // a NaN beta, glo = INT_MIN and the NaN passes whatever its sign, as it
-    // does the compare.
-    const float g1 = __int_as_float(max(__float_as_int(f1 - beta), glo));
-    const float g2 = __int_as_float(max(__float_as_int(f2 - beta), glo));
+    // does the compare.  The sentinel rule (min1 = FLT_MAX -> 0, then
+    // min2 = FLT_MAX -> min1) is applied last, to the finished magnitudes:
+    // g0 = the magnitude of a min of 0, so the rule's compares run beside
+    // the subtract and the max instead of before them.
+    const int g1s = max(__float_as_int(__uint_as_float(m1) - beta), glo);
+    const int g2s = max(__float_as_int(__uint_as_float(m2) - beta), glo);
+    const int g1i = m1 == 0x7f7fffffu ? g0 : g1s;
+    const float g1 = __int_as_float(g1i);
+    const float g2 = __int_as_float(m2 == 0x7f7fffffu ? g1i : g2s);
     // sign of R_e = XOR of all q signs but e's own
     const uint32_t par = __popc(qs) & 1u;
     const uint32_t ns = qs ^ (par ? ((1u << D) - 1u) : 0u);
@@ -520,6 +523,7 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
 
     const float beta = a.beta;
     const int glo = beta != beta ? INT_MIN : 0;  // the floor of |R|'s bits (none for a NaN beta)
+    const int g0 = max(__float_as_int(0.f - beta), glo);  // |R| where the min is the FLT_MAX sentinel
     const int bq = a.beta_q, C = a.app_clamp;
     for (int it = 0; it < a.iterations; ++it) {
         if (all_done(fl, cws)) break;
@@ -538,7 +542,7 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
             constexpr int L = decltype(lc)::value;
             if (wave_live) {
                 if constexpr (I8) layer_i8<G, ZC, L, KEEPQ && KIND != SYN_KIND_BG1>(ln, geo, st, bq, clo, chi, act);
-                else layer_f32<G, ZC, L, KEEPQ>(ln, geo, st, beta, glo, act);
+                else layer_f32<G, ZC, L, KEEPQ>(ln, geo, st, beta, glo, g0, act);
             }
             if constexpr (barrier_after<G>(L)) __syncthreads();
         });
