@@ -128,6 +128,10 @@ struct SynResidentTb {
     unsigned *ok;           // per ring index: 1 if every check passed (host memory)
 };
 
+// Bytes of the resident decoder's device-side queue block (two 128-byte
+// cache lines).
+#define SYN_RESIDENT_QUEUE_BYTES 256
+
 struct SynResidentCtl {
     unsigned int tail;  // items published; written by the host after the item
     unsigned int stop;  // non-zero: every workgroup leaves at its next wait

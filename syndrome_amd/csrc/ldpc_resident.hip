@@ -27,6 +27,14 @@
 
 namespace syn {
 
+// A word of a record in host memory, read at system scope so that no GPU
+// cache can return an older copy of it.
+template <class T>
+__device__ __forceinline__ T host_load(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // One decoded code block's share of its transport block's checks, from its
 // posterior still in LDS (hard bit i = posterior[i] < 0), by the whole
 // workgroup:
@@ -49,11 +57,12 @@ __device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, c
     __shared__ unsigned char *s_out;
     if (threadIdx.x == 0) {
         const SynTbWork *tw = tbs.work + ti;
-        s_id = tw->id;
-        s_c = tw->num_cb;
-        s_p = tw->p;
-        s_a = tw->tb_size;
-        s_out = tw->out;
+        // host-written record: system-scope loads, past every cache
+        s_id = host_load(&tw->id);
+        s_c = host_load(&tw->num_cb);
+        s_p = host_load(&tw->p);
+        s_a = host_load(&tw->tb_size);
+        s_out = host_load(&tw->out);
     }
     __syncthreads();
     const int c = s_c, p = s_p, a = s_a;
@@ -92,13 +101,19 @@ __device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, c
     __syncthreads();
 }
 
-// Device-side queue words (one 64-byte block in device memory): the claim
-// counter, the device's copy of the host's tail and stop words, and the
-// lock that lets one waiting workgroup at a time refresh that copy from
-// host memory (so idle workgroups spin on device memory, not on PCIe).
+// Device-side queue words (SYN_RESIDENT_QUEUE_BYTES of device memory): the
+// claim counter, alone on its cache line, then the device's copy of the
+// host's tail and stop words and the lock that lets one waiting workgroup
+// at a time refresh that copy from host memory (so idle workgroups spin on
+// device memory, not on PCIe).  Waiters poll the second line; a claim's
+// compare-and-swap on the first does not queue behind their traffic.
 struct ResidentQueue {
-    unsigned head, tail, stop, poll_lock;
+    unsigned head;
+    unsigned pad0[31];
+    unsigned tail, stop, poll_lock;
+    unsigned pad1[29];
 };
+static_assert(sizeof(ResidentQueue) == SYN_RESIDENT_QUEUE_BYTES, "queue block size");
 
 __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict__ codes,
                                                         const SynWork *__restrict__ ring, unsigned cap,
@@ -117,6 +132,13 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         if (threadIdx.x == 0) {
             unsigned w = 0;
             int have = 0;
+            int backoff = 1;  // sleeps of 512 cycles between idle polls, doubling to 16
+            // Idle workgroups back off exponentially between polls: each
+            // poll's acquire invalidates this XCD's L2 (each XCD has its own),
+            // and idle workgroups polling without pause kept evicting the
+            // decoding workgroups' data, which slowed a lone BG1 Z=384 block
+            // from 43 to about 100 us.  The item's ring entry is read with
+            // system-scope loads (host_load), which no cache serves.
             for (;;) {
                 const unsigned h = __hip_atomic_load(&q->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 unsigned t = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -137,6 +159,10 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
                     unsigned expect = h;
                     if (__hip_atomic_compare_exchange_strong(&q->head, &expect, h + 1, __ATOMIC_RELAXED,
                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        // the item's inputs (HBM or host memory) may have been
+                        // rewritten since this XCD's L2 last held them: one
+                        // agent-scope acquire invalidates those lines
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                         w = h;
                         have = 1;
                         __hip_atomic_store(last_claim, (long long)wall_clock64(), __ATOMIC_RELAXED,
@@ -152,15 +178,16 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
                 if (__hip_atomic_load(&q->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || since > idle_ticks ||
                     now - t0 > life_ticks)
                     break;
-                __builtin_amdgcn_s_sleep(8);
+                for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(8);
+                backoff = backoff < 16 ? 2 * backoff : 16;
             }
             s_w = w;
             s_have = have;
             if (have) {
                 const SynWork *wk = ring + w % cap;
-                s_code = wk->code;
-                s_count = wk->count;
-                s_aligned = wk->aligned;
+                s_code = host_load(&wk->code);
+                s_count = host_load(&wk->count);
+                s_aligned = host_load(&wk->aligned);
             }
         }
         __syncthreads();
@@ -170,11 +197,11 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         __syncthreads();
         if ((int)threadIdx.x < s_count) {
             const SynWork *wk = ring + w % cap;
-            s_slots[threadIdx.x] = wk->slot[threadIdx.x];
-            s_nf[threadIdx.x] = wk->nfill[threadIdx.x];
-            s_src[threadIdx.x] = wk->src[threadIdx.x];
-            s_tb[threadIdx.x] = wk->tb[threadIdx.x];
-            s_tbci[threadIdx.x] = wk->tb_ci[threadIdx.x];
+            s_slots[threadIdx.x] = host_load(&wk->slot[threadIdx.x]);
+            s_nf[threadIdx.x] = host_load(&wk->nfill[threadIdx.x]);
+            s_src[threadIdx.x] = host_load(&wk->src[threadIdx.x]);
+            s_tb[threadIdx.x] = host_load(&wk->tb[threadIdx.x]);
+            s_tbci[threadIdx.x] = host_load(&wk->tb_ci[threadIdx.x]);
             if (s_tb[threadIdx.x] >= 0) s_any_tb = 1;
         }
         __syncthreads();

@@ -59,6 +59,26 @@ constexpr int VALID_Z[51] = {2,  3,  4,  5,  6,  7,  8,  9,  10,  11,  12,  13, 
                              22, 24, 26, 28, 30, 32, 36, 40, 44,  48,  52,  56,  60,  64,  72,  80,  88,
                              96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384};
 
+// The server lock.  Its critical sections last well under a microsecond,
+// while a thread that blocks in the kernel on a contended std::mutex takes
+// tens of microseconds to be woken: a waiter retries for a while before it
+// blocks.  That wake-up was most of a code block's host-side latency.
+class ServerMutex {
+  public:
+    void lock()
+    {
+        for (int i = 0; i < SPINS; ++i)
+            if (m_.try_lock()) return;
+        m_.lock();
+    }
+    bool try_lock() { return m_.try_lock(); }
+    void unlock() { m_.unlock(); }
+
+  private:
+    static constexpr int SPINS = 4000;
+    std::mutex m_;
+};
+
 int z_index(int z)
 {
     for (int i = 0; i < 51; ++i)
@@ -159,7 +179,12 @@ struct SyndromeServer {
     std::vector<int> free_list;
     std::deque<int> submitted, done;
     std::vector<Lane> lanes;
-    std::mutex mu;
+    ServerMutex mu;
+    // Sizes of submitted, done, rs.tb_done_q and rs.tb_outstanding, stored
+    // (sync_counts, under mu) after every change: a thread with nothing to
+    // take reads them instead of taking the lock.  A count may be stale;
+    // a non-zero one only sends the reader to the lock, where it checks.
+    std::atomic<int> n_submitted{0}, n_done{0}, n_tb_done{0}, n_tb_out{0};
     std::atomic<bool> stop{false};
     std::thread worker;
     std::atomic<long long> batches{0}, decoded{0};
@@ -208,6 +233,13 @@ struct SyndromeServer {
         std::deque<int> tb_done_q;
     } rs;
 
+    void sync_counts()
+    {
+        n_submitted.store((int)submitted.size(), std::memory_order_release);
+        n_done.store((int)done.size(), std::memory_order_release);
+        n_tb_done.store((int)rs.tb_done_q.size(), std::memory_order_release);
+        n_tb_out.store((int)rs.tb_outstanding.size(), std::memory_order_release);
+    }
     int32_t launch_batch(Lane &ln, std::vector<int> &batch);
     int32_t init_resident();
     int32_t launch_resident();
@@ -362,7 +394,7 @@ void SyndromeServer::run()
         for (Lane &ln : lanes) {
             if (ln.busy && hipEventQuery(ln.done) == hipSuccess) {
                 const double t = now_ns();
-                std::lock_guard<std::mutex> lk(mu);
+                std::lock_guard<ServerMutex> lk(mu);
                 for (size_t k = 0; k < ln.slots.size(); ++k) {
                     const int s = ln.slots[k];
                     h_it[s] = ln.h_iters[k];
@@ -371,6 +403,7 @@ void SyndromeServer::run()
                     done.push_back(s);
                 }
                 decoded += (long long)ln.slots.size();
+                sync_counts();
                 ln.slots.clear();
                 ln.busy = false;
                 progressed = true;
@@ -379,18 +412,19 @@ void SyndromeServer::run()
         for (Lane &ln : lanes) {
             if (ln.busy) continue;
             batch.clear();
-            {
-                std::lock_guard<std::mutex> lk(mu);
+            if (n_submitted.load(std::memory_order_acquire) > 0) {
+                std::lock_guard<ServerMutex> lk(mu);
                 while (!submitted.empty() && (int)batch.size() < max_batch) {
                     const int s = submitted.front();
                     submitted.pop_front();
                     st[s].state = 3;
                     batch.push_back(s);
                 }
+                sync_counts();
             }
             if (batch.empty()) break;
             if (int32_t rc = launch_batch(ln, batch)) {
-                std::lock_guard<std::mutex> lk(mu);
+                std::lock_guard<ServerMutex> lk(mu);
                 error = rc;
                 for (int s : batch) {  // report the failure per code block
                     st[s].state = 4;
@@ -398,6 +432,7 @@ void SyndromeServer::run()
                     h_it[s] = -1;
                     done.push_back(s);
                 }
+                sync_counts();
                 continue;
             }
             ln.slots = batch;
@@ -443,10 +478,11 @@ int32_t SyndromeServer::init_resident()
         hipHostGetDevicePointer((void **)&rs.ring_dev, rs.ring, 0) != hipSuccess ||
         hipHostGetDevicePointer((void **)&rs.ctl_dev, rs.ctl, 0) != hipSuccess ||
         hipHostGetDevicePointer((void **)&rs.done_dev, rs.done, 0) != hipSuccess ||
-        hipMalloc((void **)&rs.d_head, 64) != hipSuccess || hipMalloc((void **)&rs.d_last, 64) != hipSuccess ||
+        hipMalloc((void **)&rs.d_head, SYN_RESIDENT_QUEUE_BYTES) != hipSuccess ||
+        hipMalloc((void **)&rs.d_last, 64) != hipSuccess ||
         hipMalloc((void **)&rs.d_offs, 8 * (size_t)slots) != hipSuccess ||
         hipMemcpy(rs.d_offs, offs.data(), 8 * (size_t)slots, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(rs.d_head, 0, 64) != hipSuccess || hipMemset(rs.d_last, 0, 64) != hipSuccess ||
+        hipMemset(rs.d_head, 0, SYN_RESIDENT_QUEUE_BYTES) != hipSuccess || hipMemset(rs.d_last, 0, 64) != hipSuccess ||
         hipStreamCreateWithFlags(&rs.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&rs.ev, hipEventDisableTiming) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess)
@@ -520,7 +556,7 @@ void SyndromeServer::run_resident()
         {
             int open_seen = 0;
             double t = 0;
-            std::unique_lock<std::mutex> lk(mu, std::defer_lock);
+            std::unique_lock<ServerMutex> lk(mu, std::defer_lock);
             for (size_t k = 0; k < rs.outstanding.size() && open_seen <= rs.grid; ++k) {
                 Outstanding &o = rs.outstanding[k];
                 if (o.fin) continue;
@@ -541,10 +577,11 @@ void SyndromeServer::run_resident()
                 o.fin = true;
                 progressed = true;
             }
+            if (lk.owns_lock()) sync_counts();
             while (!rs.outstanding.empty() && rs.outstanding.front().fin) rs.outstanding.pop_front();
         }
-        {
-            std::lock_guard<std::mutex> lk(mu);
+        if (n_tb_out.load(std::memory_order_acquire) > 0) {
+            std::lock_guard<ServerMutex> lk(mu);
             for (size_t k = 0; k < rs.tb_outstanding.size();) {
                 const int ti = rs.tb_outstanding[k];
                 bool blocks_done = true;  // and every block's item seen complete, so release_tb can free them
@@ -560,14 +597,16 @@ void SyndromeServer::run_resident()
                     ++k;
                 }
             }
+            sync_counts();
         }
         pend.clear();
-        {
-            std::lock_guard<std::mutex> lk(mu);
+        if (n_submitted.load(std::memory_order_acquire) > 0) {
+            std::lock_guard<ServerMutex> lk(mu);
             while (!submitted.empty()) {
                 pend.push_back(submitted.front());
                 submitted.pop_front();
             }
+            sync_counts();
         }
         if (!pend.empty()) {
             for (int sl : pend) by_code[st[sl].code].push_back(sl);
@@ -603,19 +642,20 @@ void SyndromeServer::run_resident()
                 v.clear();
             }
             {
-                std::lock_guard<std::mutex> lk(mu);
+                std::lock_guard<ServerMutex> lk(mu);
                 for (int sl : pend) st[sl].state = 3;
                 for (auto it = back.rbegin(); it != back.rend(); ++it) {
                     st[*it].state = 2;
                     submitted.push_front(*it);
                 }
+                sync_counts();
             }
             __atomic_store_n(&rs.ctl->tail, rs.published, __ATOMIC_RELEASE);
             progressed = true;
         }
         if (!rs.running && !rs.outstanding.empty()) {
             if (int32_t rc = launch_resident()) {
-                std::lock_guard<std::mutex> lk(mu);
+                std::lock_guard<ServerMutex> lk(mu);
                 error = rc;
                 for (const Outstanding &o : rs.outstanding)
                     for (int j = 0; j < (o.fin ? 0 : o.n); ++j) {
@@ -625,6 +665,7 @@ void SyndromeServer::run_resident()
                         done.push_back(o.slot[j]);
                     }
                 rs.outstanding.clear();
+                sync_counts();
             }
             progressed = true;
         }
@@ -750,7 +791,7 @@ int32_t syndrome_server_acquire(SyndromeServer *s, int32_t base_graph, size_t z,
         if (!s || !out_llr || !out_slot) return SYNDROME_NULL_POINTER;
         const int zi = z <= 384 ? z_index((int)z) : -1;
         if ((base_graph != 0 && base_graph != 1) || zi < 0) return SYNDROME_INVALID_PARAM;
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (s->free_list.empty()) return SYNDROME_BUSY;
         const int slot = s->free_list.back();
         s->free_list.pop_back();
@@ -770,13 +811,14 @@ int32_t syndrome_server_submit(SyndromeServer *s, uint32_t slot, uint64_t tag)
 {
     return guard([&]() -> int32_t {
         if (!s) return SYNDROME_NULL_POINTER;
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (slot >= (uint32_t)s->slots || s->st[slot].state != 1) return SYNDROME_INVALID_PARAM;
         SyndromeServer::Slot &sl = s->st[slot];
         sl.tag = tag;
         sl.t_submit = now_ns();
         sl.state = 2;
         s->submitted.push_back((int)slot);
+        s->sync_counts();
         return SYNDROME_OK;
     });
 }
@@ -785,7 +827,7 @@ int32_t syndrome_server_submit_from(SyndromeServer *s, uint32_t slot, const floa
 {
     return guard([&]() -> int32_t {
         if (!s || !src) return SYNDROME_NULL_POINTER;
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (slot >= (uint32_t)s->slots || s->st[slot].state != 1) return SYNDROME_INVALID_PARAM;
         SyndromeServer::Slot &sl = s->st[slot];
         sl.src = src;
@@ -793,6 +835,7 @@ int32_t syndrome_server_submit_from(SyndromeServer *s, uint32_t slot, const floa
         sl.t_submit = now_ns();
         sl.state = 2;
         s->submitted.push_back((int)slot);
+        s->sync_counts();
         return SYNDROME_OK;
     });
 }
@@ -801,10 +844,12 @@ int32_t syndrome_server_try_recv(SyndromeServer *s, SyndromeServed *out)
 {
     return guard([&]() -> int32_t {
         if (!s || !out) return SYNDROME_NULL_POINTER;
-        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->n_done.load(std::memory_order_acquire) == 0) return SYNDROME_BUSY;
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (s->done.empty()) return SYNDROME_BUSY;
         const int slot = s->done.front();
         s->done.pop_front();
+        s->sync_counts();
         SyndromeServer::Slot &sl = s->st[slot];
         sl.state = 5;
         out->slot = (uint32_t)slot;
@@ -822,7 +867,7 @@ int32_t syndrome_server_release(SyndromeServer *s, uint32_t slot)
 {
     return guard([&]() -> int32_t {
         if (!s) return SYNDROME_NULL_POINTER;
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (slot >= (uint32_t)s->slots || (s->st[slot].state != 5 && s->st[slot].state != 1) || s->st[slot].tb >= 0)
             return SYNDROME_INVALID_PARAM;
         s->st[slot].state = 0;
@@ -845,7 +890,7 @@ int32_t syndrome_server_acquire_many(SyndromeServer *s, const int32_t *base_grap
                 return SYNDROME_INVALID_PARAM;
             code[i] = base_graph[i] * 51 + zi[i];
         }
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         size_t k = 0;
         for (; k < n && !s->free_list.empty(); ++k) {
             const int slot = s->free_list.back();
@@ -869,7 +914,7 @@ int32_t syndrome_server_submit_many(SyndromeServer *s, const uint32_t *slots, co
 {
     return guard([&]() -> int32_t {
         if (!s || (n && !slots)) return SYNDROME_NULL_POINTER;
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (!claim_slots(s, slots, n, {1}, -2, false)) return SYNDROME_INVALID_PARAM;
         const double t = now_ns();
         for (size_t i = 0; i < n; ++i) {
@@ -880,6 +925,7 @@ int32_t syndrome_server_submit_many(SyndromeServer *s, const uint32_t *slots, co
             sl.state = 2;
             s->submitted.push_back((int)slots[i]);
         }
+        s->sync_counts();
         return SYNDROME_OK;
     });
 }
@@ -889,7 +935,8 @@ int32_t syndrome_server_try_recv_many(SyndromeServer *s, SyndromeServed *out, si
     return guard([&]() -> int32_t {
         if (!s || !out_n || (cap && !out)) return SYNDROME_NULL_POINTER;
         *out_n = 0;
-        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->n_done.load(std::memory_order_acquire) == 0) return SYNDROME_BUSY;
+        std::lock_guard<ServerMutex> lk(s->mu);
         size_t k = 0;
         bool failed = false;
         for (; k < cap && !s->done.empty(); ++k) {
@@ -907,6 +954,7 @@ int32_t syndrome_server_try_recv_many(SyndromeServer *s, SyndromeServed *out, si
             o.done_ns = sl.t_done;
             failed = failed || o.iterations < 0;
         }
+        s->sync_counts();
         *out_n = k;
         if (!k) return SYNDROME_BUSY;
         return failed ? SYNDROME_DEVICE_ERROR : SYNDROME_OK;
@@ -917,7 +965,7 @@ int32_t syndrome_server_release_many(SyndromeServer *s, const uint32_t *slots, s
 {
     return guard([&]() -> int32_t {
         if (!s || (n && !slots)) return SYNDROME_NULL_POINTER;
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (!claim_slots(s, slots, n, {1, 5}, -3, true)) return SYNDROME_INVALID_PARAM;
         for (size_t i = 0; i < n; ++i) {
             s->st[slots[i]].state = 0;
@@ -934,7 +982,7 @@ int32_t syndrome_server_submit_tb(SyndromeServer *s, const uint32_t *slots, cons
         if (!s || !slots || !tb_out) return SYNDROME_NULL_POINTER;
         if (!s->resident || num_cb == 0 || num_cb > (size_t)s->slots || tb_size == 0 || tb_size > (size_t)INT_MAX)
             return SYNDROME_INVALID_PARAM;
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         const SyndromeServer::Slot &f = s->st[slots[0] < (uint32_t)s->slots ? slots[0] : 0];
         for (size_t i = 0; i < num_cb; ++i) {
             if (slots[i] >= (uint32_t)s->slots) return SYNDROME_INVALID_PARAM;
@@ -969,6 +1017,7 @@ int32_t syndrome_server_submit_tb(SyndromeServer *s, const uint32_t *slots, cons
             s->submitted.push_back((int)slots[i]);
         }
         s->rs.tb_outstanding.push_back(ti);
+        s->sync_counts();
         return SYNDROME_OK;
     });
 }
@@ -977,10 +1026,12 @@ int32_t syndrome_server_try_recv_tb(SyndromeServer *s, SyndromeServedTb *out)
 {
     return guard([&]() -> int32_t {
         if (!s || !out) return SYNDROME_NULL_POINTER;
-        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->n_tb_done.load(std::memory_order_acquire) == 0) return SYNDROME_BUSY;
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (s->rs.tb_done_q.empty()) return SYNDROME_BUSY;
         const int ti = s->rs.tb_done_q.front();
         s->rs.tb_done_q.pop_front();
+        s->sync_counts();
         SyndromeServer::Resident::TbRec &r = s->rs.tbr[ti];
         r.state = 5;
         int its = 0;
@@ -1001,7 +1052,7 @@ int32_t syndrome_server_release_tb(SyndromeServer *s, uint32_t tb)
 {
     return guard([&]() -> int32_t {
         if (!s) return SYNDROME_NULL_POINTER;
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<ServerMutex> lk(s->mu);
         if (!s->resident || tb >= s->rs.tb_cap || s->rs.tbr[tb].state != 5) return SYNDROME_INVALID_PARAM;
         SyndromeServer::Resident::TbRec &r = s->rs.tbr[tb];
         for (int sl : r.slots) {
