@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         if (threadIdx.x == 0) {
             unsigned w = 0;
             int have = 0;
-            int backoff = 1;  // sleeps of 512 cycles between idle polls, doubling to 16
+            int backoff = 1;  // sleeps of 512 cycles between idle polls, doubling to 64
             // Idle workgroups back off exponentially between polls: each
             // poll's acquire invalidates this XCD's L2 (each XCD has its own),
             // and idle workgroups polling without pause kept evicting the
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
                     now - t0 > life_ticks)
                     break;
                 for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(8);
-                backoff = backoff < 16 ? 2 * backoff : 16;
+                backoff = backoff < 64 ? 2 * backoff : 64;
             }
             s_w = w;
             s_have = have;
