@@ -37,6 +37,67 @@ def _blocks(oracle, count, seed):
 
 
 @pytest.mark.parametrize("streams", [3, 0], ids=["lanes", "resident"])
+def test_server_concurrent_clients(oracle, streams):
+    """Four client threads share one server, each acquiring, submitting,
+    receiving (whichever blocks are ready, its own or not) and releasing at
+    once: the server's lock and its lock-free queue counts under contention.
+    Every block must come back exactly once, equal to the oracle's decode."""
+    import threading
+    import syndrome_amd
+    from syndrome_amd import Server
+    syndrome_amd.lib()
+    say(f"[serve] 4 concurrent clients x 30 blocks, streams={streams}")
+    blocks = _blocks(oracle, 120, 23)
+    srv = Server(0.5, 8, hbm_budget_bytes=64 << 20, max_batch=32, streams=streams)
+    results, lock = {}, threading.Lock()
+    received = [0]
+    errors = []
+
+    def client(k):
+        try:
+            mine = list(range(k, len(blocks), 4))
+            t_end = time.time() + 60
+            i = 0
+            while time.time() < t_end:
+                if i < len(mine):
+                    b, z, nf, llr = blocks[mine[i]]
+                    got = srv.acquire(b, z, nf)
+                    if got is not None:
+                        slot, buf = got
+                        buf[:] = llr
+                        srv.submit(slot, mine[i])
+                        i += 1
+                r = srv.try_recv()
+                if r is not None:
+                    slot, tag, its, hard, _, _ = r
+                    with lock:
+                        assert tag not in results, f"block {tag} delivered twice"
+                        results[tag] = (its, hard)
+                        received[0] += 1
+                    srv.release(slot)
+                with lock:
+                    if received[0] == len(blocks):
+                        return
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(90)
+    srv.close()
+    assert not errors, errors
+    assert len(results) == len(blocks), f"only {len(results)} of {len(blocks)} completed"
+    for tag, (b, z, nf, llr) in enumerate(blocks):
+        code = oracle.code_3gpp(b, z)
+        ri, _, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, llr, nf), 8)
+        its, hard = results[tag]
+        assert its == ri, f"block {tag} (BG{b + 1} Z={z}): iterations {its} != {ri}"
+        assert np.array_equal(hard, rh), f"block {tag}: hard decisions differ"
+
+
+@pytest.mark.parametrize("streams", [3, 0], ids=["lanes", "resident"])
 def test_server_results_equal_alone_and_in_the_mix(oracle, streams):
     import syndrome_amd
     from syndrome_amd import QcLdpcDecoder, Server, _lib as L
