@@ -98,20 +98,21 @@ __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ 
             }
             return v;
         };
-        // Without HARQ combining and with E <= m, a position has at most one
-        // selection, k = its rank r, and only when r < E: the general element
-        // below reduces to 0 + el[q(r)] (the add keeps the reference's "+="
-        // into zeros, which turns -0 into +0).
-        // The load is unconditional (index 0 when nothing is selected; E is a
-        // multiple of Qm, so q(r) < E for r < E) so that a lane's loads go
-        // out together instead of one branch and wait each.
-        const bool simple = !combine && qm > 0 && m > 0 && e > 0 && e <= m;
+        // With E <= m a position has at most one selection, k = its rank r,
+        // and only when r < E: the general element below reduces to
+        // v0 + el[q(r)], v0 = 0 or, when combining, the value already held
+        // (the add keeps the reference's "+=", which turns 0 + -0 into +0).
+        // The loads are unconditional (index 0 when nothing is selected; E
+        // is a multiple of Qm, so q(r) < E for r < E) so that a lane's loads
+        // go out together instead of one branch and wait each.
+        const bool simple = qm > 0 && m > 0 && e > 0 && e <= m;
         auto elem1 = [&](int p) {
             const int c = p - 2 * z;
             const int r = c >= k0 ? below(c) - bk0 : (bncb - bk0) + below(c);
             const bool take = c >= 0 && !(c >= fs && c < fe) && r < e;
             const float x = el[take ? q_of(r) : 0];
-            return take ? 0.f + x : 0.f;
+            const float v0 = (combine && c >= 0) ? out[out_off + p] : 0.f;
+            return take ? v0 + x : v0;
         };
         // 4 consecutive elements per lane, one 16-byte store when aligned
         const bool vec = (out_off & 3) == 0;
