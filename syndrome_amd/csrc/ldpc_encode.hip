@@ -40,7 +40,7 @@ namespace syn {
 
 namespace {
 
-constexpr int ENC_THREADS = 256;
+constexpr int ENC_THREADS = 128;
 constexpr int ENC_SMALL_N = 8192;  // codeword bits up to which a workgroup is one wave
 
 // Sizes of one code, shared by every phase.
