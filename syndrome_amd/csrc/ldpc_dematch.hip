@@ -37,7 +37,7 @@ namespace syn {
 
 namespace {
 
-constexpr int DM_THREADS = 256;
+constexpr int DM_THREADS = 128;
 // Workgroups of a launch at most (one per chunk below that; they stride
 // over the chunk table above it).
 constexpr int DM_GRID = 1 << 20;
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(DM_THREADS) dematch(const float *__restrict__ 
         // 4 consecutive elements per lane, one 16-byte store when aligned
         const bool vec = (out_off & 3) == 0;
         if (simple && vec && p_end - ch.p0 == SYN_DEMATCH_CHUNK) {
-            // a full chunk: every load of the lane's four groups is issued
+            // a full chunk: every load of the lane's G groups is issued
             // before the first store
             constexpr int G = SYN_DEMATCH_CHUNK / (4 * DM_THREADS);
             float v[G][4];
