@@ -1271,18 +1271,16 @@ int32_t syndrome_server_loadgen_tb(SyndromeServer *s, const float *pool, const i
                 Pending &pd = pending.front();
                 const int p = pd.pool_tb, c = tb_num_cb[p], f = tb_first_cb[p];
                 bool busy = false;
-                while ((int)pd.slots.size() < c) {  // a transport block holds its slots as it gets them
-                    float *llr = nullptr;
-                    uint32_t slot = 0;
-                    const int cb = f + (int)pd.slots.size();
-                    const int32_t rc = syndrome_server_acquire(s, cb_bg[cb], (size_t)cb_z[cb],
-                                                               cb_nfill ? (size_t)cb_nfill[cb] : 0, &llr, &slot);
-                    if (rc == SYNDROME_BUSY) {
-                        busy = true;
-                        break;
-                    }
-                    if (rc) return rc;
-                    pd.slots.push_back(slot);
+                if ((int)pd.slots.size() < c) {  // a transport block holds its slots as it gets them
+                    const int have = (int)pd.slots.size(), want = c - have;
+                    pd.slots.resize(c);
+                    size_t got = 0;
+                    const int32_t rc = syndrome_server_acquire_many(s, cb_bg + f + have, cb_z + f + have,
+                                                                    cb_nfill ? cb_nfill + f + have : nullptr,
+                                                                    (size_t)want, nullptr, pd.slots.data() + have, &got);
+                    pd.slots.resize(have + got);
+                    if (rc && rc != SYNDROME_BUSY) return rc;
+                    busy = (int)got < want;
                 }
                 if (!busy) {
                     srcs.resize(c);
