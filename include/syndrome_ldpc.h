@@ -203,11 +203,14 @@ int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int
                                   const int64_t *offsets, size_t count, float offset_beta, int32_t int8,
                                   int32_t block, SyndromeLdpcPlan **out_plan);
 void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *plan);
-/* One persistent launch over the plan.  int8 plans take i16 (or f32 with
- * SYNDROME_BATCH_QUANTIZE) inputs and write i16 posteriors. */
+/* The plan's persistent launch (two when its largest Z exceeds 256: the
+ * codes of Z <= 256 get a second launch of 256-lane workgroups).  int8
+ * plans take i16 (or f32 with SYNDROME_BATCH_QUANTIZE) inputs and write i16
+ * posteriors. */
 int32_t syndrome_ldpc_plan_run(SyndromeLdpcPlan *plan, const void *d_in, void *d_post, uint8_t *d_hard,
                                int32_t *d_iters, size_t iterations, float scale, int16_t app_clamp, uint32_t flags,
                                void *stream);
+/* The main launch's block, grid and LDS bytes; items of both launches. */
 int32_t syndrome_ldpc_plan_shape(const SyndromeLdpcPlan *plan, int32_t *out_block, int32_t *out_grid,
                                  int32_t *out_items, int32_t *out_lds_bytes);
 /* Launch-shape tuner: times the plan at each workgroup size from the

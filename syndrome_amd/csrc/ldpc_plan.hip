@@ -42,13 +42,17 @@ struct SyndromeLdpcPlan {
     // unique codes
     std::vector<SynCode> codes;
     std::vector<int> cb_code;
-    // current launch shape
+    // current launch shape; a plan whose largest Z exceeds SPLIT_LANES
+    // decodes its codes of Z <= SPLIT_LANES in a second launch (the *_s
+    // fields) of SPLIT_LANES-lane workgroups
     int block = 0, grid = 0, n_items = 0;
     size_t lds = 0;
+    int block_s = 0, grid_s = 0, n_items_s = 0;
+    size_t lds_s = 0;
     std::vector<int> cws_of_code;
     // device buffers
     SynCode *d_codes = nullptr;
-    SynItem *d_items = nullptr;
+    SynItem *d_items = nullptr, *d_items_s = nullptr;
     int *d_ids = nullptr, *d_nfill = nullptr, *d_head = nullptr;
     long long *d_offs = nullptr;
     SynDematchCb *d_dm = nullptr;
@@ -97,6 +101,11 @@ int32_t upload(T **dst, const std::vector<T> &src)
     return SYNDROME_OK;
 }
 
+// Lanes of the small codes' workgroups in a split plan, and the spacing
+// (ints, one 128-byte line) of the two launches' queue counters.
+constexpr int SPLIT_LANES = 256;
+constexpr int HEAD_STRIDE = 32;
+
 size_t group_bytes(const SynCode &c, int cws, bool i8)
 {
     const size_t esz = i8 ? 2 : 4, n = syn::code_n(c);
@@ -104,22 +113,31 @@ size_t group_bytes(const SynCode &c, int cws, bool i8)
     return pad + ((cws * n * esz + 3) & ~(size_t)3) + 3 * sizeof(int) * (size_t)cws;
 }
 
-// Items for a workgroup of `block` lanes: each code's blocks in groups of
-// as many codewords as fit the lanes and the LDS, longest work first.
+// Items for workgroups of `block` lanes: each code's blocks in groups of as
+// many codewords as fit the lanes and the LDS, longest work first.  When the
+// plan's largest Z exceeds SPLIT_LANES, codes of Z <= SPLIT_LANES form a
+// second launch of SPLIT_LANES-lane workgroups with their own (smaller) LDS:
+// more of those share a CU (the int8 kernel fits 12 waves per CU, one
+// 512-lane workgroup only 8), and their waves spread evenly over the SIMDs.
 int32_t build_items(SyndromeLdpcPlan *p, int block)
 {
     if (block <= 0 || block > 512 || block % 64) return SYNDROME_INVALID_PARAM;
     const size_t nc = p->codes.size();
-    std::vector<int> cws(nc);
-    size_t lds = 0;
+    int max_z = 0;
+    for (const SynCode &code : p->codes) max_z = std::max(max_z, code.z);
+    const bool split = max_z > SPLIT_LANES && block > SPLIT_LANES;
+    std::vector<int> cws(nc), part(nc, 0);
+    size_t lds[2] = {0, 0};
     for (size_t c = 0; c < nc; ++c) {
         const SynCode &code = p->codes[c];
-        if (code.z > block) return SYNDROME_INVALID_PARAM;
-        int k = std::max(1, block / code.z);
+        part[c] = split && code.z <= SPLIT_LANES ? 1 : 0;
+        const int lanes = part[c] ? SPLIT_LANES : block;
+        if (code.z > lanes) return SYNDROME_INVALID_PARAM;
+        int k = std::max(1, lanes / code.z);
         while (k > 1 && group_bytes(code, k, p->i8) > (size_t)p->lds_limit) --k;
         if (group_bytes(code, k, p->i8) > (size_t)p->lds_limit) return SYNDROME_INVALID_PARAM;
         cws[c] = k;
-        lds = std::max(lds, group_bytes(code, k, p->i8));
+        lds[part[c]] = std::max(lds[part[c]], group_bytes(code, k, p->i8));
     }
     std::vector<std::vector<int>> members(nc);
     for (size_t i = 0; i < p->cb_code.size(); ++i) members[p->cb_code[i]].push_back((int)i);
@@ -127,7 +145,7 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         SynItem it;
         long long cost;
     };
-    std::vector<Item> items;
+    std::vector<Item> items[2];
     std::vector<int> ids;
     for (size_t c = 0; c < nc; ++c)
         for (size_t j = 0; j < members[c].size(); j += (size_t)cws[c]) {
@@ -137,23 +155,35 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
             x.cost = (long long)p->codes[c].n_edges * p->codes[c].z * ((cnt + 0LL));
             x.it.first = (int)ids.size();
             for (int q = 0; q < cnt; ++q) ids.push_back(members[c][j + q]);
-            items.push_back(x);
+            items[part[c]].push_back(x);
         }
-    // longest-processing-time first: the queue hands out big items early
-    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
-    std::vector<SynItem> flat;
-    for (auto &x : items) flat.push_back(x.it);
-    if (int32_t rc = upload(&p->d_items, flat)) return rc;
+    std::vector<SynItem> flat[2];
+    for (int q = 0; q < 2; ++q) {
+        // longest-processing-time first: the queue hands out big items early
+        std::stable_sort(items[q].begin(), items[q].end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+        for (auto &x : items[q]) flat[q].push_back(x.it);
+    }
+    if (int32_t rc = upload(&p->d_items, flat[0])) return rc;
+    if (int32_t rc = upload(&p->d_items_s, flat[1])) return rc;
     if (int32_t rc = upload(&p->d_ids, ids)) return rc;
     const void *fn = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32();
     if (syn::allow_max_dynamic_lds(fn)) return SYNDROME_DEVICE_ERROR;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds[0]) != hipSuccess || per_cu < 1)
         per_cu = 1;
     p->block = block;
-    p->lds = lds;
-    p->n_items = (int)flat.size();
+    p->lds = lds[0];
+    p->n_items = (int)flat[0].size();
     p->grid = std::max(1, std::min(p->n_items, per_cu * p->cus));
+    p->block_s = SPLIT_LANES;
+    p->lds_s = lds[1];
+    p->n_items_s = (int)flat[1].size();
+    p->grid_s = 0;
+    if (p->n_items_s) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, SPLIT_LANES, lds[1]) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        p->grid_s = std::max(1, std::min(p->n_items_s, per_cu * p->cus));
+    }
     p->cws_of_code = cws;
     return SYNDROME_OK;
 }
@@ -182,18 +212,23 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
         const float b = p->beta * scale;
         a.beta_q = !(b - b == 0.f) ? 127 : (int)std::min(127.f, std::max(0.f, roundf(b)));
     }
-    if (syn::launch_reset(p->d_head, stream)) return SYNDROME_DEVICE_ERROR;
     const void *fn = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32();
     const SynCode *codes = p->d_codes;
-    const SynItem *items = p->d_items;
-    int n_items = p->n_items;
-    int *head = p->d_head;
     const int *ids = p->d_ids;
     const long long *offs = p->d_offs;
     const int *nf = p->d_nfill;
-    void *args[] = {&codes, &items, &n_items, &head, &ids, &offs, &nf, &a};
-    if (hipLaunchKernel(fn, dim3(p->grid), dim3(p->block), args, p->lds, stream) != hipSuccess)
-        return SYNDROME_DEVICE_ERROR;
+    // the main launch, then the small codes' (each with its own counter)
+    for (int q = 0; q < 2; ++q) {
+        const SynItem *items = q ? p->d_items_s : p->d_items;
+        int n_items = q ? p->n_items_s : p->n_items;
+        if (n_items == 0) continue;
+        int *head = p->d_head + q * HEAD_STRIDE;
+        if (syn::launch_reset(head, stream)) return SYNDROME_DEVICE_ERROR;
+        void *args[] = {&codes, &items, &n_items, &head, &ids, &offs, &nf, &a};
+        if (hipLaunchKernel(fn, dim3(q ? p->grid_s : p->grid), dim3(q ? p->block_s : p->block), args,
+                            q ? p->lds_s : p->lds, stream) != hipSuccess)
+            return SYNDROME_DEVICE_ERROR;
+    }
     return SYNDROME_OK;
 }
 
@@ -242,7 +277,7 @@ int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int
         if (int32_t rc = upload(&p->d_codes, p->codes)) return rc;
         if (int32_t rc = upload(&p->d_offs, p->offs)) return rc;
         if (int32_t rc = upload(&p->d_nfill, p->nfill)) return rc;
-        if (hipMalloc((void **)&p->d_head, 64) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        if (hipMalloc((void **)&p->d_head, 2 * HEAD_STRIDE * sizeof(int)) != hipSuccess) return SYNDROME_DEVICE_ERROR;
         const int min_block = ((max_z + 63) / 64) * 64;
         const int b = block > 0 ? block : std::max(min_block, 512);
         if (int32_t rc = build_items(p.get(), std::max(b, min_block))) return rc;
@@ -256,7 +291,7 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *p)
     if (!p) return;
     try {
         DeviceScope ds(p->device);
-        for (void *q : {(void *)p->d_codes, (void *)p->d_items, (void *)p->d_ids, (void *)p->d_nfill,
+        for (void *q : {(void *)p->d_codes, (void *)p->d_items, (void *)p->d_items_s, (void *)p->d_ids, (void *)p->d_nfill,
                         (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm, (void *)p->d_dm_chunks,
                         (void *)p->d_tbs})
             if (q) (void)hipFree(q);
@@ -286,7 +321,7 @@ int32_t syndrome_ldpc_plan_shape(const SyndromeLdpcPlan *p, int32_t *block, int3
         if (!p) return SYNDROME_NULL_POINTER;
         if (block) *block = p->block;
         if (grid) *grid = p->grid;
-        if (items) *items = p->n_items;
+        if (items) *items = p->n_items + p->n_items_s;  // both launches' items
         if (lds_bytes) *lds_bytes = (int32_t)p->lds;
         return SYNDROME_OK;
     });
