@@ -589,10 +589,13 @@ def extra_benches(torch, args, peak_gbps):
             ("encode_bg1_z384", lambda: bench_encode(torch, args)))
 
 
-def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, waves_per_cw, edges, iters, clock_ghz=2.4, cus=256):
+def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, waves_per_cw, edges, iters, clock_ghz=2.4, cus=256,
+                 kernel_ms=None, lds_bytes=4):
     """Append the SQ counters of the profiled kernel's full-batch dispatches
     (mean over them) to a profile summary, with the VALU issue share they
-    imply: VALU instructions x 4 cycles over cus x 4 SIMDs x kernel time."""
+    imply: VALU instructions x 4 cycles over cus x 4 SIMDs x kernel time.
+    The kernel time is the summary's traffic JSON's, or kernel_ms when given;
+    lds_bytes is what each LDS instruction moves per lane."""
     import csv
     from collections import defaultdict
     vals = defaultdict(list)
@@ -604,8 +607,9 @@ def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, waves_per_cw, edges, iter
             if int(r["Grid_Size"]) == big:
                 vals[r["Counter_Name"]].append(float(r["Counter_Value"]))
     mean = {k: float(np.mean(v)) for k, v in vals.items()}
-    traffic = json.load(open(out_md[:-3] + "_traffic.json"))
-    kern_s = traffic["average_ms_full_batch"] / 1e3
+    if kernel_ms is None:
+        kernel_ms = json.load(open(out_md[:-3] + "_traffic.json"))["average_ms_full_batch"]
+    kern_s = kernel_ms / 1e3
     waves = mean.get("SQ_WAVES", 0.0)
     valu = mean.get("SQ_INSTS_VALU", 0.0)
     issue = valu * 4 / (cus * 4 * kern_s * clock_ghz * 1e9)
@@ -625,9 +629,10 @@ def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, waves_per_cw, edges, iter
                   f"{kern_s * 1e3:.3f} ms kernel ({clock_ghz} GHz) = {issue * 100:.0f} % of the device's VALU issue slots."]
     lds = mean.get("SQ_INSTS_LDS", 0.0)
     if lds:
-        lds_tbs = lds * 64 * 4 / kern_s / 1e12  # every LDS instruction of this kernel moves 4 bytes per lane
+        lds_tbs = lds * 64 * lds_bytes / kern_s / 1e12  # every LDS instruction moves lds_bytes per lane
         lds_peak = cus * 128 * clock_ghz * 1e9 / 1e12
-        lines += [f"LDS throughput: {lds:.3e} LDS instructions x 64 lanes x 4 B over the kernel = {lds_tbs:.1f} TB/s, "
+        lines += [f"LDS throughput: {lds:.3e} LDS instructions x 64 lanes x {lds_bytes} B over the kernel = "
+                  f"{lds_tbs:.1f} TB/s, "
                   f"{lds_tbs / lds_peak * 100:.0f} % of the {lds_peak:.1f} TB/s the CUs' LDS can serve (128 B per CU per cycle)."]
     open(out_md, "a").write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -667,10 +672,17 @@ def main():
         summarize_trace(*sys.argv[2:7])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-sq":
-        # bench.py --summarize-sq OUT_MD KERNEL WAVES_PER_CW EDGES MEAN_ITERS SQ_DIR1 CMD1 [SQ_DIR2 CMD2 ...]
+        # bench.py --summarize-sq [--kernel-ms MS] [--lds-bytes B] OUT_MD KERNEL WAVES_PER_CW EDGES MEAN_ITERS
+        #          SQ_DIR1 CMD1 [SQ_DIR2 CMD2 ...]
         a = sys.argv[2:]
+        opt = {}
+        while a and a[0] in ("--kernel-ms", "--lds-bytes"):
+            opt[a[0]] = a[1]
+            a = a[2:]
         rest = a[5:]
-        summarize_sq(rest[0::2], a[0], a[1], rest[1::2], int(a[2]), int(a[3]), float(a[4]))
+        summarize_sq(rest[0::2], a[0], a[1], rest[1::2], int(a[2]), int(a[3]), float(a[4]),
+                     kernel_ms=float(opt["--kernel-ms"]) if "--kernel-ms" in opt else None,
+                     lds_bytes=int(opt.get("--lds-bytes", 4)))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-profiles":
         # bench.py --summarize-profiles TRACE_DIR FETCH_DIR WRITE_DIR OUT_PREFIX KERNEL CMD_TRACE CMD_PMC
