@@ -102,6 +102,16 @@ __device__ __forceinline__ int med3i(int x, int lo, int hi)
     return r;
 }
 
+// (a ^ b) + c in one VALU operation.  Written out because the compiler
+// rewrites "(m ^ n) + c - n" (a sign applied to a magnitude, then added) as
+// two bit-field extracts, an xor and an add3, one operation more.
+__device__ __forceinline__ int xad(int a, int b, int c)
+{
+    int r;
+    asm("v_xad_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 template <int I, int N>
 struct SFor {
     template <class F>

@@ -271,6 +271,16 @@ __device__ __forceinline__ int r_i8(int g1, int g2, uint32_t s, uint32_t am, int
     return (m ^ neg) - neg;
 }
 
+// x + R for the R of r_i8, as (m ^ n) + x - n with n = 0 or -1 from the
+// sign bit.  Given the inverted sign bits it returns x - R.
+template <int D>
+__device__ __forceinline__ int add_r_i8(int x, int g1, int g2, uint32_t s, uint32_t am, int e)
+{
+    const int m = (am == (uint32_t)e) ? g2 : g1;
+    const int n = __builtin_amdgcn_sbfe((int)s, D - 1 - e, 1);  // one v_bfe_i32
+    return xad(m, n, x) - n;
+}
+
 // One layer of the f32 decode (src/qc_ldpc.rs:1072-1204).  KEEPQ keeps q's
 // bits in registers from pass 1 to pass 2, which saves one operation per
 // edge in pass 2 for D more live registers.
@@ -307,8 +317,13 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC, WM> &
         if constexpr (!KEEPQ) am = (a <= m1) ? (uint32_t)e : am;
         // min(m2, max(m1, a)) as m1 <= m2: one v_med3 (the register-lean
         // build keeps the compiler's max/min form, which needs fewer VGPRs)
-        if constexpr (KEEPQ) m2 = med3u(a, m1, m2);
-        else m2 = clampu(a, m1, m2);
+        // (on edge 0, m1 == m2 == FLT_MAX and m2 stays)
+        if constexpr (e == 0) {
+        } else if constexpr (KEEPQ) {
+            m2 = med3u(a, m1, m2);
+        } else {
+            m2 = clampu(a, m1, m2);
+        }
         m1 = a < m1 ? a : m1;
     });
     // mag = me - beta, then (mag < 0) ? 0 : mag, as one signed-integer max
@@ -364,7 +379,7 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
     static_assert(D >= 2 && D <= 27, "row degree: its signs and argmin share one 32-bit word");
     int16_t *adr[D];
-    int Av[D], ro[D];
+    int Av[D], qv[D];
     SFor<0, D>::run([&](auto ec) {
         constexpr int e = decltype(ec)::value;
         adr[e] = (int16_t *)geo.template addr<B + e, 2>(ln);
@@ -378,18 +393,20 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
     int av[D];  // |q| per edge, kept for pass 2's argmin
     SFor<0, D>::run([&](auto ec) {
         constexpr int e = decltype(ec)::value;
-        ro[e] = r_i8<D>(o1, o2, os, oam, e);
-        // Q-build.  The reference clamps q to [-127, 127]; the clamp changes
-        // no output here, so it is left out: the sign is the same, and with
+        // Q-build: q = A - R_old (R_old's sign from the inverted bits).
+        // The reference clamps q to [-127, 127]; the clamp changes no output here, so it is left out: the sign is the same, and with
         // m1 = m2 = 127 at the start an |q| above 127 moves neither min1 nor
         // min2.  It could only change "a == m1" / "a <= m1" when min1 ends
         // at 127, and then min2 == min1, so the argmin does not matter.
-        const int q = Av[e] - ro[e];
+        const int q = add_r_i8<D>(Av[e], o1, o2, ~os, oam, e);
+        qv[e] = q;
         qs = __builtin_amdgcn_alignbit(qs, (uint32_t)q, 31);
         const int a = q < 0 ? -q : q;
         if constexpr (KEEPA) av[e] = a;
         else am = (a <= m1) ? (uint32_t)e : am;  // pass 1: sorted insertion
-        m2 = med3i(a, m1, m2);  // min(m2, max(m1, a)), as m1 <= m2 always
+        // min(m2, max(m1, a)), as m1 <= m2 always; on edge 0, m1 == m2 == 127
+        // and m2 stays
+        if constexpr (e > 0) m2 = med3i(a, m1, m2);
         m1 = a < m1 ? a : m1;
     });
     int g1 = m1 - bq;
@@ -402,17 +419,18 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
             constexpr int e = decltype(ec)::value;
             // the argmin is the last edge whose |q| equals min1 (where pass
             // 1's sorted insertion "a <= m1" ends); R_new takes g2 there
-            int rn;
+            // A + (R_new - R_old) = q + R_new; the saturating add and the
+            // clamp after it equal the clamp alone
+            int nv;
             if constexpr (KEEPA) {
                 const bool at = av[e] == m1;
                 am = at ? (uint32_t)e : am;
                 const int mag = at ? g2 : g1;
-                const int neg = -(int)((ns >> (D - 1 - e)) & 1u);
-                rn = (mag ^ neg) - neg;
+                const int neg = __builtin_amdgcn_sbfe((int)ns, D - 1 - e, 1);
+                nv = xad(mag, neg, qv[e]) - neg;
             } else {
-                rn = r_i8<D>(g1, g2, ns, am, e);
+                nv = add_r_i8<D>(qv[e], g1, g2, ns, am, e);
             }
-            const int nv = Av[e] + (rn - ro[e]);  // saturating add + clamp == clamp
             // clamp to [lo, hi] = [-C, C] (C >= 0) as one v_med3 on bounds
             // held in VGPRs: the compiler's max/min pair reads them from
             // SGPRs, and one VALU instruction cannot read two SGPRs
