@@ -417,10 +417,10 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
     if (act) {
         SFor<0, D>::run([&](auto ec) {
             constexpr int e = decltype(ec)::value;
-            // the argmin is the last edge whose |q| equals min1 (where pass
-            // 1's sorted insertion "a <= m1" ends); R_new takes g2 there
+            // The argmin is the last edge whose |q| equals min1 (where pass
+            // 1's sorted insertion "a <= m1" ends); R_new takes g2 there.
             // A + (R_new - R_old) = q + R_new; the saturating add and the
-            // clamp after it equal the clamp alone
+            // clamp after it equal the clamp alone.
             int nv;
             if constexpr (KEEPA) {
                 const bool at = av[e] == m1;
