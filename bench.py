@@ -36,7 +36,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "info Gbit/s (BG1 Z=384, 8 it) batched vs plain; p50/p99 decode latency"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fast_f32_bg1_z384_b4096_traffic.json")
 
 
 def log(*a):
@@ -84,13 +83,76 @@ def time_launches(fn, steps, warmup, torch):
 
 
 def _cpu_model():
+    """The host CPU as lscpu names it (model, family/model/stepping, sockets,
+    cores, threads), or /proc/cpuinfo's model name without lscpu."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        keys = ("Model name", "Vendor ID", "CPU family", "Model", "Stepping", "Socket(s)", "Core(s) per socket",
+                "Thread(s) per core", "CPU(s)", "CPU max MHz")
+        got = {k: kv[k] for k in keys if k in kv}
+        if got:
+            return got
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+                return {"Model name": line.split(":", 1)[1].strip()}
     except OSError:
         pass
-    return "unknown"
+    return {"Model name": "unknown"}
+
+
+def _host_cores():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(16, n))
+
+
+CPU_DESC = ("the reference's CPU decode restated: its AVX2 {kind} kernels (simd_avx2.rs) driven through its "
+            "SPSC worker pipeline (ldpc_pipeline.rs), workers pinned, tested bit-exact against the oracle "
+            "(tests/test_cpu_baseline.py)")
+
+
+def cpu_pipeline_baseline(codes, frame_code, offs, arena, info_bits, iters, beta, int8=False, n_filler=None,
+                          budget_s=8.0, seed=7, points=None):
+    """The CPU pipeline (oracle/cpu_baseline.cpp) on a bounded sample of a
+    workload: frames drawn at random from the workload's own decoder inputs
+    (arena, per-frame code and offset), a worker sweep over `points` (default
+    1 and every core this process may use, at most 16), each point sized for
+    about budget_s / len(points) seconds.  `value` = info Gbit/s of the best
+    point (info_bits per frame, summed over the stream)."""
+    from oracle import orc
+    cores = _host_cores()
+    points = sorted(set(points or (1, cores)))
+    rng = np.random.default_rng(seed)
+    nfr = len(frame_code)
+    probe = rng.integers(0, nfr, 32).astype(np.int32)
+    t1, _, _ = orc.pipeline_mixed(codes, frame_code, offs, arena, iters, 1, order=probe, int8=int8, beta=beta,
+                                  n_filler=n_filler)
+    per_frame = t1 / len(probe)
+    sweep, best = {}, None
+    for w in points:
+        count = int(min(max(budget_s / len(points) / per_frame * w * 0.8, 64), 200000))
+        order = rng.integers(0, nfr, count).astype(np.int32)
+        el, its, _ = orc.pipeline_mixed(codes, frame_code, offs, arena, iters, w, order=order, int8=int8,
+                                        beta=beta, n_filler=n_filler)
+        bits = float(np.asarray(info_bits, np.float64)[order].sum())
+        gbps = bits / el / 1e9
+        sweep[str(w)] = round(gbps, 6)
+        if best is None or gbps > best[0]:
+            best = (gbps, w, count, el, float(its[np.unique(order)].mean()))
+    gbps, w, count, el, mit = best
+    return {"value": round(gbps, 6), "unit": "Gbit/s", "cores": w, "kind": "port",
+            "sample": f"{count} frames drawn at random from this workload's own decoder inputs, through "
+                      + CPU_DESC.format(kind="int8" if int8 else "f32") + f"; {w} workers, {el:.1f} s; "
+                      f"mean iterations {mit:.3f}",
+            "worker_sweep_gbps": sweep, "cpu_model": _cpu_model(), "host_cores_available": cores}
 
 
 def cpu_baseline(llr, iters, beta, budget_s=12.0):
@@ -101,7 +163,7 @@ def cpu_baseline(llr, iters, beta, budget_s=12.0):
     budget_s / (sweep points) seconds; `value` is the best point."""
     from oracle import orc
     code = orc.code_3gpp(0, 384)
-    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    cores = _host_cores()
     t1, _, _ = orc.pipeline_run(code, beta, llr[:96], iters, 1)
     one = 96 / t1  # codewords per second on one worker
     points = sorted({w for w in (1, 2, 4, 8, 16) if w <= cores} | {cores})
@@ -116,10 +178,75 @@ def cpu_baseline(llr, iters, beta, budget_s=12.0):
         if gbps > best:
             best, best_w, best_n, best_el = gbps, w, count, el
     return {"value": round(best, 6), "unit": "Gbit/s", "cores": best_w, "kind": "port",
-            "sample": f"{best_n} codewords of the same workload through an AVX2 restatement of the reference decode "
-                      f"and its SPSC worker pipeline, {best_w} pinned workers, {best_el:.1f} s; "
-                      f"mean iterations {mean_it:.3f}",
+            "sample": f"{best_n} codewords of the same workload through " + CPU_DESC.format(kind="f32")
+                      + f"; {best_w} workers, {best_el:.1f} s; mean iterations {mean_it:.3f}",
             "worker_sweep_gbps": sweep, "cpu_model": _cpu_model(), "host_cores_available": cores}
+
+
+# ---------------------------------------------------------------------------
+# roofline: the decode kernels are bound by VALU issue, not by HBM
+# ---------------------------------------------------------------------------
+
+# MI355X VALU peak in lane-operations per second: 256 CUs x 4 SIMDs x 16
+# lanes x 2.4 GHz = 39.3 T (the published 157.3 TFLOP/s fp32 counts an FMA
+# as two operations and packed-FP32 dual issue as two more).
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+# Algorithmic operations per edge-slot-iteration ((edge, z) pair per layered
+# pass), counted from the reference's scalar kernel with nothing done twice
+# (its pass 2 recomputes |q| and q's sign, counted once):
+#  f32 (qc_ldpc.rs:1092, 1155-1178, 1181-1204): Q-build sub 1; pass 1: |q|
+#   mask, sign mask, sign xor, a <= m1, a < m2 (5); pass 2: exclusive sign
+#   xor, a == m1, select, - beta, mag < 0, select, or sign, R' - R, L + d (9):
+#   15 in all;
+#  int8 (qc_ldpc.rs:1639-1640, 1679-1690, 1708-1742): Q-build sub, clamp
+#   max, min (3); pass 1: abs, sign mask, xor, min, max, min (6); pass 2:
+#   exclusive sign xor, a == m1, select, - beta_q, max 0, sign as neg =
+#   s >> 7, xor, sub (3), delta, saturating add, clamp max, min (13): 22.
+ALG_OPS_PER_EDGE_SLOT = {"f32": 15, "i8": 22}
+
+
+def _profile_json(name, suffix):
+    path = os.path.join(ROOT, "profiles", f"{name}_{suffix}.json")
+    if os.path.exists(path):
+        try:
+            return json.load(open(path))
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def valu_roofline(kernel, dtype, edge_slot_iters, kernel_ms, compulsory_bytes, profile):
+    """Roofline of one decode launch: achieved algorithmic lane-operations
+    (ALG_OPS_PER_EDGE_SLOT x executed edge-slot-iterations) over the
+    launch's average duration against the VALU peak, which is the binding
+    roof; beside it the VALU issue share and VALU instructions per
+    edge-slot-iteration that the SQ counters of profiles/<profile>.md read,
+    and HBM: the compulsory bytes (input, posterior, hard decisions) over the
+    same time, with the counter traffic of that profile."""
+    ops = ALG_OPS_PER_EDGE_SLOT[dtype] * float(edge_slot_iters)
+    t = kernel_ms * 1e-3
+    tops = ops / t / 1e12
+    traffic = (_profile_json(profile, "traffic") or {}).get("hbm_bytes_per_launch")
+    sq = _profile_json(profile, "sq") or {}
+    hbm = compulsory_bytes / t / 1e9
+    return {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tlane-op/s",
+            "frac": round(tops / VALU_PEAK_TOPS, 4), "traffic": traffic, "kernel": kernel,
+            "kernel_ms": round(kernel_ms, 4), "alg_ops_per_edge_slot_iteration": ALG_OPS_PER_EDGE_SLOT[dtype],
+            "edge_slot_iterations_per_launch": float(edge_slot_iters),
+            "valu_issue_frac_profiled": sq.get("valu_issue_frac"),
+            "valu_per_edge_slot_iteration_profiled": sq.get("valu_per_edge_slot_iteration"),
+            "hbm": {"compulsory_bytes_per_launch": float(compulsory_bytes), "achieved": round(hbm, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm / HBM_PEAK_GBS, 4),
+                    "traffic_achieved": round(traffic / t / 1e9, 1) if traffic else None},
+            "profile": f"profiles/{profile}.md"}
+
+
+def _launch_key(r):
+    """(grid work-items, workgroup size) of a rocprofv3 CSV row: two launches
+    of one kernel can share a grid size with different workgroups."""
+    g = r["Grid_Size_X"] if "Grid_Size_X" in r else r.get("Grid_Size", 0)
+    w = r["Workgroup_Size_X"] if "Workgroup_Size_X" in r else r.get("Workgroup_Size", 0)
+    return int(g), int(w)
 
 
 def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_substr, cmd_trace, cmd_pmc):
@@ -127,8 +254,11 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
     committed text summary and the traffic JSON the roofline reads.
 
     HBM bytes per launch = FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of
-    the decode kernel, each scaled by what the same counter reads for the
-    device copy of the input batch (known bytes) in the same run."""
+    the profiled kernel, each scaled by what the same counter reads for the
+    calibration copy of known bytes in the same run (the run's largest
+    device copy).  A workload whose step launches the kernel at several grid
+    sizes (a plan's two launches) also gets per-step figures: the sums over
+    its grids of the per-grid averages."""
     import csv
     from collections import defaultdict
 
@@ -139,56 +269,64 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
     by_grid = defaultdict(list)
     for r in tr:
         if kernel_substr in r["Kernel_Name"]:
-            by_grid[int(r["Grid_Size_X"]) if "Grid_Size_X" in r else int(r.get("Grid_Size", 0))].append(
-                (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+            by_grid[_launch_key(r)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
     stats = rows(trace_dir, "run_kernel_stats.csv")
     counters = {}
     for nm, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
-        ker, cpy = [], []
+        ker, cpy = defaultdict(list), []
         for r in rows(d, "run_counter_collection.csv"):
             v = float(r["Counter_Value"])
-            if kernel_substr in r["Kernel_Name"] and int(r["Grid_Size"]) == max(by_grid):
-                ker.append(v)
+            if kernel_substr in r["Kernel_Name"]:
+                ker[_launch_key(r)].append(v)
             elif "copyBuffer" in r["Kernel_Name"]:
                 cpy.append((int(r["Grid_Size"]), v))
-        # the calibration copy is the largest device copy of the run
         big = max(g for g, _ in cpy)
-        counters[nm] = (float(np.median(ker)), float(np.median([v for g, v in cpy if g == big])))
-    calib = json.loads(open(os.path.join(fetch_dir, "..", "bench_copycal.json")).read())["copy_calibration_bytes"] \
-        if os.path.exists(os.path.join(fetch_dir, "..", "bench_copycal.json")) else None
+        counters[nm] = ({g: float(np.median(v)) for g, v in ker.items()},
+                        float(np.median([v for g, v in cpy if g == big])))
+    calib_path = os.path.join(fetch_dir, "..", "bench_copycal.json")
+    calib = json.loads(open(calib_path).read())["copy_calibration_bytes"] if os.path.exists(calib_path) else None
     lines = [f"# rocprofv3 summary: {kernel_substr}", "", "Kernel trace command:", "", f"    {cmd_trace}", "",
              "Counter commands (one counter per run):", "", f"    {cmd_pmc}", "", "## kernel trace (stats CSV, all dispatches)", "",
              "| kernel | calls | average ns | min ns | max ns |", "|---|---|---|---|---|"]
     for r in stats[:6]:
         lines.append(f"| {r['Name'][:70]} | {r['Calls']} | {float(r['AverageNs']):.0f} | {r['MinNs']} | {r['MaxNs']} |")
-    lines += ["", "## profiled kernel by grid size (trace CSV)", "", "| grid (work-items) | launches | average ms |",
-              "|---|---|---|"]
+    lines += ["", "## profiled kernel by launch shape (trace CSV)", "",
+              "| grid (work-items) | workgroup | launches | average ms |", "|---|---|---|---|"]
     for g, v in sorted(by_grid.items()):
-        lines.append(f"| {g} | {len(v)} | {np.mean(v):.4f} |")
-    out = {"kernel": kernel_substr, "grid_work_items": max(by_grid),
-           "average_ms_full_batch": float(np.mean(by_grid[max(by_grid)]))}
+        lines.append(f"| {g[0]} | {g[1]} | {len(v)} | {np.mean(v):.4f} |")
+    top = max(by_grid)
+    out = {"kernel": kernel_substr, "grid_work_items": top[0], "workgroup": top[1],
+           "average_ms_full_batch": float(np.mean(by_grid[top])),
+           "average_ms_per_step_all_grids": float(sum(np.mean(v) for v in by_grid.values()))}
+    if len(by_grid) > 1:
+        lines += ["", f"One step launches every shape above once: {out['average_ms_per_step_all_grids']:.4f} ms per "
+                      f"step in all."]
     if calib:
         fk, fc = counters["FETCH_SIZE"]
         wk, wc = counters["WRITE_SIZE"]
         f_scale = calib / (fc * 1024.0)
         w_scale = calib / (wc * 1024.0)
-        fetch_b = fk * 1024.0 * f_scale
-        write_b = wk * 1024.0 * w_scale
-        out.update({"fetch_kib_raw": fk, "write_kib_raw": wk, "copy_fetch_kib_raw": fc, "copy_write_kib_raw": wc,
-                    "copy_bytes": calib, "fetch_scale": f_scale, "write_scale": w_scale,
+        fetch_b = fk[top] * 1024.0 * f_scale
+        write_b = wk[top] * 1024.0 * w_scale
+        step_b = sum(fk.values()) * 1024.0 * f_scale + sum(wk.values()) * 1024.0 * w_scale
+        out.update({"fetch_kib_raw": fk[top], "write_kib_raw": wk[top], "copy_fetch_kib_raw": fc,
+                    "copy_write_kib_raw": wc, "copy_bytes": calib, "fetch_scale": f_scale, "write_scale": w_scale,
                     "hbm_fetch_bytes_per_launch": fetch_b, "hbm_write_bytes_per_launch": write_b,
-                    "hbm_bytes_per_launch": fetch_b + write_b})
-        lines += ["", "## HBM traffic per full-batch launch (--pmc)", "",
-                  f"FETCH_SIZE {fk:.0f} KiB raw; the calibration copy ({calib} B) reads {fc:.0f} KiB raw -> scale {f_scale:.4f}",
-                  f"WRITE_SIZE {wk:.0f} KiB raw; the calibration copy writes {wc:.0f} KiB raw -> scale {w_scale:.4f}",
-                  f"=> {fetch_b / 1e6:.1f} MB fetched + {write_b / 1e6:.1f} MB written = "
-                  f"{(fetch_b + write_b) / 1e6:.1f} MB per launch"]
+                    "hbm_bytes_per_launch": fetch_b + write_b if len(by_grid) == 1 else step_b,
+                    "hbm_bytes_full_batch_launch": fetch_b + write_b, "hbm_bytes_per_step_all_grids": step_b})
+        lines += ["", "## HBM traffic (--pmc)", "",
+                  f"The calibration copy ({calib} B) reads FETCH_SIZE {fc:.0f} KiB raw -> scale {f_scale:.4f}, "
+                  f"and WRITE_SIZE {wc:.0f} KiB raw -> scale {w_scale:.4f}.", "",
+                  "| grid (work-items) | workgroup | FETCH_SIZE KiB raw | WRITE_SIZE KiB raw | HBM MB (scaled) |",
+                  "|---|---|---|---|---|"]
+        for g in sorted(fk):
+            mb = (fk[g] * f_scale + wk.get(g, 0.0) * w_scale) * 1024.0 / 1e6
+            lines.append(f"| {g[0]} | {g[1]} | {fk[g]:.0f} | {wk.get(g, 0.0):.0f} | {mb:.1f} |")
+        lines += ["", f"=> {out['hbm_bytes_per_launch'] / 1e6:.1f} MB of HBM traffic per step."]
     os.makedirs(os.path.dirname(out_prefix), exist_ok=True)
     open(out_prefix + ".md", "w").write("\n".join(lines) + "\n")
     open(out_prefix + "_traffic.json", "w").write(json.dumps(out, indent=1) + "\n")
     print("\n".join(lines))
-
-
 
 
 # ---------------------------------------------------------------------------
@@ -239,18 +377,28 @@ def bench_int8(torch, args):
     q = QuantParams(8.0, 32767)
     flags = L.INIT_5G | L.WRITE_POST | L.WRITE_HARD | L.QUANTIZE
     fast = lambda: dec.decode_batch_i8(d_in, d_post, d_hard, d_it, args.iters, flags=flags, quant=q)
-    t = _timed(fast, 10, torch)
+    wall, per = time_launches(fast, 10, 2, torch)
+    t = wall / 10
     hard = d_hard.cpu().numpy()
     bler = float(((hard[:, :k] != info.reshape(B, k)).any(axis=1)).mean())
     its = d_it.cpu().numpy()
     d_edge = torch.empty((B, dec.required_edge_buffer()), dtype=torch.int8, device="cuda")
     plain = lambda: dec.decode_batch_i8(d_in, d_post, d_hard, d_it, args.iters, flags=flags | L.PLAIN,
                                         d_edge=d_edge, quant=q)
-    tp = _timed(plain, 2, torch)
-    return {"workload": "BG2 Z=208 rate 1/5, Eb/N0 U[0,3] dB, int8 (scale 8, fused quantizer), max 8 it, "
-                        "batch 16,384", "value": round(B * k / t / 1e9, 4), "unit": "Gbit/s",
-            "ms_per_launch": round(t * 1e3, 4), "plain_value": round(B * k / tp / 1e9, 4),
-            "mean_iterations": round(float(its.mean()), 3), "bler": bler}
+    tp = _timed(plain, 2, torch) if not args.no_plain else float("nan")
+    out = {"workload": "BG2 Z=208 rate 1/5, Eb/N0 U[0,3] dB, int8 (scale 8, fused quantizer), max 8 it, "
+                       "batch 16,384", "value": round(B * k / t / 1e9, 4), "unit": "Gbit/s",
+           "ms_per_launch": round(t * 1e3, 4), "plain_value": round(B * k / tp / 1e9, 4),
+           "mean_iterations": round(float(its.mean()), 3), "bler": bler,
+           # compulsory bytes per codeword: f32 input, i16 posterior, u8 hard decisions
+           "roofline": valu_roofline("syn::fast_i8<1, 208> (BG2, Z=208)", "i8", float(its.sum()) * 197 * z,
+                                     float(np.mean(per)), float(B) * n * 7, "fast_i8_bg2_z208_b16384")}
+    if not args.no_cpu:
+        from oracle import orc
+        code = orc.code_3gpp(1, z)
+        out["cpu_baseline"] = cpu_pipeline_baseline([code], np.zeros(B, np.int32), np.arange(B, dtype=np.int64) * n,
+                                                    llr.ravel(), np.full(B, k), args.iters, args.beta, int8=True)
+    return out
 
 
 def calibrate_ebno(torch, args, blocks=2048, steps=8):
@@ -321,10 +469,11 @@ def bench_uneven(torch, args):
     d_it = torch.empty(B, dtype=torch.int32, device="cuda")
     flags = L.INIT_5G | L.WRITE_POST | L.WRITE_HARD
     plan.dematch(d_e, d_llr)
-    block = plan.tune(d_llr, d_post, d_hard, d_it, args.iters, flags)
+    block = plan.tune(d_llr, d_post, d_hard, d_it, args.iters, flags) if not args.no_tune else plan.shape()["block"]
     full = lambda: (plan.dematch(d_e, d_llr), plan.run(d_llr, d_post, d_hard, d_it, args.iters, flags))
     t = _timed(full, 5, torch)
-    t_dec = _timed(lambda: plan.run(d_llr, d_post, d_hard, d_it, args.iters, flags), 5, torch)
+    _, per_dec = time_launches(lambda: plan.run(d_llr, d_post, d_hard, d_it, args.iters, flags), 5, 1, torch)
+    t_dec = float(np.mean(per_dec)) / 1e3
     # the de-match kernel alone (HIP events on the launch stream); its
     # algorithmic bytes: E f32 read + N f32 written per block
     _, per_dm = time_launches(lambda: plan.dematch(d_e, d_llr), 20, 2, torch)
@@ -348,17 +497,32 @@ def bench_uneven(torch, args):
     def per_config():
         for dec, x, p, h, it in per:
             dec.decode_batch_f32(x, p, h, it, args.iters, flags=flags)
-    t_per = _timed(per_config, 3, torch)
+    t_per = _timed(per_config, 3, torch) if not args.no_plain else float("nan")
     # int8 variant on the same blocks (quantizer fused)
     plan8 = DecodePlan(bg, z, np.zeros(B), offs, args.beta, int8=True)
     d_post8 = torch.empty(int(n.sum()), dtype=torch.int16, device="cuda")
     q = QuantParams(8.0, 32767)
     f8 = flags | L.QUANTIZE
-    plan8.tune(d_llr, d_post8, d_hard, d_it, args.iters, f8, quant=q)
+    if not args.no_tune:
+        plan8.tune(d_llr, d_post8, d_hard, d_it, args.iters, f8, quant=q)
     t8 = _timed(lambda: plan8.run(d_llr, d_post8, d_hard, d_it, args.iters, f8, quant=q), 5, torch)
     bits = float(k.sum())
+    edges = np.where(bg == 0, 316, 197) * z
+    roof = valu_roofline("syn::persist_decode<false> (both launches of the plan)", "f32",
+                         float((edges * its).sum()), t_dec * 1e3, float(n.sum()) * 9, "persist_f32_uneven_b32768")
+    cpu = cpu8 = None
+    if not args.no_cpu:
+        from oracle import orc
+        keys = sorted({(int(b), int(zz)) for b, zz in zip(bg, z)})
+        kidx = {kk: i for i, kk in enumerate(keys)}
+        codes = [orc.code_3gpp(b, zz) for b, zz in keys]
+        fcode = np.array([kidx[(int(b), int(zz))] for b, zz in zip(bg, z)], np.int32)
+        host = d_llr.cpu().numpy()
+        cpu = cpu_pipeline_baseline(codes, fcode, offs, host, k, args.iters, args.beta)
+        cpu8 = cpu_pipeline_baseline(codes, fcode, offs, host, k, args.iters, args.beta, int8=True)
     return {"workload": "32,768 code blocks, BG1/BG2, Z over the 51 sizes, rates 1/3..8/9 via rate matching, "
                         "Eb/N0 at the measured 10% BLER point +-1 dB, max 8 it",
+            "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_int8": cpu8,
             "value": round(bits / t / 1e9, 4), "unit": "Gbit/s", "ms_dematch_plus_decode": round(t * 1e3, 3),
             "decode_only_value": round(bits / t_dec / 1e9, 4), "per_config_launches_value": round(bits / t_per / 1e9, 4),
             "per_config_launches": len(groups), "int8_value": round(bits / t8 / 1e9, 4), "tuned_block": block,
@@ -440,6 +604,18 @@ def bench_serving(torch, args, peak_ref_gbps):
     def pct(x, q):
         return round(float(np.percentile(x, q)) / 1e6, 4)
 
+    if not args.no_cpu:
+        # the same code blocks through the CPU pipeline: its saturated
+        # throughput (a latency sweep at the GPU's offered loads would only
+        # measure an unbounded queue: they are far above this peak)
+        from oracle import orc
+        keys = sorted({(int(b), int(zz)) for b, zz in zip(cb_bg, cb_z)})
+        kidx = {kk: i for i, kk in enumerate(keys)}
+        codes = [orc.code_3gpp(b, zz) for b, zz in keys]
+        fcode = np.array([kidx[(int(b), int(zz))] for b, zz in zip(cb_bg, cb_z)], np.int32)
+        out["cpu_baseline"] = cpu_pipeline_baseline(codes, fcode, offs, host_pool.array, info_bits, args.iters,
+                                                    args.beta, n_filler=cb_nf, budget_s=6.0)
+
     for mode, streams, pools in (("resident", 0, (("inputs_in_hbm", d_llr), ("inputs_in_pinned_host", host_pool))),
                                  ("lanes", 3, (("inputs_in_hbm", d_llr),))):
         srv = Server(args.beta, args.iters, hbm_budget_bytes=1 << 30, max_batch=1024, streams=streams)
@@ -447,12 +623,12 @@ def bench_serving(torch, args, peak_ref_gbps):
             run = lambda rate, dur: serve_loadgen(srv, pool, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
                                                   rate, dur, seed=args.seed)
             run(0, 0.5)  # warm
-            cbl, tbl, peak_bps, _ = run(0, 2.0)
+            cbl, tbl, peak_bps, _ = run(0, args.serve_seconds)
             peak_tb_rate = peak_bps / mean_tb_bits
-            res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_cb_per_s": round(len(cbl) / 2.0, 1),
+            res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_cb_per_s": round(len(cbl) / args.serve_seconds, 1),
                    "loads": []}
             for load in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0):
-                cbl, tbl, bps, waits = run(load * peak_tb_rate, 2.0)
+                cbl, tbl, bps, waits = run(load * peak_tb_rate, args.serve_seconds)
                 res["loads"].append({"offered_fraction_of_peak": load, "achieved_info_gbps": round(bps / 1e9, 4),
                                      "cb_latency_ms_p50": pct(cbl, 50), "cb_latency_ms_p99": pct(cbl, 99),
                                      "tb_latency_ms_p50": pct(tbl, 50), "tb_latency_ms_p99": pct(tbl, 99),
@@ -466,12 +642,12 @@ def bench_serving(torch, args, peak_ref_gbps):
                                                       tr["tb_size"], tr["d_tb_out"], tr["tb_out_off"], rate, dur,
                                                       seed=args.seed)
             runt(0, 0.5)
-            tbl, ncb, peak_bps, fails, _ = runt(0, 2.0)
+            tbl, ncb, peak_bps, fails, _ = runt(0, args.serve_seconds)
             peak_tb_rate = peak_bps / mean_tb_bits
-            res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_tb_per_s": round(len(tbl) / 2.0, 1),
+            res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_tb_per_s": round(len(tbl) / args.serve_seconds, 1),
                    "crc_failures_at_peak": int(fails), "loads": []}
             for load in (0.1, 0.3, 0.5, 0.7, 0.9):
-                tbl, ncb, bps, fails, waits = runt(load * peak_tb_rate, 2.0)
+                tbl, ncb, bps, fails, waits = runt(load * peak_tb_rate, args.serve_seconds)
                 res["loads"].append({"offered_fraction_of_peak": load, "achieved_info_gbps": round(bps / 1e9, 4),
                                      "tb_latency_ms_p50": pct(tbl, 50), "tb_latency_ms_p99": pct(tbl, 99),
                                      "transport_blocks": int(len(tbl)), "code_blocks": int(ncb),
@@ -481,7 +657,7 @@ def bench_serving(torch, args, peak_ref_gbps):
             # the same traffic offered at 70% of the batched BG1 Z=384
             # decoder's peak (measured by the headline run)
             cbl, tbl, bps, waits = serve_loadgen(srv, d_llr, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
-                                                 0.7 * peak_ref_gbps * 1e9 / mean_tb_bits, 2.0, seed=args.seed)
+                                                 0.7 * peak_ref_gbps * 1e9 / mean_tb_bits, args.serve_seconds, seed=args.seed)
             out["resident_inputs_in_hbm_at_70pct_of_batched_bg1_peak"] = {
                 "offered_info_gbps": round(0.7 * peak_ref_gbps, 4), "achieved_info_gbps": round(bps / 1e9, 4),
                 "cb_latency_ms_p50": pct(cbl, 50), "cb_latency_ms_p99": pct(cbl, 99),
@@ -538,14 +714,7 @@ def bench_encode(torch, args):
     d_cw = torch.empty((B, n), dtype=torch.uint8, device="cuda")
     run = lambda: dec.encode_batch_device(d_info, d_cw)
     if getattr(args, "copy_calibrate", False):  # known bytes to scale HBM counters
-        d_copy = torch.empty_like(d_cw)
-        for _ in range(3):
-            d_copy.copy_(d_cw)
-        torch.cuda.synchronize()
-        del d_copy
-        cal = os.environ.get("SYNDROME_COPYCAL_OUT")
-        if cal:
-            open(cal, "w").write(json.dumps({"copy_calibration_bytes": int(d_cw.numel())}))
+        copy_calibrate(torch, int(d_cw.numel()))
     _, per = time_launches(run, 50, 5, torch)
     t = float(np.mean(per)) / 1e3
     sample = 256
@@ -589,52 +758,80 @@ def extra_benches(torch, args, peak_gbps):
             ("encode_bg1_z384", lambda: bench_encode(torch, args)))
 
 
-def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, waves_per_cw, edges, iters, clock_ghz=2.4, cus=256,
-                 kernel_ms=None, lds_bytes=4):
+def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, edge_slot_iters, clock_ghz=2.4, cus=256, kernel_ms=None,
+                 lds_bytes=4, grid=None):
     """Append the SQ counters of the profiled kernel's full-batch dispatches
-    (mean over them) to a profile summary, with the VALU issue share they
-    imply: VALU instructions x 4 cycles over cus x 4 SIMDs x kernel time.
-    The kernel time is the summary's traffic JSON's, or kernel_ms when given;
-    lds_bytes is what each LDS instruction moves per lane."""
+    (the largest grid, or `grid`; mean over them) to a profile summary, with
+    what they imply, and write them to <summary>_sq.json for bench.py:
+      * VALU issue share = VALU instructions x 4 cycles over cus x 4 SIMDs x
+        the kernel time (the traffic JSON's average, or kernel_ms);
+      * VALU lane-instructions per edge-slot-iteration = VALU instructions x
+        64 lanes over the launch's executed edge-slot-iterations
+        (edge_slot_iters: (edge, z) pairs x layered passes, summed over its
+        codewords), comparable with the algorithmic count in bench.py;
+      * LDS throughput, every LDS instruction moving lds_bytes per lane."""
     import csv
     from collections import defaultdict
     vals = defaultdict(list)
+    per_grid = defaultdict(lambda: defaultdict(list))
+    durations = defaultdict(dict)  # launch shape -> dispatch -> ms, from the counter runs' own timestamps
     for d in sq_dirs:
         rows = list(csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
         ker = [r for r in rows if kernel_substr in r["Kernel_Name"]]
-        big = max(int(r["Grid_Size"]) for r in ker)
+        big = max(_launch_key(r) for r in ker)
         for r in ker:
-            if int(r["Grid_Size"]) == big:
+            key = _launch_key(r)
+            per_grid[r["Counter_Name"]][key].append(float(r["Counter_Value"]))
+            if "Start_Timestamp" in r:
+                durations[key][(d, r["Dispatch_Id"])] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+            if key == big:
                 vals[r["Counter_Name"]].append(float(r["Counter_Value"]))
-    mean = {k: float(np.mean(v)) for k, v in vals.items()}
+    if grid == "all":  # one step = one launch of every grid: sum the per-grid means
+        mean = {k: float(sum(np.mean(v) for v in g.values())) for k, g in per_grid.items()}
+    else:
+        mean = {k: float(np.mean(v)) for k, v in vals.items()}
     if kernel_ms is None:
-        kernel_ms = json.load(open(out_md[:-3] + "_traffic.json"))["average_ms_full_batch"]
+        tj = json.load(open(out_md[:-3] + "_traffic.json"))
+        kernel_ms = tj["average_ms_per_step_all_grids"] if grid == "all" else tj["average_ms_full_batch"]
+    elif kernel_ms < 0:  # the counter runs' own dispatch durations (mean per dispatch)
+        ds = [v for dd in durations.values() for v in dd.values()]
+        kernel_ms = float(np.mean(ds)) if ds else 0.0
     kern_s = kernel_ms / 1e3
-    waves = mean.get("SQ_WAVES", 0.0)
     valu = mean.get("SQ_INSTS_VALU", 0.0)
-    issue = valu * 4 / (cus * 4 * kern_s * clock_ghz * 1e9)
-    lines = ["", f"## SQ counters of the same kernel (full-batch dispatches, mean of {len(vals.get('SQ_WAVES', []))})",
+    issue = valu * 4 / (cus * 4 * kern_s * clock_ghz * 1e9) if kern_s > 0 else None
+    per_slot = valu * 64 / float(edge_slot_iters) if edge_slot_iters else None
+    lines = ["", f"## SQ counters of the same kernel (full-batch dispatches, mean of "
+                 f"{max((len(v) for v in vals.values()), default=0)})",
              "", "Commands (counters within the per-block limits, one pass each):", ""]
     lines += [f"    {c}" for c in cmds]
     lines += ["", "| counter | value |", "|---|---|"]
     for k in sorted(mean):
-        note = ""
-        if k == "SQ_WAVES" and waves_per_cw:
-            note = f" ({waves / waves_per_cw:,.0f} codewords x {waves_per_cw} waves)"
-        if k == "SQ_INSTS_VALU" and waves:
-            note = (f" ({valu / waves:,.0f} per wave: {valu / waves / iters:,.0f} per wave per iteration at {iters} "
-                    f"mean iterations, {valu / waves / iters / edges:,.1f} per edge)")
-        lines.append(f"| {k} | {mean[k]:,.0f}{note} |")
-    lines += ["", f"VALU issue share: {valu:.3e} VALU instructions x 4 cycles over {cus} CUs x 4 SIMDs and the "
-                  f"{kern_s * 1e3:.3f} ms kernel ({clock_ghz} GHz) = {issue * 100:.0f} % of the device's VALU issue slots."]
+        lines.append(f"| {k} | {mean[k]:,.0f} |")
+    if issue is not None:
+        lines += ["", f"VALU issue share: {valu:.4e} VALU instructions x 4 cycles over {cus} CUs x 4 SIMDs and the "
+                      f"{kern_s * 1e3:.4f} ms kernel ({clock_ghz} GHz) = {issue * 100:.1f} % of the device's VALU "
+                      f"issue slots."]
+    else:
+        lines += [""]
+    if per_slot:
+        lines += [f"VALU lane-instructions per edge-slot-iteration: {valu:.4e} x 64 lanes over "
+                  f"{float(edge_slot_iters):.4e} executed edge-slot-iterations = {per_slot:.2f}."]
+    waits, cyc = mean.get("SQ_WAIT_ANY"), mean.get("SQ_WAVE_CYCLES")
+    if waits and cyc:
+        lines += [f"Waves waiting: SQ_WAIT_ANY / SQ_WAVE_CYCLES = {waits / cyc * 100:.1f} % of wave cycles."]
     lds = mean.get("SQ_INSTS_LDS", 0.0)
-    if lds:
-        lds_tbs = lds * 64 * lds_bytes / kern_s / 1e12  # every LDS instruction moves lds_bytes per lane
+    if lds and kern_s > 0:
+        lds_tbs = lds * 64 * lds_bytes / kern_s / 1e12
         lds_peak = cus * 128 * clock_ghz * 1e9 / 1e12
         lines += [f"LDS throughput: {lds:.3e} LDS instructions x 64 lanes x {lds_bytes} B over the kernel = "
-                  f"{lds_tbs:.1f} TB/s, "
-                  f"{lds_tbs / lds_peak * 100:.0f} % of the {lds_peak:.1f} TB/s the CUs' LDS can serve (128 B per CU per cycle)."]
+                  f"{lds_tbs:.1f} TB/s, {lds_tbs / lds_peak * 100:.0f} % of the {lds_peak:.1f} TB/s the CUs' LDS "
+                  f"can serve (128 B per CU per cycle)."]
     open(out_md, "a").write("\n".join(lines) + "\n")
+    json.dump({"kernel": kernel_substr, "kernel_ms": kernel_ms, "counters": mean,
+               "valu_issue_frac": round(issue, 4) if issue is not None else None,
+               "valu_per_edge_slot_iteration": round(per_slot, 3) if per_slot else None,
+               "edge_slot_iterations": float(edge_slot_iters)},
+              open(out_md[:-3] + "_sq.json", "w"), indent=1)
     print("\n".join(lines))
 
 
@@ -666,23 +863,51 @@ def summarize_trace(trace_dir, out_md, title, cmd, extra_json=None):
     print("\n".join(lines))
 
 
+def copy_calibrate(torch, nbytes=256 << 20):
+    """Device copies of a known size (three of nbytes), so that the HBM
+    counters of a profiled run can be scaled by what they read for known
+    bytes; the size goes to $SYNDROME_COPYCAL_OUT when set."""
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    cal = os.environ.get("SYNDROME_COPYCAL_OUT")
+    if cal:
+        open(cal, "w").write(json.dumps({"copy_calibration_bytes": int(nbytes)}))
+    return int(nbytes)
+
+
+def _number_arg(x):
+    """A number, or @FILE:KEY.PATH naming one inside a JSON line this script
+    printed (the last line of FILE), e.g.
+    @out.json:int8_bg2_z208.roofline.edge_slot_iterations_per_launch."""
+    if not x.startswith("@"):
+        return float(x)
+    path, key = x[1:].split(":", 1)
+    obj = json.loads(open(path).read().strip().splitlines()[-1])
+    for part in key.split("."):
+        obj = obj[part]
+    return float(obj)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-trace":
         # bench.py --summarize-trace TRACE_DIR OUT_MD TITLE CMD [ONLY_JSON]
         summarize_trace(*sys.argv[2:7])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-sq":
-        # bench.py --summarize-sq [--kernel-ms MS] [--lds-bytes B] OUT_MD KERNEL WAVES_PER_CW EDGES MEAN_ITERS
+        # bench.py --summarize-sq [--kernel-ms MS] [--lds-bytes B] [--grid G] OUT_MD KERNEL EDGE_SLOT_ITERS
         #          SQ_DIR1 CMD1 [SQ_DIR2 CMD2 ...]
         a = sys.argv[2:]
         opt = {}
-        while a and a[0] in ("--kernel-ms", "--lds-bytes"):
+        while a and a[0] in ("--kernel-ms", "--lds-bytes", "--grid"):
             opt[a[0]] = a[1]
             a = a[2:]
-        rest = a[5:]
-        summarize_sq(rest[0::2], a[0], a[1], rest[1::2], int(a[2]), int(a[3]), float(a[4]),
-                     kernel_ms=float(opt["--kernel-ms"]) if "--kernel-ms" in opt else None,
-                     lds_bytes=int(opt.get("--lds-bytes", 4)))
+        rest = a[3:]
+        summarize_sq(rest[0::2], a[0], a[1], rest[1::2], _number_arg(a[2]),
+                     kernel_ms=_number_arg(opt["--kernel-ms"]) if "--kernel-ms" in opt else None,
+                     lds_bytes=int(opt.get("--lds-bytes", 4)), grid=opt.get("--grid"))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-profiles":
         # bench.py --summarize-profiles TRACE_DIR FETCH_DIR WRITE_DIR OUT_PREFIX KERNEL CMD_TRACE CMD_PMC
@@ -699,7 +924,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain and per-config comparison decoders")
+    ap.add_argument("--no-tune", action="store_true", help="plans keep their default launch shape (profiling)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="duration of each serving load point")
     ap.add_argument("--quick", action="store_true", help="headline only (no int8 / uneven / serving / Wi-Fi extras)")
     ap.add_argument("--calibrate-ebno", action="store_true",
                     help="measure the 10%% BLER Eb/N0 of each (base graph, rate) of the uneven workload and exit")
@@ -727,6 +954,8 @@ def main():
         fn = dict(extra_benches(torch, args, None)).get(args.only)
         if fn is None:
             sys.exit(f"unknown extra {args.only!r}")
+        if args.copy_calibrate and args.only != "encode_bg1_z384":
+            copy_calibrate(torch)
         print(json.dumps({args.only: fn()}), flush=True)
         return
     dev = QcLdpcDecoder.with_lifting_size(syndrome_amd.BG1, 384, args.beta)
@@ -757,35 +986,16 @@ def main():
     mean_it = float(its.mean())
     value = B * k / (ms * 1e-3) / 1e9
 
-    # roofline of the dominant kernel (fast_f32<BG1>), algorithmic bytes
-    # per codeword: N*4 (input) + N*4 (posterior) + N*1 (hard) + t*2*E*Z*4 (R read+write per iteration)
-    alg_bytes = float(B * (n * 4 + n * 4 + n) + int(its.sum()) * 2 * ez * 4)
+    # roofline of the dominant kernel, fast_f32<BG1, 384>: VALU-bound (the
+    # check-to-variable messages stay in registers, so HBM sees only the
+    # compulsory N*4 input + N*4 posterior + N*1 hard-decision bytes)
     kern_ms = float(per.mean())
-    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_PROFILE):
-        try:
-            traffic = json.load(open(TRAFFIC_PROFILE)).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "syn::fast_f32<0, 384> (BG1, Z=384)", "kernel_ms": round(kern_ms, 4),
-            "algorithmic_bytes_per_launch": alg_bytes,
-            "measured_hbm_gbs": round(traffic / (kern_ms * 1e-3) / 1e9, 1) if traffic else None,
-            "profile": "profiles/fast_f32_bg1_z384_b4096.md"}
+    roof = valu_roofline("syn::fast_f32<0, 384> (BG1, Z=384)", "f32", float(its.sum()) * ez, kern_ms,
+                         float(B) * n * 9, "fast_f32_bg1_z384_b4096")
 
     out = {}
     if args.copy_calibrate:
-        d_copy = torch.empty_like(d_in)
-        for _ in range(3):
-            d_copy.copy_(d_in)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = int(d_in.numel() * 4)
-        del d_copy
-        cal = os.environ.get("SYNDROME_COPYCAL_OUT")
-        if cal:
-            open(cal, "w").write(json.dumps({"copy_calibration_bytes": out["copy_calibration_bytes"]}))
+        out["copy_calibration_bytes"] = copy_calibrate(torch, int(d_in.numel() * 4))
     if not args.no_plain:
         log("[bench] timing the plain per-config decoder")
         d_edge = torch.empty((B, ez), dtype=torch.float32, device="cuda")

@@ -206,7 +206,14 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *plan);
 /* The plan's persistent launch (two when its largest Z exceeds 256: the
  * codes of Z <= 256 get a second launch of 256-lane workgroups).  int8
  * plans take i16 (or f32 with SYNDROME_BATCH_QUANTIZE) inputs and write i16
- * posteriors. */
+ * posteriors.  Each call takes the next of 16 queue counters, so up to 16
+ * runs of one plan may be in flight at once (on different streams, or a
+ * graph replay beside eager runs); a 17th issued while the first still runs
+ * would share its counter.  A HIP graph captured from plan_run keeps the
+ * counter of its capture: replay one graph on one stream at a time.
+ * tune, set_dematch and set_tbs replace the plan's device tables but keep
+ * the old ones until plan_destroy, so a graph captured before them still
+ * replays, with the launch shape and tables it was captured with. */
 int32_t syndrome_ldpc_plan_run(SyndromeLdpcPlan *plan, const void *d_in, void *d_post, uint8_t *d_hard,
                                int32_t *d_iters, size_t iterations, float scale, int16_t app_clamp, uint32_t flags,
                                void *stream);

@@ -259,6 +259,13 @@ def cpb():
         L.cpb_pipeline_run.argtypes = [C.POINTER(Code), C.c_float, C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_int,
                                        C.c_long, C.c_void_p, C.c_void_p]
         L.cpb_pipeline_run.restype = C.c_longlong
+        L.cpb_decode_i8_avx2.argtypes = [C.POINTER(Code), C.c_int8, C.c_int16, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_long]
+        L.cpb_decode_i8_avx2.restype = C.c_long
+        L.cpb_pipeline_mixed.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                         C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int8, C.c_int16, C.c_long,
+                                         C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.cpb_pipeline_mixed.restype = C.c_longlong
         _cpb = L
     return _cpb
 
@@ -279,6 +286,37 @@ def pipeline_run(code, beta, llr_batch, iterations, workers, init5g=True, n_fill
     hard = np.zeros((count, code.n), np.uint8)
     ns = cpb().cpb_pipeline_run(C.byref(code), beta, _p(llr_batch), count, iterations, workers, int(init5g),
                                 n_filler, _p(iters), _p(hard))
+    return ns * 1e-9, iters, hard
+
+
+def avx2_decode_i8(code, beta_q_value, app, iterations, app_clamp=32767):
+    """The AVX2 int8 restatement: returns (iters, app, edge_r, hard)."""
+    app = np.array(app, dtype=np.int16, copy=True)
+    er = np.zeros(code.edge_buf, np.int8)
+    hard = np.zeros(code.n, np.uint8)
+    it = cpb().cpb_decode_i8_avx2(C.byref(code), beta_q_value, app_clamp, _p(app), _p(er), _p(hard), iterations)
+    if it < 0:
+        raise ValueError("AVX2 int8 decode refused the code")
+    return it, app, er, hard
+
+
+def pipeline_mixed(codes, frame_code, in_off, llr_arena, iterations, workers, order=None, int8=False, beta=0.5,
+                   scale=8.0, app_clamp=32767, init5g=True, n_filler=None):
+    """A stream through the CPU pipeline over mixed codes (cpb_pipeline_mixed).
+    codes: list of Code; frame f has code codes[frame_code[f]] and its N_f
+    LLRs at llr_arena[in_off[f]:]; order: the frames in stream order
+    (default each once).  Returns (seconds, iters[frames], hard arena)."""
+    arr = (Code * len(codes))(*codes)
+    fc = np.ascontiguousarray(frame_code, np.int32)
+    off = np.ascontiguousarray(in_off, np.int64)
+    nf = None if n_filler is None else np.ascontiguousarray(n_filler, np.int64)
+    order = np.arange(fc.size, dtype=np.int32) if order is None else np.ascontiguousarray(order, np.int32)
+    x = np.ascontiguousarray(llr_arena, np.float32)
+    iters = np.zeros(fc.size, np.int32)
+    hard = np.zeros(x.size, np.uint8)
+    ns = cpb().cpb_pipeline_mixed(arr, _p(fc), _p(off), None if nf is None else _p(nf), _p(order), order.size,
+                                  _p(x), int(bool(int8)), beta, scale, beta_q(beta, scale) if int8 else 0,
+                                  app_clamp, iterations, workers, int(bool(init5g)), _p(iters), _p(hard))
     return ns * 1e-9, iters, hard
 
 

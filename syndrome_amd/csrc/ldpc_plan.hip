@@ -60,6 +60,13 @@ struct SyndromeLdpcPlan {
     int n_dm_chunks = 0;
     SynTb *d_tbs = nullptr;
     int n_tbs = 0;
+    // queue counters: RUN_RING runs, two launches each, one 128-byte line
+    // per counter; run k uses ring entry k % RUN_RING
+    unsigned run_seq = 0;
+    // device tables replaced by tune / set_dematch / set_tbs: kept until the
+    // plan is destroyed, so a HIP graph captured before the change still
+    // replays valid pointers (with the launch shape it was captured with)
+    std::vector<void *> retired;
     std::mutex mu;
 };
 
@@ -90,9 +97,9 @@ struct DeviceScope {
 };
 
 template <class T>
-int32_t upload(T **dst, const std::vector<T> &src)
+int32_t upload(SyndromeLdpcPlan *p, T **dst, const std::vector<T> &src)
 {
-    if (*dst) (void)hipFree(*dst);
+    if (*dst) p->retired.push_back((void *)*dst);
     *dst = nullptr;
     if (src.empty()) return SYNDROME_OK;
     if (hipMalloc((void **)dst, sizeof(T) * src.size()) != hipSuccess) return SYNDROME_DEVICE_ERROR;
@@ -101,10 +108,12 @@ int32_t upload(T **dst, const std::vector<T> &src)
     return SYNDROME_OK;
 }
 
-// Lanes of the small codes' workgroups in a split plan, and the spacing
-// (ints, one 128-byte line) of the two launches' queue counters.
+// Lanes of the small codes' workgroups in a split plan, the spacing (ints,
+// one 128-byte line) of the queue counters, and how many runs of one plan
+// may be in flight at once, each with its own pair of counters.
 constexpr int SPLIT_LANES = 256;
 constexpr int HEAD_STRIDE = 32;
+constexpr unsigned RUN_RING = 16;
 
 size_t group_bytes(const SynCode &c, int cws, bool i8)
 {
@@ -163,9 +172,9 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         std::stable_sort(items[q].begin(), items[q].end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
         for (auto &x : items[q]) flat[q].push_back(x.it);
     }
-    if (int32_t rc = upload(&p->d_items, flat[0])) return rc;
-    if (int32_t rc = upload(&p->d_items_s, flat[1])) return rc;
-    if (int32_t rc = upload(&p->d_ids, ids)) return rc;
+    if (int32_t rc = upload(p, &p->d_items, flat[0])) return rc;
+    if (int32_t rc = upload(p, &p->d_items_s, flat[1])) return rc;
+    if (int32_t rc = upload(p, &p->d_ids, ids)) return rc;
     const void *fn = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32();
     if (syn::allow_max_dynamic_lds(fn)) return SYNDROME_DEVICE_ERROR;
     int per_cu = 0;
@@ -217,12 +226,15 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
     const int *ids = p->d_ids;
     const long long *offs = p->d_offs;
     const int *nf = p->d_nfill;
+    // this run's pair of counters: runs in flight at once (on different
+    // streams, or a graph replay beside an eager run) never share one
+    int *heads = p->d_head + (size_t)(p->run_seq++ % RUN_RING) * 2 * HEAD_STRIDE;
     // the main launch, then the small codes' (each with its own counter)
     for (int q = 0; q < 2; ++q) {
         const SynItem *items = q ? p->d_items_s : p->d_items;
         int n_items = q ? p->n_items_s : p->n_items;
         if (n_items == 0) continue;
-        int *head = p->d_head + q * HEAD_STRIDE;
+        int *head = heads + q * HEAD_STRIDE;
         if (syn::launch_reset(head, stream)) return SYNDROME_DEVICE_ERROR;
         void *args[] = {&codes, &items, &n_items, &head, &ids, &offs, &nf, &a};
         if (hipLaunchKernel(fn, dim3(q ? p->grid_s : p->grid), dim3(q ? p->block_s : p->block), args,
@@ -274,10 +286,11 @@ int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int
             max_z = std::max(max_z, (int)z[i]);
         }
         DeviceScope ds(p->device);
-        if (int32_t rc = upload(&p->d_codes, p->codes)) return rc;
-        if (int32_t rc = upload(&p->d_offs, p->offs)) return rc;
-        if (int32_t rc = upload(&p->d_nfill, p->nfill)) return rc;
-        if (hipMalloc((void **)&p->d_head, 2 * HEAD_STRIDE * sizeof(int)) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        if (int32_t rc = upload(p.get(), &p->d_codes, p->codes)) return rc;
+        if (int32_t rc = upload(p.get(), &p->d_offs, p->offs)) return rc;
+        if (int32_t rc = upload(p.get(), &p->d_nfill, p->nfill)) return rc;
+        if (hipMalloc((void **)&p->d_head, RUN_RING * 2 * HEAD_STRIDE * sizeof(int)) != hipSuccess)
+            return SYNDROME_DEVICE_ERROR;
         const int min_block = ((max_z + 63) / 64) * 64;
         const int b = block > 0 ? block : std::max(min_block, 512);
         if (int32_t rc = build_items(p.get(), std::max(b, min_block))) return rc;
@@ -295,6 +308,7 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *p)
                         (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm, (void *)p->d_dm_chunks,
                         (void *)p->d_tbs})
             if (q) (void)hipFree(q);
+        for (void *q : p->retired) (void)hipFree(q);
     } catch (...) {
     }
     delete p;
@@ -391,8 +405,8 @@ int32_t syndrome_ldpc_plan_set_dematch(SyndromeLdpcPlan *p, const int64_t *e_off
         if (ch.size() > (size_t)INT_MAX) return SYNDROME_INVALID_PARAM;
         std::lock_guard<std::mutex> lk(p->mu);
         DeviceScope ds(p->device);
-        if (int32_t rc = upload(&p->d_dm, v)) return rc;
-        if (int32_t rc = upload(&p->d_dm_chunks, ch)) return rc;
+        if (int32_t rc = upload(p, &p->d_dm, v)) return rc;
+        if (int32_t rc = upload(p, &p->d_dm_chunks, ch)) return rc;
         p->n_dm_chunks = (int)ch.size();
         return SYNDROME_OK;
     });
@@ -435,7 +449,7 @@ int32_t syndrome_ldpc_plan_set_tbs(SyndromeLdpcPlan *p, size_t n_tb, const int32
         }
         std::lock_guard<std::mutex> lk(p->mu);
         DeviceScope ds(p->device);
-        if (int32_t rc = upload(&p->d_tbs, v)) return rc;
+        if (int32_t rc = upload(p, &p->d_tbs, v)) return rc;
         p->n_tbs = (int)n_tb;
         return SYNDROME_OK;
     });

@@ -25,6 +25,74 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _need(t, name, what, dtypes, numel):
+    """A device buffer handed to a kernel: a contiguous torch tensor of one
+    of `dtypes` with at least `numel` elements (residency is checked after
+    every buffer's shape, by _resident).  Anything else raises ValueError
+    here, before a launch could read or write past it."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: {name} must be a torch tensor")
+    if t.dtype not in dtypes:
+        names = " or ".join(str(d).replace("torch.", "") for d in dtypes)
+        raise ValueError(f"{what}: {name} must be {names}, got {str(t.dtype).replace('torch.', '')}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: {name} must be contiguous")
+    if t.numel() < numel:
+        raise ValueError(f"{what}: {name} holds {t.numel()} elements, {numel} needed")
+
+
+def _resident(what, **tensors):
+    for name, t in tensors.items():
+        if t is not None and not t.is_cuda:
+            raise ValueError(f"{what}: {name} must be in device memory (a CUDA tensor)")
+
+
+def check_batch_tensors(what, n, ez, int8, flags, d_in, d_post, d_hard, d_iters, d_edge):
+    """Validate the tensors of a device batch decode of d_iters.numel()
+    codewords of N = n variables and E*Z = ez messages (f32 path, or the
+    int8 path with i16 / f32-QUANTIZE input).  Returns the batch size."""
+    import torch
+    _need(d_iters, "d_iters", what, (torch.int32,), 0)
+    batch = d_iters.numel()
+    if int8:
+        in_types = (torch.float32,) if flags & _lib.QUANTIZE else (torch.int16,)
+        post_t, edge_t = (torch.int16,), (torch.int8,)
+    else:
+        if flags & _lib.QUANTIZE:
+            raise ValueError(f"{what}: QUANTIZE is an int8 flag")
+        in_types, post_t, edge_t = (torch.float32,), (torch.float32,), (torch.float32,)
+    _need(d_in, "d_in", what, in_types, batch * n)
+    if flags & _lib.WRITE_POST:
+        _need(d_post, "d_post", what, post_t, batch * n)
+    if flags & _lib.WRITE_HARD:
+        _need(d_hard, "d_hard", what, (torch.uint8,), batch * n)
+    if d_edge is not None or flags & (_lib.WRITE_EDGE | _lib.PLAIN):
+        _need(d_edge, "d_edge", what, edge_t, batch * ez)
+    _resident(what, d_in=d_in, d_post=d_post, d_hard=d_hard, d_iters=d_iters, d_edge=d_edge)
+    return batch
+
+
+def check_plan_tensors(what, arena, count, int8, flags, d_in, d_post, d_hard, d_iters):
+    """Validate a plan run's arenas: `arena` = max over blocks of offset + N_i
+    elements; iterations for `count` blocks."""
+    import torch
+    if int8:
+        in_types = (torch.float32,) if flags & _lib.QUANTIZE else (torch.int16,)
+        post_t = (torch.int16,)
+    else:
+        if flags & _lib.QUANTIZE:
+            raise ValueError(f"{what}: QUANTIZE is an int8 flag")
+        in_types, post_t = (torch.float32,), (torch.float32,)
+    _need(d_in, "d_in", what, in_types, arena)
+    if flags & _lib.WRITE_POST:
+        _need(d_post, "d_post", what, post_t, arena)
+    if flags & _lib.WRITE_HARD:
+        _need(d_hard, "d_hard", what, (torch.uint8,), arena)
+    _need(d_iters, "d_iters", what, (torch.int32,), count)
+    _resident(what, d_in=d_in, d_post=d_post, d_hard=d_hard, d_iters=d_iters)
+
+
 class QuantParams:
     """QuantParams (src/quantize.rs:217-305)."""
 
@@ -181,7 +249,8 @@ class QcLdpcDecoder:
         import torch
         if flags is None:
             flags = _lib.WRITE_POST | _lib.WRITE_HARD
-        batch = d_iters.numel()
+        batch = check_batch_tensors("decode_batch_f32", self.variable_node_count(), self.required_edge_buffer(),
+                                    False, flags, d_in, d_post, d_hard, d_iters, d_edge)
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_decode_batch_f32(
             self._h, d_in.data_ptr(), d_post.data_ptr() if d_post is not None else None,
@@ -194,7 +263,8 @@ class QcLdpcDecoder:
         q = quant or QuantParams()
         if flags is None:
             flags = _lib.WRITE_POST | _lib.WRITE_HARD
-        batch = d_iters.numel()
+        batch = check_batch_tensors("decode_batch_i8", self.variable_node_count(), self.required_edge_buffer(),
+                                    True, flags, d_in, d_post, d_hard, d_iters, d_edge)
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_decode_batch_i8(
             self._h, d_in.data_ptr(), d_post.data_ptr() if d_post is not None else None,
@@ -268,6 +338,9 @@ class QcLdpcEncoder:
 
 def quantize_llr_i16_device(d_llr, d_out, scale=DEFAULT_SCALE, stream=None):
     import torch
+    _need(d_llr, "d_llr", "quantize_llr_i16_device", (torch.float32,), 0)
+    _need(d_out, "d_out", "quantize_llr_i16_device", (torch.int16,), d_llr.numel())
+    _resident("quantize_llr_i16_device", d_llr=d_llr, d_out=d_out)
     s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
     check(lib().syndrome_ldpc_quantize_i16_device(d_llr.data_ptr(), d_out.data_ptr(), d_llr.numel(), scale, s),
           "quantize")
@@ -292,7 +365,13 @@ class DecodePlan:
         self.z = np.ascontiguousarray(z, np.int32)
         self.n_filler = np.ascontiguousarray(n_filler if n_filler is not None else np.zeros(len(self.bg)), np.int32)
         self.offsets = np.ascontiguousarray(offsets, np.int64)
+        if not (self.bg.ndim == self.z.ndim == self.n_filler.ndim == self.offsets.ndim == 1 and
+                len(self.bg) == len(self.z) == len(self.n_filler) == len(self.offsets)):
+            raise ValueError("DecodePlan: bg, z, n_filler and offsets must be 1-D arrays of one length")
         self.int8 = bool(int8)
+        # elements every arena (input, posterior, hard decisions) must hold
+        self.arena = int((self.offsets + np.where(self.bg == 0, 68, 52).astype(np.int64) * self.z).max()) \
+            if len(self.bg) else 0
         out = C.c_void_p()
         check(lib().syndrome_ldpc_plan_create(_ptr(self.bg), _ptr(self.z), _ptr(self.n_filler), _ptr(self.offsets),
                                               len(self.bg), offset_beta, int(self.int8), block, C.byref(out)),
@@ -321,6 +400,7 @@ class DecodePlan:
     def run(self, d_in, d_post, d_hard, d_iters, iterations, flags, quant=None, stream=None):
         import torch
         q = quant or QuantParams()
+        check_plan_tensors("DecodePlan.run", self.arena, self.count, self.int8, flags, d_in, d_post, d_hard, d_iters)
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_plan_run(self._h, d_in.data_ptr(), d_post.data_ptr() if d_post is not None else None,
                                            d_hard.data_ptr() if d_hard is not None else None, d_iters.data_ptr(),
@@ -328,6 +408,8 @@ class DecodePlan:
 
     def tune(self, d_in, d_post, d_hard, d_iters, iterations, flags, quant=None, reps=2):
         q = quant or QuantParams()
+        check_plan_tensors("DecodePlan.tune", self.arena, self.count, self.int8, flags, d_in, d_post, d_hard,
+                           d_iters)
         best = C.c_int32()
         check(lib().syndrome_ldpc_plan_tune(self._h, d_in.data_ptr(), d_post.data_ptr() if d_post is not None else None,
                                             d_hard.data_ptr() if d_hard is not None else None, d_iters.data_ptr(),
@@ -338,6 +420,10 @@ class DecodePlan:
     def set_dematch(self, e_offsets, e_lengths, qm=None, rv=None):
         self._eo = np.ascontiguousarray(e_offsets, np.int64)
         self._el = np.ascontiguousarray(e_lengths, np.int32)
+        for name, a in (("e_offsets", self._eo), ("e_lengths", self._el), ("qm", qm), ("rv", rv)):
+            if a is not None and np.size(a) != self.count:
+                raise ValueError(f"DecodePlan.set_dematch: {name} must have one entry per block")
+        self._e_arena = int((self._eo + self._el).max()) if self.count else 0
         self._qm = None if qm is None else np.ascontiguousarray(qm, np.int32)
         self._rv = None if rv is None else np.ascontiguousarray(rv, np.int32)
         check(lib().syndrome_ldpc_plan_set_dematch(self._h, _ptr(self._eo), _ptr(self._el),
@@ -346,6 +432,11 @@ class DecodePlan:
 
     def dematch(self, d_e_llr, d_llr_out, stream=None):
         import torch
+        if not hasattr(self, "_e_arena"):
+            raise ValueError("DecodePlan.dematch: set_dematch first")
+        _need(d_e_llr, "d_e_llr", "DecodePlan.dematch", (torch.float32,), self._e_arena)
+        _need(d_llr_out, "d_llr_out", "DecodePlan.dematch", (torch.float32,), self.arena)
+        _resident("DecodePlan.dematch", d_e_llr=d_e_llr, d_llr_out=d_llr_out)
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_plan_dematch(self._h, d_e_llr.data_ptr(), d_llr_out.data_ptr(), s), "plan_dematch")
 
@@ -355,12 +446,23 @@ class DecodePlan:
         self._tn = np.ascontiguousarray(tb_num_cb, np.int32)
         self._ts = np.ascontiguousarray(tb_size, np.int32)
         self._to = np.ascontiguousarray(tb_out_off, np.int64)
+        if not (len(self._tf) == len(self._tn) == len(self._ts) == len(self._to)):
+            raise ValueError("DecodePlan.set_tbs: one entry per transport block in every array")
+        self._tb_arena = int((self._to + self._ts).max()) if len(self._tf) else 0
         check(lib().syndrome_ldpc_plan_set_tbs(self._h, len(self._tf), _ptr(self._tf), _ptr(self._tn), _ptr(self._ts),
                                                _ptr(self._to)), "plan_set_tbs")
 
     def check_tbs(self, d_hard, d_tb_out, d_tb_ok, d_cb_ok, stream=None):
         """CRC-24B per block, reassembly and CRC-24A per transport block."""
         import torch
+        if not hasattr(self, "_tb_arena"):
+            raise ValueError("DecodePlan.check_tbs: set_tbs first")
+        w = "DecodePlan.check_tbs"
+        _need(d_hard, "d_hard", w, (torch.uint8,), self.arena)
+        _need(d_tb_out, "d_tb_out", w, (torch.uint8,), self._tb_arena)
+        _need(d_tb_ok, "d_tb_ok", w, (torch.uint8,), len(self._tf))
+        _need(d_cb_ok, "d_cb_ok", w, (torch.uint8,), self.count)
+        _resident(w, d_hard=d_hard, d_tb_out=d_tb_out, d_tb_ok=d_tb_ok, d_cb_ok=d_cb_ok)
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_plan_check_tbs(self._h, d_hard.data_ptr(), d_tb_out.data_ptr(), d_tb_ok.data_ptr(),
                                                  d_cb_ok.data_ptr(), s), "plan_check_tbs")
@@ -370,6 +472,11 @@ class DecodePlan:
         transmission's de-matched LLRs to the decoder-input slots of d_acc.
         rv 0..3 overrides every block's redundancy version."""
         import torch
+        if not hasattr(self, "_e_arena"):
+            raise ValueError("DecodePlan.harq_combine: set_dematch first")
+        _need(d_e_llr, "d_e_llr", "DecodePlan.harq_combine", (torch.float32,), self._e_arena)
+        _need(d_acc, "d_acc", "DecodePlan.harq_combine", (torch.float32,), self.arena)
+        _resident("DecodePlan.harq_combine", d_e_llr=d_e_llr, d_acc=d_acc)
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(lib().syndrome_ldpc_plan_harq_combine(self._h, d_e_llr.data_ptr(), d_acc.data_ptr(), int(rv), s),
               "plan_harq_combine")

@@ -267,3 +267,31 @@ def test_harq_recovers_blocks_that_fail_alone(dev_ok, oracle):
             assert its[j] == ri and np.array_equal(h, rh), f"shot {shot} block {j}: differs from the oracle"
             ok = np.array_equal(h[:kp], infos[j][:kp])
             assert ok == (shot == 1), f"shot {shot} block {j}: decoded={ok}"
+
+
+def test_plan_runs_in_flight_together_match_one_run(dev_ok, oracle):
+    """Runs of one plan issued back to back on three streams, with no wait
+    between them, overlap on the device; each takes its own queue counter,
+    so each output set equals what one run alone gives."""
+    import syndrome_amd._lib as L
+    from syndrome_amd import DecodePlan
+    say("[plan] three runs in flight on three streams")
+    bg, z, nf, offs, arena, blocks = _mixed_inputs(oracle, 42, 9)
+    plan = DecodePlan(bg, z, nf, offs, 0.5)
+    d_in = torch.from_numpy(arena).cuda()
+    flags = L.INIT_5G | L.WRITE_POST | L.WRITE_HARD
+    ref = (torch.empty_like(d_in), torch.empty(arena.size, dtype=torch.uint8, device="cuda"),
+           torch.empty(len(bg), dtype=torch.int32, device="cuda"))
+    plan.run(d_in, ref[0], ref[1], ref[2], 8, flags)
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    outs = [(torch.full_like(d_in, float("nan")), torch.full((arena.size,), 9, dtype=torch.uint8, device="cuda"),
+             torch.full((len(bg),), -1, dtype=torch.int32, device="cuda")) for _ in streams]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for s, (p, h, it) in zip(streams, outs):
+            plan.run(d_in, p, h, it, 8, flags, stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    for p, h, it in outs:
+        assert torch.equal(it, ref[2]) and torch.equal(h, ref[1])
+        assert torch.equal(p.view(torch.int32), ref[0].view(torch.int32))
