@@ -69,6 +69,26 @@ int32_t syndrome_ldpc_decode_5g(SyndromeLdpcDecoder *decoder, float *llr, size_t
                                 uint8_t *hard_output, size_t hard_output_len, size_t iterations,
                                 size_t *out_iterations_used);
 
+/* Allocation-free syndrome_ldpc_decode_5g.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, code that must not call hipMalloc).  Same arguments,
+ * validation, status codes and results as syndrome_ldpc_decode_5g, plus the
+ * HIP stream to run on (a hipStream_t; NULL = the default stream).  The
+ * codeword is staged in one file-scope device array per GPU of
+ * SYNDROME_STATIC_SCRATCH_BYTES, which the library holds until the process
+ * ends; no hipMalloc, hipFree or stream creation happens in the call.  A
+ * code whose staging (posterior N*4 + messages E*Z*4 + hard decisions N
+ * bytes, each rounded up to 256, + 256) exceeds that size is refused with
+ * INVALID_PARAM (every 3GPP and 802.11 code fits: BG1 Z=384 needs 616,192
+ * bytes).  Calls on one GPU share its array: the library serialises them
+ * with a lock, and each returns only after its results are copied back, so
+ * nothing else may use the array meanwhile. */
+#define SYNDROME_STATIC_SCRATCH_BYTES (640u * 1024u)
+int32_t syndrome_ldpc_decode_5g_static(SyndromeLdpcDecoder *decoder, float *llr, size_t llr_len, size_t n_filler,
+                                       float *edge_r, size_t edge_r_len, float *layer_scratch,
+                                       size_t layer_scratch_len, uint8_t *hard_output, size_t hard_output_len,
+                                       size_t iterations, size_t *out_iterations_used, void *stream);
+
 /* capi/src/lib.rs:369-484 (QcLdpcEncoder, qc_ldpc.rs:2408-2614).  Host-side
  * systematic encoder: codeword = [info | parity] with H * codeword = 0. */
 int32_t syndrome_ldpc_encoder_create(int32_t base_graph, size_t z, SyndromeLdpcEncoder **out_encoder);

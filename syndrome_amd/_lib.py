@@ -46,6 +46,7 @@ SIGNATURES = {
     "syndrome_ldpc_decoder_required_edge_buffer": (SZ, [P]),
     "syndrome_ldpc_decoder_required_layer_buffer": (SZ, [P]),
     "syndrome_ldpc_decode_5g": (I32, [P, P, SZ, SZ, P, SZ, P, SZ, P, SZ, SZ, C.POINTER(SZ)]),
+    "syndrome_ldpc_decode_5g_static": (I32, [P, P, SZ, SZ, P, SZ, P, SZ, P, SZ, SZ, C.POINTER(SZ), P]),
     "syndrome_ldpc_encoder_create": (I32, [I32, SZ, C.POINTER(P)]),
     "syndrome_ldpc_encoder_destroy": (None, [P]),
     "syndrome_ldpc_encoder_info_bit_count": (SZ, [P]),

@@ -198,47 +198,112 @@ int32_t ensure_stage(SyndromeLdpcDecoder *d, size_t bytes)
     return SYNDROME_OK;
 }
 
-// One codeword through the GPU from host buffers: the path behind the
+// Where one codeword's staging lives in a device buffer: posterior (elem
+// bytes per variable), R messages, hard decisions, iteration count, then
+// the optional trace; each part 256-byte aligned.
+struct StageLayout {
+    size_t off_edge, off_hard, off_iter, off_trace, total;
+};
+
+StageLayout stage_layout(const SynCode &c, bool i8, size_t trace_floats)
+{
+    const size_t n = syn::code_n(c), ez = syn::code_ez(c);
+    const size_t pe = i8 ? 2 : 4, me = i8 ? 1 : 4;
+    StageLayout L;
+    L.off_edge = (n * pe + 255) & ~(size_t)255;
+    L.off_hard = L.off_edge + ((ez * me + 255) & ~(size_t)255);
+    L.off_iter = L.off_hard + ((n + 255) & ~(size_t)255);
+    L.off_trace = L.off_iter + 256;
+    L.total = L.off_trace + trace_floats * 4;
+    return L;
+}
+
+// One codeword through the GPU from host buffers, staged at `base` (device
+// memory laid out by stage_layout) on `stream`: the path behind the
 // reference-shaped entry points.  `elem` = 4 (f32 posterior, f32 messages)
-// or 2 (i16 posterior, i8 messages).
-int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, size_t edge_elems, uint8_t *hard,
-                    size_t iterations, size_t n_filler, bool init5g, float scale, int16_t app_clamp,
-                    size_t *out_used, float *trace = nullptr)
+// or 2 (i16 posterior, i8 messages).  Returns once the results are back in
+// the caller's buffers.
+int32_t host_decode_at(SyndromeLdpcDecoder *d, char *base, hipStream_t stream, bool i8, void *llr, void *edge_r,
+                       uint8_t *hard, size_t iterations, size_t n_filler, bool init5g, float scale,
+                       int16_t app_clamp, size_t *out_used, float *trace)
 {
     const SynCode &c = d->code;
     const size_t n = syn::code_n(c), ez = syn::code_ez(c);
     const size_t pe = i8 ? 2 : 4, me = i8 ? 1 : 4;
-    const size_t off_edge = (n * pe + 255) & ~(size_t)255;
-    const size_t off_hard = off_edge + ((ez * me + 255) & ~(size_t)255);
-    const size_t off_iter = off_hard + ((n + 255) & ~(size_t)255);
-    const size_t off_trace = off_iter + 256;
-    const size_t total = off_trace + (trace ? iterations * n * 4 : 0);
-    std::lock_guard<std::mutex> lk(d->mu);
-    DeviceScope ds(d->device);
-    if (int32_t rc = ensure_stage(d, total)) return rc;
-    char *base = (char *)d->d_stage;
+    const StageLayout L = stage_layout(c, i8, trace ? iterations * n : 0);
     const uint32_t flags = (init5g ? SYNDROME_BATCH_INIT_5G : 0u) | SYNDROME_BATCH_WRITE_POST |
                            SYNDROME_BATCH_WRITE_HARD | SYNDROME_BATCH_WRITE_EDGE;
-    if (hipMemcpyAsync(base, llr, n * pe, hipMemcpyHostToDevice, d->stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-    int32_t rc = run_batch(d, i8, base, base, base + off_edge, (uint8_t *)(base + off_hard), (int32_t *)(base + off_iter),
-                           1, iterations, n_filler, scale, app_clamp, flags, d->stream,
-                           trace ? (float *)(base + off_trace) : nullptr);
+    if (hipMemcpyAsync(base, llr, n * pe, hipMemcpyHostToDevice, stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    int32_t rc = run_batch(d, i8, base, base, base + L.off_edge, (uint8_t *)(base + L.off_hard),
+                           (int32_t *)(base + L.off_iter), 1, iterations, n_filler, scale, app_clamp, flags, stream,
+                           trace ? (float *)(base + L.off_trace) : nullptr);
     if (rc) return rc;
     int32_t used = 0;
-    if (hipMemcpyAsync(llr, base, n * pe, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
-        hipMemcpyAsync(edge_r, base + off_edge, ez * me, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
-        hipMemcpyAsync(hard, base + off_hard, n, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
-        hipMemcpyAsync(&used, base + off_iter, 4, hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
-        hipStreamSynchronize(d->stream) != hipSuccess)
+    if (hipMemcpyAsync(llr, base, n * pe, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipMemcpyAsync(edge_r, base + L.off_edge, ez * me, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipMemcpyAsync(hard, base + L.off_hard, n, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipMemcpyAsync(&used, base + L.off_iter, 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
         return SYNDROME_DEVICE_ERROR;
     if (trace && used > 0 &&
-        (hipMemcpyAsync(trace, base + off_trace, (size_t)used * n * 4, hipMemcpyDeviceToHost, d->stream) !=
+        (hipMemcpyAsync(trace, base + L.off_trace, (size_t)used * n * 4, hipMemcpyDeviceToHost, stream) !=
              hipSuccess ||
-         hipStreamSynchronize(d->stream) != hipSuccess))
+         hipStreamSynchronize(stream) != hipSuccess))
         return SYNDROME_DEVICE_ERROR;
-    (void)edge_elems;
     if (out_used) *out_used = (size_t)used;
     return SYNDROME_OK;
+}
+
+// The handle's own staging: allocated on first use (and grown), one call at
+// a time per handle.
+int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, size_t edge_elems, uint8_t *hard,
+                    size_t iterations, size_t n_filler, bool init5g, float scale, int16_t app_clamp,
+                    size_t *out_used, float *trace = nullptr)
+{
+    (void)edge_elems;
+    const StageLayout L = stage_layout(d->code, i8, trace ? iterations * syn::code_n(d->code) : 0);
+    std::lock_guard<std::mutex> lk(d->mu);
+    DeviceScope ds(d->device);
+    if (int32_t rc = ensure_stage(d, L.total)) return rc;
+    return host_decode_at(d, (char *)d->d_stage, d->stream, i8, llr, edge_r, hard, iterations, n_filler, init5g,
+                          scale, app_clamp, out_used, trace);
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free decode_5g: staging in a file-scope device array
+// ---------------------------------------------------------------------------
+
+// Staging of syndrome_ldpc_decode_5g_static: one array per GPU, held until
+// the process ends.  In host code `scratch` names a host-side stand-in, not
+// the array's device address: host code only ever uses the address
+// hipGetSymbolAddress returns for the current device (scratch_on).
+static __device__ __align__(256) unsigned char scratch[SYNDROME_STATIC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_DEVICES = 64;
+struct ScratchSlot {
+    std::once_flag once;
+    void *ptr = nullptr;
+    int32_t status = SYNDROME_OK;
+    std::mutex mu;  // one call at a time on this GPU's array
+};
+ScratchSlot g_scratch[SCRATCH_DEVICES];
+
+// This device's copy of `scratch`, looked up once per device with that
+// device current (each GPU holds its own copy at its own address).
+ScratchSlot *scratch_on(int device)
+{
+    if (device < 0 || device >= SCRATCH_DEVICES) return nullptr;
+    ScratchSlot &sl = g_scratch[device];
+    std::call_once(sl.once, [&] {
+        DeviceScope ds(device);
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            sl.status = SYNDROME_DEVICE_ERROR;
+            return;
+        }
+        sl.ptr = p;
+    });
+    return &sl;
 }
 
 // init_5g_llr on host memory (src/qc_ldpc.rs:711-732): applied when a later
@@ -313,6 +378,36 @@ int32_t syndrome_ldpc_decode_5g(SyndromeLdpcDecoder *d, float *llr, size_t llr_l
             return rc;
         }
         return host_decode(d, false, llr, edge_r, edge_r_len, hard, iterations, n_filler, true, 0.f, 0, out_used);
+    });
+}
+
+int32_t syndrome_ldpc_decode_5g_static(SyndromeLdpcDecoder *d, float *llr, size_t llr_len, size_t n_filler,
+                                       float *edge_r, size_t edge_r_len, float *layer_scratch,
+                                       size_t layer_scratch_len, uint8_t *hard, size_t hard_len, size_t iterations,
+                                       size_t *out_used, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if (!d || !llr || !edge_r || !layer_scratch || !hard) return SYNDROME_NULL_POINTER;
+        const SynCode &c = d->code;
+        const size_t n = syn::code_n(c);
+        if (llr_len != n) return SYNDROME_INVALID_PARAM;  // init_5g_llr
+        int32_t rc = SYNDROME_OK;
+        if (hard_len != n) rc = SYNDROME_INVALID_PARAM;
+        else if (edge_r_len < syn::code_ez(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
+        else if (layer_scratch_len < syn::code_layer_buf(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
+        if (rc) {
+            init_5g_host<float>(c, llr, n_filler, 1000000.0f);
+            return rc;
+        }
+        // the codeword's staging must fit the array
+        if (stage_layout(c, false, 0).total > SYNDROME_STATIC_SCRATCH_BYTES) return SYNDROME_INVALID_PARAM;
+        ScratchSlot *sl = scratch_on(d->device);
+        if (!sl) return SYNDROME_DEVICE_ERROR;
+        if (sl->status) return sl->status;
+        std::lock_guard<std::mutex> lk(sl->mu);
+        DeviceScope ds(d->device);
+        return host_decode_at(d, (char *)sl->ptr, (hipStream_t)stream, false, llr, edge_r, hard, iterations, n_filler,
+                              true, 0.f, 0, out_used, nullptr);
     });
 }
 
