@@ -1014,6 +1014,26 @@ def main():
         out["batched_vs_plain"] = round(value / out["plain"]["value"], 3)
         del d_edge
 
+    # the same batch through the work-queue (persistent) decoder: a plan of
+    # 4,096 BG1 Z=384 blocks, one persistent launch
+    from syndrome_amd import DecodePlan
+    plan = DecodePlan(np.zeros(B, np.int32), np.full(B, 384), np.zeros(B), np.arange(B, dtype=np.int64) * n,
+                      args.beta)
+    d_post3, d_hard3, d_it3 = torch.empty_like(d_in), torch.empty_like(d_hard), torch.empty_like(d_it)
+    if not args.no_tune:
+        plan.tune(d_in, d_post3, d_hard3, d_it3, args.iters, flags)
+    qwall, qper = time_launches(lambda: plan.run(d_in, d_post3, d_hard3, d_it3, args.iters, flags), args.steps,
+                                args.warmup, torch)
+    qms = qwall * 1e3 / args.steps
+    out["persistent"] = {"value": round(B * k / (qms * 1e-3) / 1e9, 4), "unit": "Gbit/s", "ms_per_step": round(qms, 4),
+                         "kernel": "syn::persist_decode<false, 0>", "shape": plan.shape(),
+                         "vs_per_config_kernel": round(B * k / (qms * 1e-3) / 1e9 / value, 4),
+                         "identical_decoded_bits": bool(torch.equal(d_hard3, d_hard) and torch.equal(d_it3, d_it) and
+                                                        torch.equal(d_post3.view(torch.int32), d_post.view(torch.int32)))}
+    if "plain" in out:
+        out["persistent_vs_plain"] = round(out["persistent"]["value"] / out["plain"]["value"], 3)
+    del plan, d_post3, d_hard3
+
     # single code block latency (one codeword per launch, inputs resident)
     one = lambda: dev.decode_batch_f32(d_in[:1], d_post[:1], d_hard[:1], d_it[:1], args.iters, flags=flags)
     lat1 = time_launches(one, args.latency_samples, 10, torch)[1] if args.latency_samples > 0 else np.zeros(1)

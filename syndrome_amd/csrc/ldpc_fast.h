@@ -616,13 +616,44 @@ __global__ void __launch_bounds__(512) fast_i8(const SynCode *__restrict__ code,
 
 namespace syn {
 
+// Compile-time-Z bodies a work-queue kernel carries: the lifting sizes ZS of
+// base graph KIND get the register-resident body with immediate shifts and
+// wrap masks (fast_group<KIND, Z>); other sizes run the run-time-Z body.
+template <int... ZS>
+struct ZSet {};
+
+template <int KIND, bool I8, int Z0, int... ZS>
+__device__ __forceinline__ bool static_group(int z, const SynCode *__restrict__ code, int count, const GroupIO &io,
+                                             const SynLaunch &a)
+{
+    if (z == Z0) {
+        fast_group<KIND, Z0, I8, !I8>(code, Z0, count, io, a);
+        return true;
+    }
+    if constexpr (sizeof...(ZS) > 0) return static_group<KIND, I8, ZS...>(z, code, count, io, a);
+    else return false;
+}
+
+template <int KIND, bool I8, int... ZS>
+__device__ __forceinline__ void item_group(ZSet<ZS...>, const SynCode *__restrict__ code, int count, const GroupIO &io,
+                                           const SynLaunch &a)
+{
+    const int z = code->z;
+    if constexpr (sizeof...(ZS) > 0)
+        if (static_group<KIND, I8, ZS...>(z, code, count, io, a)) return;
+    fast_group<KIND, 0, I8, !I8>(code, z, count, io, a);
+}
+
 // Persistent work-queue decoder: a grid sized to the resident workgroups
 // pulls items off a device counter until the queue is drained (every
 // workgroup reaches the exit test with the same counter, so all of them
 // leave), decoding each item with the register-resident body for its base
-// graph at its lifting size.  Items of different base graphs, lifting sizes
-// and filler counts share one launch.
-template <bool I8>
+// graph at its lifting size: a compile-time-Z body for the sizes in Z1
+// (BG1) and Z2 (BG2), the run-time-Z body for the rest.  Items of
+// different base graphs, lifting sizes and filler counts share one launch.
+// The int8 body keeps its register-lean form: both base graphs share one
+// kernel, whose VGPR budget the BG1 body sets.
+template <bool I8, class Z1, class Z2>
 __device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, const SynItem *__restrict__ items,
                                              int n_items, int *head, const int *__restrict__ ids,
                                              const long long *__restrict__ offs, const int *__restrict__ nfill,
@@ -638,10 +669,8 @@ __device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, 
         const SynItem it = items[w];
         const SynCode *code = codes + it.code;
         const GroupIO io{it.count, 0, ids + it.first, offs, nfill, nullptr, nullptr};
-        // the int8 body keeps its register-lean form here: both base
-        // graphs share one kernel, whose VGPR budget the BG1 body sets
-        if (it.kind == SYN_KIND_BG1) fast_group<SYN_KIND_BG1, 0, I8, !I8>(code, code->z, it.count, io, a);
-        else fast_group<SYN_KIND_BG2, 0, I8, !I8>(code, code->z, it.count, io, a);
+        if (it.kind == SYN_KIND_BG1) item_group<SYN_KIND_BG1, I8>(Z1{}, code, it.count, io, a);
+        else item_group<SYN_KIND_BG2, I8>(Z2{}, code, it.count, io, a);
     }
 }
 

@@ -1,9 +1,10 @@
-// Work-queue decode (f32) and the queue reset.
+// Work-queue decode (f32) for launches that hold codes with Z > 256, and the
+// queue reset.
 #include "ldpc_persist.h"
 
 namespace syn {
 
-const void *persist_kernel_f32() { return (const void *)persist_decode<false>; }
+const void *persist_kernel_f32_big() { return (const void *)persist_decode<false, 0>; }
 
 int launch_reset(int *head, hipStream_t stream)
 {

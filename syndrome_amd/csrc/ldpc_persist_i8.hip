@@ -3,6 +3,6 @@
 
 namespace syn {
 
-const void *persist_kernel_i8() { return (const void *)persist_decode<true>; }
+const void *persist_kernel_i8() { return (const void *)persist_decode<true, 0>; }
 
 }  // namespace syn

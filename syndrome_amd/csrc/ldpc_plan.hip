@@ -17,7 +17,8 @@
 #include "ldpc_common.h"
 
 namespace syn {
-const void *persist_kernel_f32();
+const void *persist_kernel_f32_big();
+const void *persist_kernel_f32_small();
 const void *persist_kernel_i8();
 int launch_reset(int *head, hipStream_t stream);
 int allow_max_dynamic_lds(const void *fn);
@@ -49,6 +50,9 @@ struct SyndromeLdpcPlan {
     size_t lds = 0;
     int block_s = 0, grid_s = 0, n_items_s = 0;
     size_t lds_s = 0;
+    // the kernel of each launch: f32 codes with Z > 256 in a launch take the
+    // kernel whose compile-time-Z bodies are the large sizes
+    const void *fn_main = nullptr, *fn_small = nullptr;
     std::vector<int> cws_of_code;
     // device buffers
     SynCode *d_codes = nullptr;
@@ -175,8 +179,15 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     if (int32_t rc = upload(p, &p->d_items, flat[0])) return rc;
     if (int32_t rc = upload(p, &p->d_items_s, flat[1])) return rc;
     if (int32_t rc = upload(p, &p->d_ids, ids)) return rc;
-    const void *fn = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32();
-    if (syn::allow_max_dynamic_lds(fn)) return SYNDROME_DEVICE_ERROR;
+    int main_z = 0;  // largest Z in the main launch
+    for (size_t c = 0; c < nc; ++c)
+        if (!part[c]) main_z = std::max(main_z, p->codes[c].z);
+    const void *fn = p->i8 ? syn::persist_kernel_i8()
+                           : (main_z > SPLIT_LANES ? syn::persist_kernel_f32_big() : syn::persist_kernel_f32_small());
+    const void *fn_s = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32_small();
+    if (syn::allow_max_dynamic_lds(fn) || syn::allow_max_dynamic_lds(fn_s)) return SYNDROME_DEVICE_ERROR;
+    p->fn_main = fn;
+    p->fn_small = fn_s;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds[0]) != hipSuccess || per_cu < 1)
         per_cu = 1;
@@ -189,7 +200,7 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     p->n_items_s = (int)flat[1].size();
     p->grid_s = 0;
     if (p->n_items_s) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, SPLIT_LANES, lds[1]) != hipSuccess || per_cu < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn_s, SPLIT_LANES, lds[1]) != hipSuccess || per_cu < 1)
             per_cu = 1;
         p->grid_s = std::max(1, std::min(p->n_items_s, per_cu * p->cus));
     }
@@ -221,7 +232,6 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
         const float b = p->beta * scale;
         a.beta_q = !(b - b == 0.f) ? 127 : (int)std::min(127.f, std::max(0.f, roundf(b)));
     }
-    const void *fn = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32();
     const SynCode *codes = p->d_codes;
     const int *ids = p->d_ids;
     const long long *offs = p->d_offs;
@@ -237,7 +247,7 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
         int *head = heads + q * HEAD_STRIDE;
         if (syn::launch_reset(head, stream)) return SYNDROME_DEVICE_ERROR;
         void *args[] = {&codes, &items, &n_items, &head, &ids, &offs, &nf, &a};
-        if (hipLaunchKernel(fn, dim3(q ? p->grid_s : p->grid), dim3(q ? p->block_s : p->block), args,
+        if (hipLaunchKernel(q ? p->fn_small : p->fn_main, dim3(q ? p->grid_s : p->grid), dim3(q ? p->block_s : p->block), args,
                             q ? p->lds_s : p->lds, stream) != hipSuccess)
             return SYNDROME_DEVICE_ERROR;
     }

@@ -38,7 +38,7 @@
 #include "ldpc_common.h"
 
 namespace syn {
-const void *persist_kernel_f32();
+const void *persist_kernel_f32_big();
 const void *resident_kernel();
 int allow_max_dynamic_lds(const void *fn);
 }  // namespace syn
@@ -369,7 +369,7 @@ int32_t SyndromeServer::launch_batch(Lane &ln, std::vector<int> &batch)
     a.iterations = iterations;
     a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD | SYN_F_ALIGNED;  // slot offsets are multiples of NMAX
     a.beta = beta;
-    const void *fn = syn::persist_kernel_f32();
+    const void *fn = syn::persist_kernel_f32_big();
     if (syn::allow_max_dynamic_lds(fn)) return SYNDROME_DEVICE_ERROR;
     const SynCode *cp = d_codes;
     const SynItem *ip = (const SynItem *)(d + lay.o_items);
