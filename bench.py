@@ -642,14 +642,19 @@ def bench_serving(torch, args, peak_ref_gbps):
                                                       tr["tb_size"], tr["d_tb_out"], tr["tb_out_off"], rate, dur,
                                                       seed=args.seed)
             runt(0, 0.5)
-            tbl, ncb, peak_bps, fails, _ = runt(0, args.serve_seconds)
+            tbl, ncb, peak_bps, fails, _, peak_good, _ = runt(0, args.serve_seconds)
             peak_tb_rate = peak_bps / mean_tb_bits
-            res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_tb_per_s": round(len(tbl) / args.serve_seconds, 1),
+            res = {"peak_info_gbps": round(peak_bps / 1e9, 4), "peak_goodput_gbps": round(peak_good / 1e9, 4),
+                   "peak_tb_per_s": round(len(tbl) / args.serve_seconds, 1),
                    "crc_failures_at_peak": int(fails), "loads": []}
+            split = lambda x, m, q: pct(x[m], q) if m.any() else None
             for load in (0.1, 0.3, 0.5, 0.7, 0.9):
-                tbl, ncb, bps, fails, waits = runt(load * peak_tb_rate, args.serve_seconds)
+                tbl, ncb, bps, fails, waits, good, okf = runt(load * peak_tb_rate, args.serve_seconds)
                 res["loads"].append({"offered_fraction_of_peak": load, "achieved_info_gbps": round(bps / 1e9, 4),
+                                     "goodput_gbps": round(good / 1e9, 4),
                                      "tb_latency_ms_p50": pct(tbl, 50), "tb_latency_ms_p99": pct(tbl, 99),
+                                     "tb_latency_ms_p99_crc_passed": split(tbl, okf, 99),
+                                     "tb_latency_ms_p99_crc_failed": split(tbl, ~okf, 99),
                                      "transport_blocks": int(len(tbl)), "code_blocks": int(ncb),
                                      "crc_failures": int(fails)})
             out["resident_transport_blocks_inputs_in_hbm"] = res

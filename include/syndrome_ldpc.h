@@ -382,8 +382,22 @@ typedef struct {
 
 int32_t syndrome_server_submit_tb(SyndromeServer *server, const uint32_t *slots, const float *const *srcs,
                                   size_t num_cb, size_t tb_size, uint8_t *tb_out, uint64_t tag);
+/* A decided transport block, or SYNDROME_BUSY when none is ready.  A
+ * transport block some of whose blocks the server could not decode (a
+ * failed launch) is delivered with crc_ok 0 and max_iterations -1, and the
+ * call returns SYNDROME_DEVICE_ERROR; release it as any other. */
 int32_t syndrome_server_try_recv_tb(SyndromeServer *server, SyndromeServedTb *out);
+/* DecodeReport::cb_crc (transport_block.rs:140-149) of a received, not yet
+ * released transport block: cb_ok[i] = 1 when code block i passed its
+ * CRC-24B (or is the transport block's only block), 0 when it failed or was
+ * never decoded; *out_n = num_cb (BUFFER_TOO_SMALL when cap < num_cb). */
+int32_t syndrome_server_tb_block_crc(SyndromeServer *server, uint32_t tb, uint8_t *cb_ok, size_t cap,
+                                     size_t *out_n);
 int32_t syndrome_server_release_tb(SyndromeServer *server, uint32_t tb);
+/* Launches and blocks decoded so far, and the slot count.  Returns the
+ * server's state: OK, or the sticky DEVICE_ERROR of a failed launch, after
+ * which every submit is refused with that error and every block or
+ * transport block not yet decoded is delivered as failed (iterations -1). */
 int32_t syndrome_server_stats(const SyndromeServer *server, int64_t *out_batches, int64_t *out_decoded,
                               int32_t *out_slots);
 /* Pinned, device-mapped host memory (usable as a submit_from source). */
@@ -408,15 +422,17 @@ int32_t syndrome_server_loadgen(SyndromeServer *server, const float *pool, const
  * syndrome_server_submit_tb (inputs from the pool, payload to tb_out +
  * tb_out_off[p], tb_size[p] bits); latency = transport-block completion,
  * CRCs decided on the device, minus scheduled arrival.  Reports completed
- * transport blocks and their code blocks, info bits/s (sum of tb_size) and
- * CRC failures. */
+ * transport blocks and their code blocks, info bits/s (sum of tb_size),
+ * goodput (tb_size of the transport blocks whose CRCs passed, per second),
+ * CRC failures, and for each latency entry whether its transport block
+ * passed (tb_lat_ok, may be NULL). */
 int32_t syndrome_server_loadgen_tb(SyndromeServer *server, const float *pool, const int64_t *cb_off,
                                    const int32_t *cb_bg, const int32_t *cb_z, const int32_t *cb_nfill,
                                    const int32_t *tb_first_cb, const int32_t *tb_num_cb, const int32_t *tb_size,
                                    uint8_t *tb_out, const int64_t *tb_out_off, size_t n_tb, double tb_rate,
-                                   double duration_s, uint64_t seed, double *tb_lat_ns, size_t lat_cap,
-                                   size_t *n_tbs, size_t *n_cbs, double *info_bps, int64_t *crc_fails,
-                                   int64_t *waits);
+                                   double duration_s, uint64_t seed, double *tb_lat_ns, uint8_t *tb_lat_ok,
+                                   size_t lat_cap, size_t *n_tbs, size_t *n_cbs, double *info_bps,
+                                   double *goodput_bps, int64_t *crc_fails, int64_t *waits);
 
 /* Synthetic code-block traffic, as the reference's benchmarks build it
  * (channel_sim.rs, QcLdpcEncoder, rate_match): for block i, K'_i = K_i -

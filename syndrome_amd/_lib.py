@@ -101,7 +101,8 @@ SIGNATURES = {
                                       C.POINTER(SZ), P, SZ, C.POINTER(SZ), C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]),
     "syndrome_server_loadgen_tb": (I32, [P, P, P, P, P, P, P, P, P, P, P, SZ, C.c_double, C.c_double, C.c_uint64,
-                                         P, SZ, P, P, P, P, P]),
+                                         P, P, SZ, P, P, P, P, P, P]),
+    "syndrome_server_tb_block_crc": (I32, [P, C.c_uint32, P, SZ, C.POINTER(SZ)]),
 }
 
 

@@ -126,6 +126,7 @@ struct SynResidentTb {
     SynTbState *state;      // per ring index (device memory)
     unsigned *done;         // per ring index: id + 1 once decided (host memory)
     unsigned *ok;           // per ring index: 1 if every check passed (host memory)
+    unsigned *cbok;         // per server slot: 1 if its CRC-24B passed or it is alone (host memory)
 };
 
 // Bytes of the resident decoder's device-side queue block (two 128-byte

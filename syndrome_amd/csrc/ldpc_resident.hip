@@ -24,8 +24,87 @@
 // when work is published after it has left.
 #include "ldpc_crc.h"
 #include "ldpc_fast.h"
+#include "resident_protocol.h"
 
 namespace syn {
+
+// The protocol's atomics on this device (resident_protocol.h): device-memory
+// words at agent scope, host-memory words at system scope.
+struct DevOps {
+    __device__ unsigned load_relaxed(const unsigned *p) const
+    {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ long long load_relaxed(const long long *p) const
+    {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ unsigned load_acquire(const unsigned *p) const
+    {
+        return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ unsigned load_acquire_system(const unsigned *p) const
+    {
+        return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __device__ unsigned load_relaxed_system(const unsigned *p) const
+    {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __device__ void store_relaxed(unsigned *p, unsigned v) const
+    {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ void store_relaxed(long long *p, long long v) const
+    {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ void store_release(unsigned *p, unsigned v) const
+    {
+        __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ void store_release_system(unsigned *p, unsigned v) const
+    {
+        __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __device__ bool try_lock(unsigned *p) const
+    {
+        unsigned unlocked = 0;
+        return __hip_atomic_compare_exchange_strong(p, &unlocked, 1u, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ void unlock(unsigned *p) const { __hip_atomic_store(p, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
+    __device__ bool cas_relaxed(unsigned *p, unsigned expect, unsigned want) const
+    {
+        return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ void fence_acquire() const { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+    __device__ void fence_release_system() const { __builtin_amdgcn_fence(__ATOMIC_RELEASE, ""); }
+    __device__ void fence_seq_cst() const { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent"); }
+    __device__ unsigned atomic_xor(unsigned *p, unsigned v) const { return atomicXor(p, v); }
+    __device__ unsigned atomic_or(unsigned *p, unsigned v) const { return atomicOr(p, v); }
+    __device__ unsigned atomic_add(unsigned *p, unsigned v) const { return atomicAdd(p, v); }
+    __device__ unsigned atomic_exch(unsigned *p, unsigned v) const { return atomicExch(p, v); }
+    __device__ long long clock() const { return (long long)wall_clock64(); }
+    // sleeps of 512 cycles, n of them
+    __device__ void backoff(int n) const
+    {
+        for (int k = 0; k < n; ++k) __builtin_amdgcn_s_sleep(8);
+    }
+};
+
+// Before the workgroup barrier that precedes lane 0's publication, every
+// wave waits until its own vector stores have completed (s_waitcnt
+// vmcnt(0): on gfx950 vmcnt counts stores as well as loads).  A system-scope
+// release is a write-back of this XCD's L2 (for data it caches) followed by
+// that same wait, and it covers only the releasing wave's stores; the
+// results published here go to coherent host memory, which the L2 does not
+// hold, so each wave's completed stores plus lane 0's release after the
+// barrier make every wave's results visible before the completion word.
+// (A full system-scope release fence in every wave, eight L2 write-backs per
+// item, measured 10 % worse p99 latency in serving.)
+__device__ __forceinline__ void release_all_waves() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
 
 // A word of a record in host memory, read at system scope so that no GPU
 // cache can return an older copy of it.
@@ -45,11 +124,12 @@ __device__ __forceinline__ T host_load(const T *p)
 //     so that the XOR of all blocks' shares is the stream's remainder;
 //   * the received CRC-24A bits that fall in its P bits;
 //   * its payload bits written to the transport block's output.
-// Lane 0 folds these into the transport block's device state; the block
-// that completes the count decides the transport block and publishes
-// ok[] and done[] (the payload writes of every block are fenced at system
-// scope before that block's count increment).
-__device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, const float *post)
+// Lane 0 records its CRC-24B outcome and folds these into the transport
+// block's device state (tb_account, resident_protocol.h); the block that
+// completes the count decides the transport block and publishes ok[] and
+// done[] (every wave fences the block's payload writes at system scope
+// before its count increment).
+__device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, int slot, const float *post)
 {
     __shared__ uint32_t red[512];
     __shared__ unsigned s_id;
@@ -73,6 +153,7 @@ __device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, c
     const long long len = a - s0 < 0 ? 0 : (a - s0 > p ? p : a - s0);
     const uint32_t part = len > 0 ? group_crc24(bit, len, CRC24A_POLY, red) : 0u;
     for (long long u = threadIdx.x; u < len; u += blockDim.x) s_out[s0 + u] = (unsigned char)bit(u);
+    release_all_waves();  // this block's payload bits are visible before it is counted
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t share = crc24_mulmod(part, crc24_xpow(a - (s0 + len), CRC24A_POLY), CRC24A_POLY);
@@ -81,39 +162,11 @@ __device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, c
             const long long t = (long long)a + i;
             if (t >= s0 && t < s0 + p) rx |= bit(t - s0) << (23 - i);
         }
-        SynTbState *st = tbs.state + ti;
-        __threadfence_system();  // this block's payload bits are visible before it is counted
-        atomicXor(&st->acc, share);
-        atomicOr(&st->rx, rx);
-        if (!cb_ok) atomicOr(&st->bad, 1u);
-        __threadfence();
-        if (atomicAdd(&st->count, 1u) == (unsigned)(c - 1)) {
-            __threadfence();
-            const unsigned acc = atomicOr(&st->acc, 0u), rxv = atomicOr(&st->rx, 0u), bad = atomicOr(&st->bad, 0u);
-            atomicExch(&st->acc, 0u);  // ready for the ring index's next transport block
-            atomicExch(&st->rx, 0u);
-            atomicExch(&st->bad, 0u);
-            atomicExch(&st->count, 0u);
-            tbs.ok[ti] = (!bad && acc == rxv) ? 1u : 0u;
-            __hip_atomic_store(tbs.done + ti, s_id + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
+        tbs.cbok[slot] = cb_ok ? 1u : 0u;  // released with the block's count
+        tb_account(DevOps{}, tbs.state + ti, share, rx, cb_ok, c, s_id, tbs.ok + ti, tbs.done + ti);
     }
     __syncthreads();
 }
-
-// Device-side queue words (SYN_RESIDENT_QUEUE_BYTES of device memory): the
-// claim counter, alone on its cache line, then the device's copy of the
-// host's tail and stop words and the lock that lets one waiting workgroup
-// at a time refresh that copy from host memory (so idle workgroups spin on
-// device memory, not on PCIe).  Waiters poll the second line; a claim's
-// compare-and-swap on the first does not queue behind their traffic.
-struct ResidentQueue {
-    unsigned head;
-    unsigned pad0[31];
-    unsigned tail, stop, poll_lock;
-    unsigned pad1[29];
-};
-static_assert(sizeof(ResidentQueue) == SYN_RESIDENT_QUEUE_BYTES, "queue block size");
 
 __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict__ codes,
                                                         const SynWork *__restrict__ ring, unsigned cap,
@@ -131,56 +184,13 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         __syncthreads();  // the previous item is finished with LDS and the s_* words
         if (threadIdx.x == 0) {
             unsigned w = 0;
-            int have = 0;
-            int backoff = 1;  // sleeps of 512 cycles between idle polls, doubling to 64
             // Idle workgroups back off exponentially between polls: each
             // poll's acquire invalidates this XCD's L2 (each XCD has its own),
             // and idle workgroups polling without pause kept evicting the
             // decoding workgroups' data, which slowed a lone BG1 Z=384 block
             // from 43 to about 100 us.  The item's ring entry is read with
             // system-scope loads (host_load), which no cache serves.
-            for (;;) {
-                const unsigned h = __hip_atomic_load(&q->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                unsigned t = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                if ((int)(t - h) <= 0) {
-                    unsigned unlocked = 0;
-                    if (__hip_atomic_compare_exchange_strong(&q->poll_lock, &unlocked, 1u, __ATOMIC_ACQUIRE,
-                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        t = __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-                        const unsigned stop = __hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                        // the lock makes this the only writer: a plain store keeps the
-                        // copy monotonic modulo 2^32
-                        __hip_atomic_store(&q->tail, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if (stop) __hip_atomic_store(&q->stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        __hip_atomic_store(&q->poll_lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                }
-                if ((int)(t - h) > 0) {
-                    unsigned expect = h;
-                    if (__hip_atomic_compare_exchange_strong(&q->head, &expect, h + 1, __ATOMIC_RELAXED,
-                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        // the item's inputs (HBM or host memory) may have been
-                        // rewritten since this XCD's L2 last held them: one
-                        // agent-scope acquire invalidates those lines
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                        w = h;
-                        have = 1;
-                        __hip_atomic_store(last_claim, (long long)wall_clock64(), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                    continue;
-                }
-                // idle = no claim by any workgroup since this grid started
-                const long long now = wall_clock64();
-                const long long lc = __hip_atomic_load(last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const long long since = now - (lc > t0 ? lc : t0);
-                if (__hip_atomic_load(&q->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || since > idle_ticks ||
-                    now - t0 > life_ticks)
-                    break;
-                for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(8);
-                backoff = backoff < 64 ? 2 * backoff : 64;
-            }
+            const int have = resident_claim(DevOps{}, q, ctl, last_claim, t0, idle_ticks, life_ticks, w) ? 1 : 0;
             s_w = w;
             s_have = have;
             if (have) {
@@ -228,14 +238,14 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
             const int n = code->n_b * code->z;
             extern __shared__ __align__(16) unsigned char smem[];
             for (int j = 0; j < s_count; ++j)
-                if (s_tb[j] >= 0) tb_fold(tbs, s_tb[j], s_tbci[j], (const float *)(smem + pad) + (size_t)j * n);
+                if (s_tb[j] >= 0) tb_fold(tbs, s_tb[j], s_tbci[j], s_slots[j], (const float *)(smem + pad) + (size_t)j * n);
         }
-        // Every lane's hard decisions and counts happen before the barrier;
-        // the system-scope release by lane 0 then publishes all of them
-        // with the item's completion word.
+        // Every wave releases its own hard decisions and counts (system
+        // scope) before the barrier; lane 0 then publishes the item's
+        // completion word with a system-scope release.
+        release_all_waves();
         __syncthreads();
-        if (threadIdx.x == 0)
-            __hip_atomic_store(done + w % cap, w + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (threadIdx.x == 0) item_complete(DevOps{}, done, cap, w);
     }
 }
 

@@ -2,7 +2,9 @@
 //
 // The caller-facing protocol is the reference pipeline's
 // (src/ldpc_pipeline.rs:499-576): acquire a slot, write the code block's
-// decoder input into it, submit, poll completions, release.  Behind it:
+// decoder input into it, submit, poll completions, release.  The protocol,
+// admission and the resident dispatcher are the host-side core
+// (serve_core.h); this file holds what touches the device.  Behind it:
 //
 //  * admission by HBM budget: the device input arena is cut into fixed
 //    slots sized for the largest code (68 x 384 f32); `budget / slot_bytes`
@@ -27,7 +29,6 @@
 #include <atomic>
 #include <chrono>
 #include <deque>
-#include <initializer_list>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -36,6 +37,7 @@
 #include "../../include/syndrome_ldpc.h"
 #include "code_host.h"
 #include "ldpc_common.h"
+#include "serve_core.h"
 
 namespace syn {
 const void *persist_kernel_f32_big();
@@ -43,9 +45,12 @@ const void *resident_kernel();
 int allow_max_dynamic_lds(const void *fn);
 }  // namespace syn
 
+struct SyndromeServer;
+
 namespace {
 
-constexpr int NMAX = 68 * 384;
+using syn::now_ns;
+constexpr int NMAX = syn::SERVE_NMAX;
 // Pinned host memory the device reads and writes while kernels run: mapped
 // into the device's address space and coherent (never held in device
 // caches), so a resident kernel sees the host's writes and the host sees
@@ -58,33 +63,6 @@ constexpr double RESIDENT_LIFE_S = 600.0;       // and never waits beyond this a
 constexpr int VALID_Z[51] = {2,  3,  4,  5,  6,  7,  8,  9,  10,  11,  12,  13,  14,  15,  16,  18,  20,
                              22, 24, 26, 28, 30, 32, 36, 40, 44,  48,  52,  56,  60,  64,  72,  80,  88,
                              96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384};
-
-// The server lock.  Its critical sections last well under a microsecond,
-// while a thread that blocks in the kernel on a contended std::mutex takes
-// tens of microseconds to be woken: a waiter retries for a while before it
-// blocks.  That wake-up was most of a code block's host-side latency.
-class ServerMutex {
-  public:
-    void lock()
-    {
-        for (int i = 0; i < SPINS; ++i)
-            if (m_.try_lock()) return;
-        m_.lock();
-    }
-    bool try_lock() { return m_.try_lock(); }
-    void unlock() { m_.unlock(); }
-
-  private:
-    static constexpr int SPINS = 4000;
-    std::mutex m_;
-};
-
-int z_index(int z)
-{
-    for (int i = 0; i < 51; ++i)
-        if (VALID_Z[i] == z) return i;
-    return -1;
-}
 
 // Batch descriptor, written by the host into mapped pinned memory:
 // items[ni] | ids[nb] | offs[nb] | src[nb] | nfill[nb] | slot[nb] | n[nb].
@@ -143,108 +121,56 @@ struct Lane {
     int *h_iters = nullptr, *hd_iters = nullptr;  // per batch position, written by the decode
 };
 
-double now_ns()
-{
-    return (double)std::chrono::duration_cast<std::chrono::nanoseconds>(
-               std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
+int32_t launch_resident(SyndromeServer &s);
+
+// The resident grid on HIP: launched on the server's stream, its end seen
+// by an event recorded after it.
+struct HipResident {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    template <class Core>
+    int32_t launch(Core &c)
+    {
+        return launch_resident(static_cast<SyndromeServer &>(c));
+    }
+    int query()
+    {
+        const hipError_t e = hipEventQuery(ev);
+        return e == hipSuccess ? 1 : (e == hipErrorNotReady ? 0 : -1);
+    }
+    void wait() { (void)hipEventSynchronize(ev); }
+};
 
 }  // namespace
 
-struct SyndromeServer {
+struct SyndromeServer : syn::ServeCore<HipResident> {
     int device = 0;
-    float beta = 0.5f;
-    int iterations = 8;
-    int max_batch = 256;
-    int slots = 0;
     int lds_limit = 0, cus = 0;
-    // arenas: decoder inputs in HBM (d_llr), their pinned staging (h_llr),
-    // hard decisions written by the decode straight into pinned memory
-    float *h_llr = nullptr, *d_llr = nullptr, *h_llr_dev = nullptr;
-    uint8_t *h_hard = nullptr, *h_hard_dev = nullptr;
-    int *h_it = nullptr, *h_it_dev = nullptr;  // iterations per slot
+    // decoder inputs in HBM (d_llr); device views of the pinned arenas
+    float *d_llr = nullptr;
+    uint8_t *h_hard_dev = nullptr;
+    int *h_it_dev = nullptr;
+    unsigned *h_cbok_dev = nullptr;
     SynCode *d_codes = nullptr;  // 102 codes: bg * 51 + z index
-    std::vector<SynCode> codes;
-    // slot state
-    struct Slot {
-        const float *src = nullptr;  // device-accessible input
-        int code = -1, n = 0, nfill = 0;
-        int tb = -1, tb_ci = 0;      // transport block (resident mode) and index in it
-        uint64_t tag = 0;
-        double t_submit = 0, t_done = 0;
-        int state = 0;  // 0 free, 1 acquired, 2 submitted, 3 on gpu, 4 done, 5 delivered
-    };
-    std::vector<Slot> st;
-    std::vector<int> free_list;
-    std::deque<int> submitted, done;
     std::vector<Lane> lanes;
-    ServerMutex mu;
-    // Sizes of submitted, done, rs.tb_done_q and rs.tb_outstanding, stored
-    // (sync_counts, under mu) after every change: a thread with nothing to
-    // take reads them instead of taking the lock.  A count may be stale;
-    // a non-zero one only sends the reader to the lock, where it checks.
-    std::atomic<int> n_submitted{0}, n_done{0}, n_tb_done{0}, n_tb_out{0};
-    std::atomic<bool> stop{false};
     std::thread worker;
-    std::atomic<long long> batches{0}, decoded{0};
-    int error = 0;
-
-    // resident mode: one grid stays on the device and pulls items from a
-    // ring in coherent pinned memory (see ldpc_resident.hip)
-    bool resident = false;
-    struct Outstanding {
-        unsigned id;
-        int n;
-        int slot[SYN_WORK_MAX];
-        bool fin;  // seen complete (popped once everything before it is)
-    };
-    struct Resident {
-        unsigned cap = 0;
-        SynWork *ring = nullptr, *ring_dev = nullptr;
-        SynResidentCtl *ctl = nullptr, *ctl_dev = nullptr;
-        unsigned *done = nullptr, *done_dev = nullptr;
+    // resident mode: device views of the ring and records, device-side state
+    struct ResidentDev {
+        SynWork *ring_dev = nullptr;
+        SynResidentCtl *ctl_dev = nullptr;
+        unsigned *done_dev = nullptr;
         unsigned *d_head = nullptr;
         long long *d_last = nullptr, *d_offs = nullptr;
-        hipStream_t stream = nullptr;
-        hipEvent_t ev = nullptr;
-        bool running = false;
-        unsigned published = 0;
-        std::deque<Outstanding> outstanding;  // in publish (= claim) order
-        std::vector<int> cws;  // code blocks per item, per code
-        int grid = 0;
         size_t lds = 0;
         long long idle_ticks = 0, life_ticks = 0;
-        // transport blocks (submit_tb): ring in coherent pinned memory,
-        // fold state in device memory
-        unsigned tb_cap = 0, tb_next = 0;
-        SynTbWork *tbw = nullptr, *tbw_dev = nullptr;
+        SynTbWork *tbw_dev = nullptr;
         SynTbState *d_tbstate = nullptr;
-        unsigned *tb_done = nullptr, *tb_done_dev = nullptr, *tb_ok = nullptr, *tb_ok_dev = nullptr;
-        struct TbRec {
-            std::vector<int> slots;
-            uint64_t tag = 0;
-            size_t tb_size = 0;
-            double t_submit = 0, t_done = 0;
-            int state = 0;  // 0 free, 2 submitted, 4 done, 5 delivered
-        };
-        std::vector<TbRec> tbr;
-        std::vector<int> tb_free, tb_outstanding;
-        std::deque<int> tb_done_q;
-    } rs;
+        unsigned *tb_done_dev = nullptr, *tb_ok_dev = nullptr;
+    } rd;
 
-    void sync_counts()
-    {
-        n_submitted.store((int)submitted.size(), std::memory_order_release);
-        n_done.store((int)done.size(), std::memory_order_release);
-        n_tb_done.store((int)rs.tb_done_q.size(), std::memory_order_release);
-        n_tb_out.store((int)rs.tb_outstanding.size(), std::memory_order_release);
-    }
     int32_t launch_batch(Lane &ln, std::vector<int> &batch);
     int32_t init_resident();
-    int32_t launch_resident();
     void run();
-    void run_resident();
 };
 
 namespace {
@@ -273,34 +199,38 @@ struct DeviceScope {
     }
 };
 
-// Marks slots[0..n) with state `mark` if each is in range, in one of the
-// states `want` (and, with no_tb, not part of a transport block) and listed
-// once (a repeat finds its own mark); on any failure every mark is undone
-// and false returned.  Called with the server lock held.
-bool claim_slots(SyndromeServer *s, const uint32_t *slots, size_t n, std::initializer_list<int> want, int mark,
-                 bool no_tb)
-{
-    std::vector<int> prev(n);
-    size_t i = 0;
-    for (; i < n; ++i) {
-        const uint32_t sl = slots[i];
-        if (sl >= (uint32_t)s->slots) break;
-        const int stt = s->st[sl].state;
-        bool ok = false;
-        for (int w : want) ok = ok || stt == w;
-        if (!ok || (no_tb && s->st[sl].tb >= 0)) break;
-        prev[i] = stt;
-        s->st[sl].state = mark;
-    }
-    if (i == n) return true;
-    while (i-- > 0) s->st[slots[i]].state = prev[i];
-    return false;
-}
-
 size_t group_bytes(const SynCode &c, int cws)
 {
     const size_t n = syn::code_n(c);
     return (size_t)(((c.z * 4) + 15) & ~15) + ((cws * n * 4 + 3) & ~(size_t)3) + 12 * (size_t)cws;
+}
+
+
+// One resident grid on the server's stream (HipResident::launch).
+int32_t launch_resident(SyndromeServer &s)
+{
+    SynLaunch a{};
+    a.hard = s.h_hard_dev;
+    a.iters = s.h_it_dev;
+    a.iterations = s.iterations;
+    a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD;  // SYN_F_ALIGNED per item
+    a.beta = s.beta;
+    const SynCode *cp = s.d_codes;
+    const SynWork *ring = s.rd.ring_dev;
+    unsigned cap = s.rs.cap;
+    SynResidentCtl *ctl = s.rd.ctl_dev;
+    unsigned *dn = s.rd.done_dev;
+    void *queue = s.rd.d_head;  // head | tail copy | stop copy | poll lock
+    long long *last = s.rd.d_last;
+    const long long *offs = s.rd.d_offs;
+    long long idle = s.rd.idle_ticks, life = s.rd.life_ticks;
+    SynResidentTb tbs{s.rd.tbw_dev, s.rd.d_tbstate, s.rd.tb_done_dev, s.rd.tb_ok_dev, s.h_cbok_dev};
+    void *args[] = {&cp, &ring, &cap, &ctl, &dn, &queue, &last, &offs, &a, &idle, &life, &tbs};
+    DeviceScope ds(s.device);
+    if (hipLaunchKernel(syn::resident_kernel(), dim3(s.rs.grid), dim3(512), args, s.rd.lds, s.be.stream) != hipSuccess ||
+        hipEventRecord(s.be.ev, s.be.stream) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    return SYNDROME_OK;
 }
 
 }  // namespace
@@ -385,6 +315,7 @@ int32_t SyndromeServer::launch_batch(Lane &ln, std::vector<int> &batch)
     return SYNDROME_OK;
 }
 
+
 void SyndromeServer::run()
 {
     DeviceScope ds(device);
@@ -394,7 +325,7 @@ void SyndromeServer::run()
         for (Lane &ln : lanes) {
             if (ln.busy && hipEventQuery(ln.done) == hipSuccess) {
                 const double t = now_ns();
-                std::lock_guard<ServerMutex> lk(mu);
+                std::lock_guard<syn::ServerMutex> lk(mu);
                 for (size_t k = 0; k < ln.slots.size(); ++k) {
                     const int s = ln.slots[k];
                     h_it[s] = ln.h_iters[k];
@@ -413,7 +344,7 @@ void SyndromeServer::run()
             if (ln.busy) continue;
             batch.clear();
             if (n_submitted.load(std::memory_order_acquire) > 0) {
-                std::lock_guard<ServerMutex> lk(mu);
+                std::lock_guard<syn::ServerMutex> lk(mu);
                 while (!submitted.empty() && (int)batch.size() < max_batch) {
                     const int s = submitted.front();
                     submitted.pop_front();
@@ -424,8 +355,8 @@ void SyndromeServer::run()
             }
             if (batch.empty()) break;
             if (int32_t rc = launch_batch(ln, batch)) {
-                std::lock_guard<ServerMutex> lk(mu);
-                error = rc;
+                std::lock_guard<syn::ServerMutex> lk(mu);
+                error.store(rc);
                 for (int s : batch) {  // report the failure per code block
                     st[s].state = 4;
                     st[s].t_done = now_ns();
@@ -456,35 +387,35 @@ int32_t SyndromeServer::init_resident()
         return SYNDROME_DEVICE_ERROR;
     const size_t room = (size_t)lds_limit - fa.sharedSizeBytes;
     rs.cws.resize(codes.size());
-    rs.lds = 0;
+    rd.lds = 0;
     for (size_t c = 0; c < codes.size(); ++c) {
         int k = std::min(SYN_WORK_MAX, std::max(1, 512 / codes[c].z));
         while (k > 1 && group_bytes(codes[c], k) > room) --k;
         if (group_bytes(codes[c], k) > room) return SYNDROME_INVALID_PARAM;
         rs.cws[c] = k;
-        rs.lds = std::max(rs.lds, group_bytes(codes[c], k));
+        rd.lds = std::max(rd.lds, group_bytes(codes[c], k));
     }
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, rs.lds) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, rd.lds) != hipSuccess || per_cu < 1)
         return SYNDROME_DEVICE_ERROR;
     rs.grid = per_cu * cus;
-    rs.idle_ticks = (long long)(RESIDENT_IDLE_US * khz / 1000.0);
-    rs.life_ticks = (long long)(RESIDENT_LIFE_S * khz * 1000.0);
+    rd.idle_ticks = (long long)(RESIDENT_IDLE_US * khz / 1000.0);
+    rd.life_ticks = (long long)(RESIDENT_LIFE_S * khz * 1000.0);
     rs.cap = RING_ITEMS;
     std::vector<long long> offs(slots);
     for (int i = 0; i < slots; ++i) offs[i] = (long long)i * NMAX;
     if (hipHostMalloc((void **)&rs.ring, sizeof(SynWork) * rs.cap, HOST_FLAGS) != hipSuccess ||
         hipHostMalloc((void **)&rs.ctl, sizeof(SynResidentCtl), HOST_FLAGS) != hipSuccess ||
         hipHostMalloc((void **)&rs.done, 4 * (size_t)rs.cap, HOST_FLAGS) != hipSuccess ||
-        hipHostGetDevicePointer((void **)&rs.ring_dev, rs.ring, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void **)&rs.ctl_dev, rs.ctl, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void **)&rs.done_dev, rs.done, 0) != hipSuccess ||
-        hipMalloc((void **)&rs.d_head, SYN_RESIDENT_QUEUE_BYTES) != hipSuccess ||
-        hipMalloc((void **)&rs.d_last, 64) != hipSuccess ||
-        hipMalloc((void **)&rs.d_offs, 8 * (size_t)slots) != hipSuccess ||
-        hipMemcpy(rs.d_offs, offs.data(), 8 * (size_t)slots, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(rs.d_head, 0, SYN_RESIDENT_QUEUE_BYTES) != hipSuccess || hipMemset(rs.d_last, 0, 64) != hipSuccess ||
-        hipStreamCreateWithFlags(&rs.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&rs.ev, hipEventDisableTiming) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rd.ring_dev, rs.ring, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rd.ctl_dev, rs.ctl, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rd.done_dev, rs.done, 0) != hipSuccess ||
+        hipMalloc((void **)&rd.d_head, SYN_RESIDENT_QUEUE_BYTES) != hipSuccess ||
+        hipMalloc((void **)&rd.d_last, 64) != hipSuccess ||
+        hipMalloc((void **)&rd.d_offs, 8 * (size_t)slots) != hipSuccess ||
+        hipMemcpy(rd.d_offs, offs.data(), 8 * (size_t)slots, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(rd.d_head, 0, SYN_RESIDENT_QUEUE_BYTES) != hipSuccess || hipMemset(rd.d_last, 0, 64) != hipSuccess ||
+        hipStreamCreateWithFlags(&be.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&be.ev, hipEventDisableTiming) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess)
         return SYNDROME_DEVICE_ERROR;
     memset(rs.ring, 0, sizeof(SynWork) * rs.cap);
@@ -494,185 +425,17 @@ int32_t SyndromeServer::init_resident()
     if (hipHostMalloc((void **)&rs.tbw, sizeof(SynTbWork) * rs.tb_cap, HOST_FLAGS) != hipSuccess ||
         hipHostMalloc((void **)&rs.tb_done, 4 * (size_t)rs.tb_cap, HOST_FLAGS) != hipSuccess ||
         hipHostMalloc((void **)&rs.tb_ok, 4 * (size_t)rs.tb_cap, HOST_FLAGS) != hipSuccess ||
-        hipHostGetDevicePointer((void **)&rs.tbw_dev, rs.tbw, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void **)&rs.tb_done_dev, rs.tb_done, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void **)&rs.tb_ok_dev, rs.tb_ok, 0) != hipSuccess ||
-        hipMalloc((void **)&rs.d_tbstate, sizeof(SynTbState) * rs.tb_cap) != hipSuccess ||
-        hipMemset(rs.d_tbstate, 0, sizeof(SynTbState) * rs.tb_cap) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        hipHostGetDevicePointer((void **)&rd.tbw_dev, rs.tbw, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rd.tb_done_dev, rs.tb_done, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&rd.tb_ok_dev, rs.tb_ok, 0) != hipSuccess ||
+        hipMalloc((void **)&rd.d_tbstate, sizeof(SynTbState) * rs.tb_cap) != hipSuccess ||
+        hipMemset(rd.d_tbstate, 0, sizeof(SynTbState) * rs.tb_cap) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
         return SYNDROME_DEVICE_ERROR;
     memset(rs.tbw, 0, sizeof(SynTbWork) * rs.tb_cap);
     memset(rs.tb_done, 0, 4 * (size_t)rs.tb_cap);
     memset(rs.tb_ok, 0, 4 * (size_t)rs.tb_cap);
-    rs.tbr.resize(rs.tb_cap);
-    for (int i = (int)rs.tb_cap - 1; i >= 0; --i) rs.tb_free.push_back(i);
+    init_tb_ring(rs.tb_cap);
     return SYNDROME_OK;
-}
-
-int32_t SyndromeServer::launch_resident()
-{
-    SynLaunch a{};
-    a.hard = h_hard_dev;
-    a.iters = h_it_dev;
-    a.iterations = iterations;
-    a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD;  // SYN_F_ALIGNED per item
-    a.beta = beta;
-    const SynCode *cp = d_codes;
-    const SynWork *ring = rs.ring_dev;
-    unsigned cap = rs.cap;
-    SynResidentCtl *ctl = rs.ctl_dev;
-    unsigned *dn = rs.done_dev;
-    void *queue = rs.d_head;  // head | tail copy | stop copy | poll lock
-    long long *last = rs.d_last;
-    const long long *offs = rs.d_offs;
-    long long idle = rs.idle_ticks, life = rs.life_ticks;
-    SynResidentTb tbs{rs.tbw_dev, rs.d_tbstate, rs.tb_done_dev, rs.tb_ok_dev};
-    void *args[] = {&cp, &ring, &cap, &ctl, &dn, &queue, &last, &offs, &a, &idle, &life, &tbs};
-    if (hipLaunchKernel(syn::resident_kernel(), dim3(rs.grid), dim3(512), args, rs.lds, rs.stream) != hipSuccess ||
-        hipEventRecord(rs.ev, rs.stream) != hipSuccess)
-        return SYNDROME_DEVICE_ERROR;
-    rs.running = true;
-    batches++;
-    return SYNDROME_OK;
-}
-
-// Resident dispatcher: completions by polling each outstanding item's done
-// word; pending submissions grouped by code into items (no waiting: an item
-// holds what is pending now) and published; the grid relaunched whenever
-// it has left while items are outstanding.  Once the grid is seen finished,
-// every item it claimed has its done word set, so items still outstanding
-// after the scan that follows were never claimed.
-void SyndromeServer::run_resident()
-{
-    DeviceScope ds(device);
-    std::vector<int> pend;
-    std::vector<std::vector<int>> by_code(codes.size());
-    while (!stop.load()) {
-        bool progressed = false;
-        const bool grid_done = rs.running && hipEventQuery(rs.ev) == hipSuccess;
-        if (grid_done) rs.running = false;
-        // The grid claims items in publish order and holds at most `grid` of
-        // them at once, so past the (grid + 1)-th unfinished item nothing
-        // has been claimed yet: the scan stops there.
-        {
-            int open_seen = 0;
-            double t = 0;
-            std::unique_lock<ServerMutex> lk(mu, std::defer_lock);
-            for (size_t k = 0; k < rs.outstanding.size() && open_seen <= rs.grid; ++k) {
-                Outstanding &o = rs.outstanding[k];
-                if (o.fin) continue;
-                if (__atomic_load_n(rs.done + o.id % rs.cap, __ATOMIC_ACQUIRE) != o.id + 1u) {
-                    ++open_seen;
-                    continue;
-                }
-                if (!lk.owns_lock()) {
-                    t = now_ns();
-                    lk.lock();
-                }
-                for (int j = 0; j < o.n; ++j) {
-                    st[o.slot[j]].state = 4;
-                    st[o.slot[j]].t_done = t;
-                    if (st[o.slot[j]].tb < 0) done.push_back(o.slot[j]);  // TB blocks are delivered per TB
-                }
-                decoded += o.n;
-                o.fin = true;
-                progressed = true;
-            }
-            if (lk.owns_lock()) sync_counts();
-            while (!rs.outstanding.empty() && rs.outstanding.front().fin) rs.outstanding.pop_front();
-        }
-        if (n_tb_out.load(std::memory_order_acquire) > 0) {
-            std::lock_guard<ServerMutex> lk(mu);
-            for (size_t k = 0; k < rs.tb_outstanding.size();) {
-                const int ti = rs.tb_outstanding[k];
-                bool blocks_done = true;  // and every block's item seen complete, so release_tb can free them
-                for (int sl : rs.tbr[ti].slots) blocks_done = blocks_done && st[sl].state == 4;
-                if (blocks_done && __atomic_load_n(rs.tb_done + ti, __ATOMIC_ACQUIRE) == rs.tbw[ti].id + 1u) {
-                    rs.tbr[ti].state = 4;
-                    rs.tbr[ti].t_done = now_ns();
-                    rs.tb_done_q.push_back(ti);
-                    rs.tb_outstanding[k] = rs.tb_outstanding.back();
-                    rs.tb_outstanding.pop_back();
-                    progressed = true;
-                } else {
-                    ++k;
-                }
-            }
-            sync_counts();
-        }
-        pend.clear();
-        if (n_submitted.load(std::memory_order_acquire) > 0) {
-            std::lock_guard<ServerMutex> lk(mu);
-            while (!submitted.empty()) {
-                pend.push_back(submitted.front());
-                submitted.pop_front();
-            }
-            sync_counts();
-        }
-        if (!pend.empty()) {
-            for (int sl : pend) by_code[st[sl].code].push_back(sl);
-            std::vector<int> back;  // what does not fit in the ring now
-            for (int sl : pend) {
-                std::vector<int> &v = by_code[st[sl].code];
-                if (v.empty()) continue;
-                const int per = rs.cws[st[sl].code];
-                size_t j = 0;
-                for (; j < v.size(); j += (size_t)per) {
-                    if (rs.outstanding.size() + 1 >= rs.cap) break;
-                    const int cnt = (int)std::min(v.size() - j, (size_t)per);
-                    SynWork &wk = rs.ring[rs.published % rs.cap];
-                    Outstanding o{rs.published, cnt, {}, false};
-                    int aligned = 1;
-                    for (int q = 0; q < cnt; ++q) {
-                        const Slot &x = st[v[j + q]];
-                        wk.slot[q] = v[j + q];
-                        wk.nfill[q] = x.nfill;
-                        wk.src[q] = x.src;
-                        wk.tb[q] = x.tb;
-                        wk.tb_ci[q] = x.tb_ci;
-                        aligned &= ((uintptr_t)x.src & 15u) == 0;
-                        o.slot[q] = v[j + q];
-                    }
-                    wk.code = st[v[j]].code;
-                    wk.count = cnt;
-                    wk.aligned = aligned;
-                    rs.outstanding.push_back(o);
-                    ++rs.published;
-                }
-                for (; j < v.size(); ++j) back.push_back(v[j]);
-                v.clear();
-            }
-            {
-                std::lock_guard<ServerMutex> lk(mu);
-                for (int sl : pend) st[sl].state = 3;
-                for (auto it = back.rbegin(); it != back.rend(); ++it) {
-                    st[*it].state = 2;
-                    submitted.push_front(*it);
-                }
-                sync_counts();
-            }
-            __atomic_store_n(&rs.ctl->tail, rs.published, __ATOMIC_RELEASE);
-            progressed = true;
-        }
-        if (!rs.running && !rs.outstanding.empty()) {
-            if (int32_t rc = launch_resident()) {
-                std::lock_guard<ServerMutex> lk(mu);
-                error = rc;
-                for (const Outstanding &o : rs.outstanding)
-                    for (int j = 0; j < (o.fin ? 0 : o.n); ++j) {
-                        st[o.slot[j]].state = 4;
-                        st[o.slot[j]].t_done = now_ns();
-                        h_it[o.slot[j]] = -1;
-                        done.push_back(o.slot[j]);
-                    }
-                rs.outstanding.clear();
-                sync_counts();
-            }
-            progressed = true;
-        }
-        if (!progressed) std::this_thread::yield();
-    }
-    __atomic_store_n(&rs.ctl->stop, 1u, __ATOMIC_RELEASE);
-    if (rs.running) (void)hipEventSynchronize(rs.ev);
 }
 
 extern "C" {
@@ -709,6 +472,8 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
             hipHostMalloc((void **)&s->h_llr, n * 4, HOST_FLAGS) != hipSuccess ||
             hipHostMalloc((void **)&s->h_hard, n, HOST_FLAGS) != hipSuccess ||
             hipHostMalloc((void **)&s->h_it, 4 * (size_t)s->slots, HOST_FLAGS) != hipSuccess ||
+            hipHostMalloc((void **)&s->h_cbok, 4 * (size_t)s->slots, HOST_FLAGS) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&s->h_cbok_dev, s->h_cbok, 0) != hipSuccess ||
             hipHostGetDevicePointer((void **)&s->h_llr_dev, s->h_llr, 0) != hipSuccess ||
             hipHostGetDevicePointer((void **)&s->h_hard_dev, s->h_hard, 0) != hipSuccess ||
             hipHostGetDevicePointer((void **)&s->h_it_dev, s->h_it, 0) != hipSuccess) {
@@ -716,8 +481,8 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
             return SYNDROME_DEVICE_ERROR;
         }
         memset(s->h_it, 0, 4 * (size_t)s->slots);
-        s->st.resize(s->slots);
-        for (int i = s->slots - 1; i >= 0; --i) s->free_list.push_back(i);
+        memset(s->h_cbok, 0, 4 * (size_t)s->slots);
+        s->init_slots(s->slots);
         if (streams == 0) {
             s->resident = true;
             if (int32_t rc = s->init_resident()) {
@@ -725,7 +490,10 @@ int32_t syndrome_server_create(float offset_beta, size_t iterations, size_t hbm_
                 return rc;
             }
             SyndromeServer *raw = s.get();
-            s->worker = std::thread([raw] { raw->run_resident(); });
+            s->worker = std::thread([raw] {
+                DeviceScope ds(raw->device);
+                raw->run_resident();
+            });
             *out = s.release();
             return SYNDROME_OK;
         }
@@ -766,18 +534,19 @@ void syndrome_server_destroy(SyndromeServer *s)
             for (void *p : {ln.d_desc, (void *)ln.d_head})
                 if (p) (void)hipFree(p);
         }
-        SyndromeServer::Resident &r = s->rs;
+        auto &r = s->rs;
+        auto &d = s->rd;
         if (r.ctl) __atomic_store_n(&r.ctl->stop, 1u, __ATOMIC_RELEASE);
-        if (r.stream) (void)hipStreamSynchronize(r.stream);
-        if (r.ev) (void)hipEventDestroy(r.ev);
-        if (r.stream) (void)hipStreamDestroy(r.stream);
+        if (s->be.stream) (void)hipStreamSynchronize(s->be.stream);
+        if (s->be.ev) (void)hipEventDestroy(s->be.ev);
+        if (s->be.stream) (void)hipStreamDestroy(s->be.stream);
         for (void *p : {(void *)r.ring, (void *)r.ctl, (void *)r.done, (void *)r.tbw, (void *)r.tb_done, (void *)r.tb_ok})
             if (p) (void)hipHostFree(p);
-        for (void *p : {(void *)r.d_head, (void *)r.d_last, (void *)r.d_offs, (void *)r.d_tbstate})
+        for (void *p : {(void *)d.d_head, (void *)d.d_last, (void *)d.d_offs, (void *)d.d_tbstate})
             if (p) (void)hipFree(p);
         for (void *p : {(void *)s->d_llr, (void *)s->d_codes})
             if (p) (void)hipFree(p);
-        for (void *p : {(void *)s->h_llr, (void *)s->h_hard, (void *)s->h_it})
+        for (void *p : {(void *)s->h_llr, (void *)s->h_hard, (void *)s->h_it, (void *)s->h_cbok})
             if (p) (void)hipHostFree(p);
     } catch (...) {
     }
@@ -789,21 +558,7 @@ int32_t syndrome_server_acquire(SyndromeServer *s, int32_t base_graph, size_t z,
 {
     return guard([&]() -> int32_t {
         if (!s || !out_llr || !out_slot) return SYNDROME_NULL_POINTER;
-        const int zi = z <= 384 ? z_index((int)z) : -1;
-        if ((base_graph != 0 && base_graph != 1) || zi < 0) return SYNDROME_INVALID_PARAM;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (s->free_list.empty()) return SYNDROME_BUSY;
-        const int slot = s->free_list.back();
-        s->free_list.pop_back();
-        SyndromeServer::Slot &sl = s->st[slot];
-        sl.code = base_graph * 51 + zi;
-        sl.n = (base_graph == 0 ? 68 : 52) * (int)z;
-        sl.nfill = (int)std::min(n_filler, (size_t)INT_MAX);
-        sl.state = 1;
-        sl.src = s->h_llr_dev + (size_t)slot * NMAX;
-        *out_llr = s->h_llr + (size_t)slot * NMAX;
-        *out_slot = (uint32_t)slot;
-        return SYNDROME_OK;
+        return s->acquire(base_graph, z, n_filler, out_llr, out_slot);
     });
 }
 
@@ -811,15 +566,7 @@ int32_t syndrome_server_submit(SyndromeServer *s, uint32_t slot, uint64_t tag)
 {
     return guard([&]() -> int32_t {
         if (!s) return SYNDROME_NULL_POINTER;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (slot >= (uint32_t)s->slots || s->st[slot].state != 1) return SYNDROME_INVALID_PARAM;
-        SyndromeServer::Slot &sl = s->st[slot];
-        sl.tag = tag;
-        sl.t_submit = now_ns();
-        sl.state = 2;
-        s->submitted.push_back((int)slot);
-        s->sync_counts();
-        return SYNDROME_OK;
+        return s->submit(slot, nullptr, tag);
     });
 }
 
@@ -827,16 +574,7 @@ int32_t syndrome_server_submit_from(SyndromeServer *s, uint32_t slot, const floa
 {
     return guard([&]() -> int32_t {
         if (!s || !src) return SYNDROME_NULL_POINTER;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (slot >= (uint32_t)s->slots || s->st[slot].state != 1) return SYNDROME_INVALID_PARAM;
-        SyndromeServer::Slot &sl = s->st[slot];
-        sl.src = src;
-        sl.tag = tag;
-        sl.t_submit = now_ns();
-        sl.state = 2;
-        s->submitted.push_back((int)slot);
-        s->sync_counts();
-        return SYNDROME_OK;
+        return s->submit(slot, src, tag);
     });
 }
 
@@ -844,22 +582,7 @@ int32_t syndrome_server_try_recv(SyndromeServer *s, SyndromeServed *out)
 {
     return guard([&]() -> int32_t {
         if (!s || !out) return SYNDROME_NULL_POINTER;
-        if (s->n_done.load(std::memory_order_acquire) == 0) return SYNDROME_BUSY;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (s->done.empty()) return SYNDROME_BUSY;
-        const int slot = s->done.front();
-        s->done.pop_front();
-        s->sync_counts();
-        SyndromeServer::Slot &sl = s->st[slot];
-        sl.state = 5;
-        out->slot = (uint32_t)slot;
-        out->tag = sl.tag;
-        out->iterations = s->h_it[slot];
-        out->hard = s->h_hard + (size_t)slot * NMAX;
-        out->n = (size_t)sl.n;
-        out->submit_ns = sl.t_submit;
-        out->done_ns = sl.t_done;
-        return out->iterations < 0 ? SYNDROME_DEVICE_ERROR : SYNDROME_OK;
+        return s->try_recv(out);
     });
 }
 
@@ -867,12 +590,7 @@ int32_t syndrome_server_release(SyndromeServer *s, uint32_t slot)
 {
     return guard([&]() -> int32_t {
         if (!s) return SYNDROME_NULL_POINTER;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (slot >= (uint32_t)s->slots || (s->st[slot].state != 5 && s->st[slot].state != 1) || s->st[slot].tb >= 0)
-            return SYNDROME_INVALID_PARAM;
-        s->st[slot].state = 0;
-        s->free_list.push_back((int)slot);
-        return SYNDROME_OK;
+        return s->release(slot);
     });
 }
 
@@ -882,30 +600,7 @@ int32_t syndrome_server_acquire_many(SyndromeServer *s, const int32_t *base_grap
 {
     return guard([&]() -> int32_t {
         if (!s || !out_acquired || (n && (!base_graph || !z || !out_slots))) return SYNDROME_NULL_POINTER;
-        *out_acquired = 0;
-        std::vector<int> code(n), zi(n);
-        for (size_t i = 0; i < n; ++i) {
-            zi[i] = z[i] > 0 && z[i] <= 384 ? z_index(z[i]) : -1;
-            if ((base_graph[i] != 0 && base_graph[i] != 1) || zi[i] < 0 || (n_filler && n_filler[i] < 0))
-                return SYNDROME_INVALID_PARAM;
-            code[i] = base_graph[i] * 51 + zi[i];
-        }
-        std::lock_guard<ServerMutex> lk(s->mu);
-        size_t k = 0;
-        for (; k < n && !s->free_list.empty(); ++k) {
-            const int slot = s->free_list.back();
-            s->free_list.pop_back();
-            SyndromeServer::Slot &sl = s->st[slot];
-            sl.code = code[k];
-            sl.n = (base_graph[k] == 0 ? 68 : 52) * z[k];
-            sl.nfill = n_filler ? n_filler[k] : 0;
-            sl.state = 1;
-            sl.src = s->h_llr_dev + (size_t)slot * NMAX;
-            if (out_llr) out_llr[k] = s->h_llr + (size_t)slot * NMAX;
-            out_slots[k] = (uint32_t)slot;
-        }
-        *out_acquired = k;
-        return k == 0 && n ? SYNDROME_BUSY : SYNDROME_OK;
+        return s->acquire_many(base_graph, z, n_filler, n, out_llr, out_slots, out_acquired);
     });
 }
 
@@ -914,19 +609,7 @@ int32_t syndrome_server_submit_many(SyndromeServer *s, const uint32_t *slots, co
 {
     return guard([&]() -> int32_t {
         if (!s || (n && !slots)) return SYNDROME_NULL_POINTER;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (!claim_slots(s, slots, n, {1}, -2, false)) return SYNDROME_INVALID_PARAM;
-        const double t = now_ns();
-        for (size_t i = 0; i < n; ++i) {
-            SyndromeServer::Slot &sl = s->st[slots[i]];
-            if (srcs && srcs[i]) sl.src = srcs[i];
-            sl.tag = tags ? tags[i] : 0;
-            sl.t_submit = t;
-            sl.state = 2;
-            s->submitted.push_back((int)slots[i]);
-        }
-        s->sync_counts();
-        return SYNDROME_OK;
+        return s->submit_many(slots, srcs, tags, n);
     });
 }
 
@@ -934,30 +617,7 @@ int32_t syndrome_server_try_recv_many(SyndromeServer *s, SyndromeServed *out, si
 {
     return guard([&]() -> int32_t {
         if (!s || !out_n || (cap && !out)) return SYNDROME_NULL_POINTER;
-        *out_n = 0;
-        if (s->n_done.load(std::memory_order_acquire) == 0) return SYNDROME_BUSY;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        size_t k = 0;
-        bool failed = false;
-        for (; k < cap && !s->done.empty(); ++k) {
-            const int slot = s->done.front();
-            s->done.pop_front();
-            SyndromeServer::Slot &sl = s->st[slot];
-            sl.state = 5;
-            SyndromeServed &o = out[k];
-            o.slot = (uint32_t)slot;
-            o.tag = sl.tag;
-            o.iterations = s->h_it[slot];
-            o.hard = s->h_hard + (size_t)slot * NMAX;
-            o.n = (size_t)sl.n;
-            o.submit_ns = sl.t_submit;
-            o.done_ns = sl.t_done;
-            failed = failed || o.iterations < 0;
-        }
-        s->sync_counts();
-        *out_n = k;
-        if (!k) return SYNDROME_BUSY;
-        return failed ? SYNDROME_DEVICE_ERROR : SYNDROME_OK;
+        return s->try_recv_many(out, cap, out_n);
     });
 }
 
@@ -965,13 +625,7 @@ int32_t syndrome_server_release_many(SyndromeServer *s, const uint32_t *slots, s
 {
     return guard([&]() -> int32_t {
         if (!s || (n && !slots)) return SYNDROME_NULL_POINTER;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (!claim_slots(s, slots, n, {1, 5}, -3, true)) return SYNDROME_INVALID_PARAM;
-        for (size_t i = 0; i < n; ++i) {
-            s->st[slots[i]].state = 0;
-            s->free_list.push_back((int)slots[i]);
-        }
-        return SYNDROME_OK;
+        return s->release_many(slots, n);
     });
 }
 
@@ -980,45 +634,7 @@ int32_t syndrome_server_submit_tb(SyndromeServer *s, const uint32_t *slots, cons
 {
     return guard([&]() -> int32_t {
         if (!s || !slots || !tb_out) return SYNDROME_NULL_POINTER;
-        if (!s->resident || num_cb == 0 || num_cb > (size_t)s->slots || tb_size == 0 || tb_size > (size_t)INT_MAX)
-            return SYNDROME_INVALID_PARAM;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        const SyndromeServer::Slot &f = s->st[slots[0] < (uint32_t)s->slots ? slots[0] : 0];
-        for (size_t i = 0; i < num_cb; ++i) {
-            if (slots[i] >= (uint32_t)s->slots) return SYNDROME_INVALID_PARAM;
-            const SyndromeServer::Slot &x = s->st[slots[i]];
-            if (x.state != 1 || x.code != f.code || x.nfill != f.nfill) return SYNDROME_INVALID_PARAM;
-            for (size_t j = 0; j < i; ++j)
-                if (slots[j] == slots[i]) return SYNDROME_INVALID_PARAM;
-        }
-        const SynCode &c = s->codes[f.code];
-        const long long kp = (long long)(c.n_b - c.m_b) * c.z - f.nfill;
-        const long long p = num_cb > 1 ? kp - 24 : kp;
-        if (p < 1 || (long long)num_cb * p < (long long)tb_size + 24) return SYNDROME_INVALID_PARAM;
-        if (s->rs.tb_free.empty()) return SYNDROME_BUSY;
-        const int ti = s->rs.tb_free.back();
-        s->rs.tb_free.pop_back();
-        const unsigned id = s->rs.tb_next++;
-        s->rs.tbw[ti] = SynTbWork{id, (int)num_cb, (int)p, (int)tb_size, tb_out};
-        SyndromeServer::Resident::TbRec &r = s->rs.tbr[ti];
-        r.slots.assign(slots, slots + num_cb);
-        r.tag = tag;
-        r.tb_size = tb_size;
-        r.t_submit = now_ns();
-        r.state = 2;
-        for (size_t i = 0; i < num_cb; ++i) {
-            SyndromeServer::Slot &x = s->st[slots[i]];
-            if (srcs && srcs[i]) x.src = srcs[i];
-            x.tb = ti;
-            x.tb_ci = (int)i;
-            x.tag = tag;
-            x.t_submit = r.t_submit;
-            x.state = 2;
-            s->submitted.push_back((int)slots[i]);
-        }
-        s->rs.tb_outstanding.push_back(ti);
-        s->sync_counts();
-        return SYNDROME_OK;
+        return s->submit_tb(slots, srcs, num_cb, tb_size, tb_out, tag);
     });
 }
 
@@ -1026,25 +642,15 @@ int32_t syndrome_server_try_recv_tb(SyndromeServer *s, SyndromeServedTb *out)
 {
     return guard([&]() -> int32_t {
         if (!s || !out) return SYNDROME_NULL_POINTER;
-        if (s->n_tb_done.load(std::memory_order_acquire) == 0) return SYNDROME_BUSY;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (s->rs.tb_done_q.empty()) return SYNDROME_BUSY;
-        const int ti = s->rs.tb_done_q.front();
-        s->rs.tb_done_q.pop_front();
-        s->sync_counts();
-        SyndromeServer::Resident::TbRec &r = s->rs.tbr[ti];
-        r.state = 5;
-        int its = 0;
-        for (int sl : r.slots) its = std::max(its, s->h_it[sl]);
-        out->tb = (uint32_t)ti;
-        out->tag = r.tag;
-        out->crc_ok = __atomic_load_n(s->rs.tb_ok + ti, __ATOMIC_ACQUIRE) ? 1 : 0;
-        out->max_iterations = its;
-        out->tb_size = r.tb_size;
-        out->num_cb = r.slots.size();
-        out->submit_ns = r.t_submit;
-        out->done_ns = r.t_done;
-        return SYNDROME_OK;
+        return s->try_recv_tb(out);
+    });
+}
+
+int32_t syndrome_server_tb_block_crc(SyndromeServer *s, uint32_t tb, uint8_t *cb_ok, size_t cap, size_t *out_n)
+{
+    return guard([&]() -> int32_t {
+        if (!s || !out_n || (cap && !cb_ok)) return SYNDROME_NULL_POINTER;
+        return s->tb_block_crc(tb, cb_ok, cap, out_n);
     });
 }
 
@@ -1052,18 +658,7 @@ int32_t syndrome_server_release_tb(SyndromeServer *s, uint32_t tb)
 {
     return guard([&]() -> int32_t {
         if (!s) return SYNDROME_NULL_POINTER;
-        std::lock_guard<ServerMutex> lk(s->mu);
-        if (!s->resident || tb >= s->rs.tb_cap || s->rs.tbr[tb].state != 5) return SYNDROME_INVALID_PARAM;
-        SyndromeServer::Resident::TbRec &r = s->rs.tbr[tb];
-        for (int sl : r.slots) {
-            s->st[sl].tb = -1;
-            s->st[sl].state = 0;
-            s->free_list.push_back(sl);
-        }
-        r.slots.clear();
-        r.state = 0;
-        s->rs.tb_free.push_back((int)tb);
-        return SYNDROME_OK;
+        return s->release_tb(tb);
     });
 }
 
@@ -1074,7 +669,7 @@ int32_t syndrome_server_stats(const SyndromeServer *s, int64_t *out_batches, int
         if (out_batches) *out_batches = s->batches.load();
         if (out_decoded) *out_decoded = s->decoded.load();
         if (out_slots) *out_slots = s->slots;
-        return SYNDROME_OK;
+        return s->error.load();
     });
 }
 
@@ -1226,9 +821,9 @@ int32_t syndrome_server_loadgen_tb(SyndromeServer *s, const float *pool, const i
                                    const int32_t *cb_bg, const int32_t *cb_z, const int32_t *cb_nfill,
                                    const int32_t *tb_first_cb, const int32_t *tb_num_cb, const int32_t *tb_size,
                                    uint8_t *tb_out, const int64_t *tb_out_off, size_t n_tb, double tb_rate,
-                                   double duration_s, uint64_t seed, double *tb_lat_ns, size_t lat_cap,
-                                   size_t *n_tbs, size_t *n_cbs, double *info_bps, int64_t *crc_fails,
-                                   int64_t *waits)
+                                   double duration_s, uint64_t seed, double *tb_lat_ns, uint8_t *tb_lat_ok,
+                                   size_t lat_cap, size_t *n_tbs, size_t *n_cbs, double *info_bps,
+                                   double *goodput_bps, int64_t *crc_fails, int64_t *waits)
 {
     return guard([&]() -> int32_t {
         if (!s || !pool || !cb_off || !cb_bg || !cb_z || !tb_first_cb || !tb_num_cb || !tb_size || !tb_out ||
@@ -1252,7 +847,7 @@ int32_t syndrome_server_loadgen_tb(SyndromeServer *s, const float *pool, const i
         const double t0 = now_ns(), t_end = t0 + duration_s * 1e9;
         double next = t0, last_done = t0;
         size_t ntb = 0, ncb = 0, next_tb = 0;
-        long long info = 0, fails = 0, wait_count = 0, outstanding = 0;
+        long long info = 0, good = 0, fails = 0, wait_count = 0, outstanding = 0;
         std::vector<const float *> srcs;
         for (;;) {
             const double now = now_ns();
@@ -1305,12 +900,17 @@ int32_t syndrome_server_loadgen_tb(SyndromeServer *s, const float *pool, const i
                 pending.clear();
             }
             SyndromeServedTb r;
-            while (syndrome_server_try_recv_tb(s, &r) == SYNDROME_OK) {
+            for (;;) {
+                const int32_t rc = syndrome_server_try_recv_tb(s, &r);
+                if (rc == SYNDROME_BUSY) break;
+                if (rc != SYNDROME_OK) return rc;  // a transport block the server could not decode
                 const auto &m = meta[r.tag];
                 if (ntb < lat_cap && tb_lat_ns) tb_lat_ns[ntb] = r.done_ns - m.first;
+                if (ntb < lat_cap && tb_lat_ok) tb_lat_ok[ntb] = r.crc_ok ? 1 : 0;
                 ++ntb;
                 ncb += r.num_cb;
                 info += tb_size[m.second];
+                if (r.crc_ok) good += tb_size[m.second];
                 if (!r.crc_ok) ++fails;
                 last_done = std::max(last_done, r.done_ns);
                 --outstanding;
@@ -1322,6 +922,7 @@ int32_t syndrome_server_loadgen_tb(SyndromeServer *s, const float *pool, const i
         if (n_tbs) *n_tbs = ntb;
         if (n_cbs) *n_cbs = ncb;
         if (info_bps) *info_bps = (double)info / ((last_done - t0) * 1e-9);
+        if (goodput_bps) *goodput_bps = (double)good / ((last_done - t0) * 1e-9);
         if (crc_fails) *crc_fails = fails;
         if (waits) *waits = wait_count;
         return SYNDROME_OK;

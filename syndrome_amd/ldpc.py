@@ -642,6 +642,13 @@ class Server:
         check(rc, "server_try_recv_tb")
         return r.tb, r.tag, bool(r.crc_ok), r.max_iterations, r.submit_ns, r.done_ns
 
+    def tb_block_crc(self, tb, num_cb):
+        """Per-code-block CRC-24B outcome of a received transport block (DecodeReport::cb_crc)."""
+        out = np.zeros(max(num_cb, 1), np.uint8)
+        n = C.c_size_t(0)
+        check(lib().syndrome_server_tb_block_crc(self._h, tb, _ptr(out), out.size, C.byref(n)), "server_tb_block_crc")
+        return out[:n.value].astype(bool)
+
     def release_tb(self, tb):
         check(lib().syndrome_server_release_tb(self._h, tb), "server_release_tb")
 
@@ -702,7 +709,8 @@ def serve_loadgen_tb(server, pool, cb_off, cb_bg, cb_z, cb_nfill, tb_first, tb_n
     """Transport-block load generator (syndrome_server_loadgen_tb) over a
     resident Server.  tb_out: device uint8 tensor or HostPool holding every
     pool transport block's payload at tb_out_off.  Returns (tb_latency_ns,
-    code_blocks, info_bits_per_s, crc_failures, refused_acquires)."""
+    code_blocks, info_bits_per_s, crc_failures, refused_acquires,
+    goodput_bits_per_s, tb_crc_ok per latency entry)."""
     pool_ptr = pool.ptr if isinstance(pool, HostPool) else pool.data_ptr()
     out_ptr = tb_out.ptr if isinstance(tb_out, HostPool) else tb_out.data_ptr()
     arrs = [np.ascontiguousarray(x, t) for x, t in ((cb_off, np.int64), (cb_bg, np.int32), (cb_z, np.int32),
@@ -710,11 +718,14 @@ def serve_loadgen_tb(server, pool, cb_off, cb_bg, cb_z, cb_nfill, tb_first, tb_n
                                                      (tb_size, np.int32), (tb_out_off, np.int64))]
     cap = 2_000_000
     lat = np.zeros(cap, np.float64)
+    ok = np.zeros(cap, np.uint8)
     n_tb, n_cb = C.c_size_t(), C.c_size_t()
-    bps = C.c_double()
+    bps, good = C.c_double(), C.c_double()
     fails, waits = C.c_int64(), C.c_int64()
     check(lib().syndrome_server_loadgen_tb(server._h, pool_ptr, *[_ptr(a) for a in arrs[:7]], out_ptr, _ptr(arrs[7]),
-                                           len(arrs[4]), tb_rate, duration_s, seed, _ptr(lat), cap, C.byref(n_tb),
-                                           C.byref(n_cb), C.byref(bps), C.byref(fails), C.byref(waits)),
+                                           len(arrs[4]), tb_rate, duration_s, seed, _ptr(lat), _ptr(ok), cap,
+                                           C.byref(n_tb), C.byref(n_cb), C.byref(bps), C.byref(good), C.byref(fails),
+                                           C.byref(waits)),
           "server_loadgen_tb")
-    return lat[:min(n_tb.value, cap)], n_cb.value, bps.value, fails.value, waits.value
+    m = min(n_tb.value, cap)
+    return lat[:m], n_cb.value, bps.value, fails.value, waits.value, good.value, ok[:m].astype(bool)

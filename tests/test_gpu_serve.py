@@ -274,7 +274,8 @@ def test_server_transport_blocks_match_oracle(oracle):
     while (len(got_tb) < len(tbs) or len(got_cb) < min(len(loose), len(tbs))) and time.time() < t_end:
         r = srv.try_recv_tb()
         if r is not None:
-            got_tb[r[1] - 1000] = r
+            ti = r[1] - 1000
+            got_tb[ti] = r + (srv.tb_block_crc(r[0], tbs[ti][1]),)
             srv.release_tb(r[0])
         r = srv.try_recv()
         if r is not None:
@@ -290,9 +291,10 @@ def test_server_transport_blocks_match_oracle(oracle):
             hards.append(rh)
             its.append(ri)
         n = hards[0].size
-        ok, _, _ = oracle.tb_check(np.concatenate(hards), np.arange(c) * n, c, kp, a)
-        _, tag, crc_ok, max_it, _, _ = got_tb[ti]
+        ok, cb_ok, _ = oracle.tb_check(np.concatenate(hards), np.arange(c) * n, c, kp, a)
+        _, tag, crc_ok, max_it, _, _, cb_crc = got_tb[ti]
         assert crc_ok == ok and max_it == max(its), f"TB {ti}: crc_ok {crc_ok} (oracle {ok}), iterations"
+        assert np.array_equal(cb_crc, cb_ok.astype(bool)), f"TB {ti}: per-block CRC-24B {cb_crc} (oracle {cb_ok})"
         p = kp - 24 if c > 1 else kp
         stream = np.concatenate([h[:p] for h in hards])[:a]
         assert np.array_equal(out.array[out_off[ti]:out_off[ti] + a], stream), f"TB {ti}: payload bits"
@@ -344,15 +346,17 @@ def test_loadgen_transport_blocks():
     out_off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
     d_out = torch.zeros(int(sum(sizes)), dtype=torch.uint8, device="cuda")
     srv = Server(0.5, 8, hbm_budget_bytes=64 << 20, max_batch=64, streams=0)
-    lat, ncb, bps, fails, _ = serve_loadgen_tb(srv, d, offs, bg, z, nf, first, num, sizes, d_out, out_off, 0, 0.3)
+    lat, ncb, bps, fails, _, good, okf = serve_loadgen_tb(srv, d, offs, bg, z, nf, first, num, sizes, d_out, out_off, 0,
+                                                         0.3)
     say(f"  saturating: {len(lat)} TBs, {ncb} blocks, {bps / 1e9:.3f} Gbit/s, {fails} CRC failures")
     assert len(lat) > 0 and fails == 0 and (lat > 0).all()
+    assert okf.all() and good == pytest.approx(bps)  # every transport block passed: goodput = throughput
     torch.cuda.synchronize()
     got = d_out.cpu().numpy()
     for t, tb in enumerate(payloads):
         assert np.array_equal(got[out_off[t]:out_off[t] + sizes[t]], tb), f"TB {t} payload"
-    lat, ncb, bps, fails, _ = serve_loadgen_tb(srv, d, offs, bg, z, nf, first, num, sizes, d_out, out_off, 150.0, 0.3,
-                                               seed=2)
+    lat, ncb, bps, fails, _, good, okf = serve_loadgen_tb(srv, d, offs, bg, z, nf, first, num, sizes, d_out, out_off,
+                                                         150.0, 0.3, seed=2)
     say(f"  poisson: {len(lat)} TBs, p50 {np.median(lat) / 1e6:.3f} ms")
     assert len(lat) > 10 and fails == 0
     assert ncb == sum(num[i % 5] for i in range(len(lat)))
