@@ -157,3 +157,37 @@ def test_concurrent_decodes_share_one_handle(oracle):
         used, a, er, hd = results[j]
         assert used == ri and np.array_equal(f32_bits(a), f32_bits(rp)), f"decode {j}"
         assert np.array_equal(f32_bits(er), f32_bits(re)) and np.array_equal(hd, rh), f"decode {j}"
+
+
+def test_device_quantizer_known_answers_and_oracle(oracle):
+    """syndrome_ldpc_quantize_i16_device against quantize.rs's known answers
+    (quantize.rs:372-437: round-half-away, saturation at +-127, NaN -> 0,
+    -inf -> -127) and against the oracle's quantize_llr_i16 on a large vector
+    holding every special value, at two scales, on a caller stream."""
+    import syndrome_amd
+    from syndrome_amd.ldpc import quantize_llr_i16_device
+    syndrome_amd.lib()
+    cases = [([0, 1, -1, 1e9, -np.inf], 8.0, [0, 8, -8, 127, -127]),
+             ([np.nan], 8.0, [0]),
+             ([1.0, -0.25, 99.0], 8.0, [8, -2, 127]),
+             ([0.0625, -0.0625, 0.1875], 8.0, [1, -1, 2])]
+    for x, scale, want in cases:
+        d_x = torch.tensor(x, dtype=torch.float32, device="cuda")
+        d_q = torch.full((len(x),), 999, dtype=torch.int16, device="cuda")
+        quantize_llr_i16_device(d_x, d_q, scale)
+        torch.cuda.synchronize()
+        assert d_q.cpu().tolist() == want, (x, scale)
+    rng = np.random.default_rng(23)
+    n = (1 << 20) + 37  # ragged tail past any vector width
+    x = _fuzz_llrs(rng, n)
+    x[: n // 2] = rng.normal(0, 20, n // 2).astype(np.float32)
+    s = torch.cuda.Stream()
+    for scale in (8.0, 3.5):
+        say(f"[quantize] {n} LLRs, scale {scale}, caller stream")
+        d_x = torch.from_numpy(x).cuda()
+        d_q = torch.empty(n, dtype=torch.int16, device="cuda")
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            quantize_llr_i16_device(d_x, d_q, scale, stream=s.cuda_stream)
+        s.synchronize()
+        assert np.array_equal(d_q.cpu().numpy(), oracle.quantize_i16(x, scale))
