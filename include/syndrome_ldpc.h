@@ -103,9 +103,20 @@ int32_t syndrome_ldpc_encode_5g(const SyndromeLdpcEncoder *encoder, const uint8_
  * ====================================================================== */
 
 /* QcLdpcDecoder::from_raw_edges (qc_ldpc.rs:412-455, 555-565): entries are
- * (row_block, col_block, shift) triples, shift already < z.  Duplicate
- * (row, col) pairs are rejected with INVALID_PARAM (the GPU schedule relies
- * on each variable appearing at most once per layer). */
+ * (row_block, col_block, shift) triples, shift already < z, z in 1..=384;
+ * any number of entries, repeated (row, col) pairs allowed, m_b and n_b >= 0.
+ * INVALID_PARAM for an entry out of range, and for sizes this library does
+ * not index: m_b*z, n_b*z or n_entries*z at or above 2^31.  A code with more
+ * than 46 row blocks, 68 column blocks or 316 edges, or with a column twice in
+ * one layer, decodes on the runtime-structured wide kernel (one codeword per
+ * workgroup; a posterior too large for LDS stays in HBM; a layer with a
+ * repeated column applies its updates edge by edge, as the reference does).
+ * Its batch decodes take stream-ordered device memory for their workspace
+ * (hipMallocAsync on the call's stream); such a code has no encoder
+ * (encode_batch / encode_batch_device return INVALID_PARAM).  decode_5g /
+ * decode_5g_i8 on a code with n_b < m_b or n_b < 2 return PANIC, as the
+ * reference's init_5g_llr panics there (qc_ldpc.rs:720-731); the batch
+ * INIT_5G flag is then INVALID_PARAM. */
 int32_t syndrome_ldpc_decoder_create_raw(const uint8_t *rows, const uint8_t *cols, const int16_t *shifts,
                                          size_t n_entries, size_t num_row_blocks, size_t num_col_blocks, size_t z,
                                          float offset_beta, SyndromeLdpcDecoder **out_decoder);

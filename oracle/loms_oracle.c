@@ -82,7 +82,7 @@ int orc_code_3gpp(orc_code_t *c, int bg, int z)
 int orc_code_raw(orc_code_t *c, const int *rows, const int *cols, const int *shifts, int n, int m_b,
                  int n_b, int z)
 {
-    if (z <= 0 || z > ORC_Z_MAX || m_b <= 0 || m_b > ORC_MAX_ROWS || n_b <= 0 || n_b > ORC_MAX_COLS ||
+    if (z <= 0 || z > ORC_Z_MAX || m_b < 0 || m_b > ORC_MAX_ROWS || n_b < 0 || n_b > ORC_MAX_COLS ||
         n < 0 || n > ORC_MAX_EDGES)
         return ORC_INVALID_PARAM;
     for (int i = 0; i < n; ++i)

@@ -21,9 +21,12 @@
 extern "C" {
 #endif
 
-#define ORC_MAX_ROWS 46
-#define ORC_MAX_COLS 68
-#define ORC_MAX_EDGES 316
+/* Table capacity of this restatement: the reference's from_raw_edges has no
+ * limit but u8 row / column indices; these hold every 3GPP and 802.11 code
+ * and the raw codes the tests build (rows past 255 stay empty). */
+#define ORC_MAX_ROWS 512
+#define ORC_MAX_COLS 1024
+#define ORC_MAX_EDGES 4096
 #define ORC_Z_MAX 384
 
 /* QcLdpcParams (src/qc_ldpc.rs:233-253). */
@@ -45,7 +48,8 @@ typedef struct {
 
 /* QcLdpcParams::new (qc_ldpc.rs:335-366): bg 0 = BG1, 1 = BG2. */
 int orc_code_3gpp(orc_code_t *code, int bg, int z);
-/* QcLdpcParams::from_raw_edges (qc_ldpc.rs:412-455). */
+/* QcLdpcParams::from_raw_edges (qc_ldpc.rs:412-455): any m_b, n_b >= 0 up
+ * to the capacity above, repeated (row, col) entries kept in order. */
 int orc_code_raw(orc_code_t *code, const int *rows, const int *cols, const int *shifts, int n_entries,
                  int m_b, int n_b, int z);
 /* wifi_ldpc_decoder / wifi_ldpc_matrix (wifi_ldpc_tables.rs:455-529). */

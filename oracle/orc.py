@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libloms_ref.so")
 
-MAX_ROWS, MAX_EDGES = 46, 316
+MAX_ROWS, MAX_EDGES = 512, 4096  # loms_oracle.h ORC_MAX_ROWS / ORC_MAX_EDGES
 
 
 class Code(C.Structure):

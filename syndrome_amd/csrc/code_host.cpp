@@ -28,22 +28,35 @@ int lifting_set(long long z)
 }
 
 // Group edges by row block, keeping the given order inside a row
-// (build_layout, src/qc_ldpc.rs:281-318).
+// (build_layout, src/qc_ldpc.rs:281-318): per-row offsets into the grouped
+// edge list, and the maximum row degree.
+void group_rows(const int *rows, const int *cols, const int *shifts, size_t n, int m_b, std::vector<int> &off,
+                std::vector<int> &gc, std::vector<int> &gs, int &max_deg)
+{
+    std::vector<int> deg((size_t)m_b, 0), fill((size_t)m_b, 0);
+    for (size_t i = 0; i < n; ++i) deg[(size_t)rows[i]]++;
+    off.assign((size_t)m_b + 1, 0);
+    max_deg = 0;
+    for (int r = 0; r < m_b; ++r) {
+        off[(size_t)r + 1] = off[(size_t)r] + deg[(size_t)r];
+        max_deg = std::max(max_deg, deg[(size_t)r]);
+    }
+    gc.assign(n, 0);
+    gs.assign(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t p = (size_t)off[(size_t)rows[i]] + (size_t)fill[(size_t)rows[i]]++;
+        gc[p] = cols[i];
+        gs[p] = shifts[i];
+    }
+}
+
 void layout(SynCode *c, const int *rows, const int *cols, const int *shifts, int n)
 {
-    int deg[SYN_MAX_ROWS] = {0}, fill[SYN_MAX_ROWS] = {0};
-    for (int i = 0; i < n; ++i) deg[rows[i]]++;
-    c->layer_off[0] = 0;
-    c->max_deg = 0;
-    for (int r = 0; r < c->m_b; ++r) {
-        c->layer_off[r + 1] = c->layer_off[r] + deg[r];
-        c->max_deg = std::max(c->max_deg, deg[r]);
-    }
-    for (int i = 0; i < n; ++i) {
-        const int p = c->layer_off[rows[i]] + fill[rows[i]]++;
-        c->cols[p] = cols[i];
-        c->shifts[p] = shifts[i];
-    }
+    std::vector<int> off, gc, gs;
+    group_rows(rows, cols, shifts, (size_t)n, c->m_b, off, gc, gs, c->max_deg);
+    std::copy(off.begin(), off.end(), c->layer_off);
+    std::copy(gc.begin(), gc.end(), c->cols);
+    std::copy(gs.begin(), gs.end(), c->shifts);
     c->n_edges = n;
 }
 
@@ -76,25 +89,74 @@ int build_code_3gpp(int bg, long long z, SynCode *out)
 }
 
 int build_code_raw(const int *rows, const int *cols, const int *shifts, size_t n, long long m_b, long long n_b,
-                   long long z, SynCode *out)
+                   long long z, SynCode *out, WideCode *wide)
 {
-    // src/qc_ldpc.rs:418-440, plus the table-size limits this build has.
-    if (z <= 0 || z > SYN_Z_MAX) return SYN_INVALID_PARAM;
-    if (m_b <= 0 || m_b > SYN_MAX_ROWS || n_b <= 0 || n_b > SYN_MAX_COLS || n > SYN_MAX_EDGES)
-        return SYN_INVALID_PARAM;
+    // src/qc_ldpc.rs:418-440
+    if (z <= 0 || z > SYN_Z_MAX || m_b < 0 || n_b < 0) return SYN_INVALID_PARAM;
     for (size_t i = 0; i < n; ++i) {
         if (rows[i] < 0 || rows[i] >= m_b) return SYN_INVALID_PARAM;
         if (cols[i] < 0 || cols[i] >= n_b) return SYN_INVALID_PARAM;
         if (shifts[i] < 0 || shifts[i] >= z) return SYN_INVALID_PARAM;
-        for (size_t j = 0; j < i; ++j)
-            if (rows[j] == rows[i] && cols[j] == cols[i]) return SYN_INVALID_PARAM;
+    }
+    // 32-bit element indices on the device
+    const long long lim = 0x7fffffffLL;
+    if (m_b > lim / z || n_b > lim / z || n > (size_t)(lim / z)) return SYN_INVALID_PARAM;
+    // rows past the last one named hold no edges: the tables cover rows
+    // 0 .. rmax only, whatever m_b and n_b are
+    int rmax = -1, cmax = -1;
+    for (size_t i = 0; i < n; ++i) rmax = std::max(rmax, rows[i]), cmax = std::max(cmax, cols[i]);
+    const int nr = rmax + 1;
+    std::vector<int> off, gc, gs;
+    int max_deg = 0;
+    group_rows(rows, cols, shifts, n, nr, off, gc, gs, max_deg);
+    // a layer with a column block twice shares variables between its rows
+    std::vector<unsigned char> dup((size_t)nr, 0);
+    bool any_dup = false;
+    {
+        std::vector<int> seen((size_t)cmax + 1, -1);
+        for (int r = 0; r < nr; ++r)
+            for (int e = off[(size_t)r]; e < off[(size_t)r + 1]; ++e) {
+                if (seen[(size_t)gc[(size_t)e]] == (int)r) dup[(size_t)r] = 1, any_dup = true;
+                seen[(size_t)gc[(size_t)e]] = (int)r;
+            }
     }
     memset(out, 0, sizeof(*out));
-    out->kind = SYN_KIND_RAW;
     out->z = (int)z;
     out->m_b = (int)m_b;
     out->n_b = (int)n_b;
-    layout(out, rows, cols, shifts, (int)n);
+    out->n_edges = (int)n;
+    out->max_deg = max_deg;
+    const bool inline_form =
+        m_b >= 1 && m_b <= SYN_MAX_ROWS && n_b >= 1 && n_b <= SYN_MAX_COLS && n <= SYN_MAX_EDGES && !any_dup;
+    if (inline_form) {
+        out->kind = SYN_KIND_RAW;
+        std::copy(off.begin(), off.end(), out->layer_off);
+        for (long long r = nr; r < m_b; ++r) out->layer_off[r + 1] = (int)n;
+        std::copy(gc.begin(), gc.end(), out->cols);
+        std::copy(gs.begin(), gs.end(), out->shifts);
+        if (wide) *wide = WideCode{};
+        return SYN_OK;
+    }
+    if (!wide) return SYN_INVALID_PARAM;
+    out->kind = SYN_KIND_WIDE;
+    // the layers that hold edges, in order
+    WideCode w;
+    w.layer_off.push_back(0);
+    for (int r = 0; r < nr; ++r) {
+        if (off[(size_t)r + 1] == off[(size_t)r]) continue;
+        w.layer_off.push_back(off[(size_t)r + 1]);
+        w.dup.push_back(dup[(size_t)r]);
+    }
+    w.cols = std::move(gc);
+    w.shifts = std::move(gs);
+    w.hdr.z = (int)z;
+    w.hdr.m_b = (int)m_b;
+    w.hdr.n_b = (int)n_b;
+    w.hdr.n_layers = (int)w.dup.size();
+    w.hdr.n_edges = (int)n;
+    w.hdr.max_deg = max_deg;
+    w.hdr.any_dup = any_dup ? 1 : 0;
+    *wide = std::move(w);
     return SYN_OK;
 }
 

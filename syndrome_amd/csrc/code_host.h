@@ -11,15 +11,34 @@ namespace syn {
 
 // QcLdpcParams::new (src/qc_ldpc.rs:335-366): 3GPP BG1 (bg = 0) or BG2 (bg = 1).
 int build_code_3gpp(int bg, long long z, SynCode *out);
-// QcLdpcParams::from_raw_edges (src/qc_ldpc.rs:412-455); duplicate
-// (row, col) pairs are rejected.
+// The host copy of a code in the wide form (SynWideCode's tables).
+struct WideCode {
+    std::vector<int> layer_off, cols, shifts;
+    std::vector<unsigned char> dup;
+    SynWideCode hdr{};  // table pointers unset
+};
+
+// QcLdpcParams::from_raw_edges (src/qc_ldpc.rs:412-455) over its whole
+// domain: z in 1..=384, every row < m_b, column < n_b, 0 <= shift < z, any
+// number of entries, repeated (row, col) pairs allowed.  A code that fits
+// SynCode's inline tables (at most SYN_MAX_ROWS rows, SYN_MAX_COLS columns
+// and SYN_MAX_EDGES edges, no column twice in a layer) comes back in that
+// form and `wide` is cleared; any other goes to `wide` (kind SYN_KIND_WIDE
+// in `out`, which then carries the sizes only), or is rejected when `wide`
+// is null.  Sizes whose m_b * z, n_b * z or n * z reach 2^31 are rejected.
 int build_code_raw(const int *rows, const int *cols, const int *shifts, size_t n, long long m_b, long long n_b,
-                   long long z, SynCode *out);
+                   long long z, SynCode *out, WideCode *wide = nullptr);
 // wifi_ldpc_matrix (src/wifi_ldpc_tables.rs:455-466).
 int build_code_wifi(long long z, long long rate_num, long long rate_den, SynCode *out);
 
 inline size_t code_n(const SynCode &c) { return (size_t)c.n_b * c.z; }
-inline size_t code_k(const SynCode &c) { return (size_t)(c.n_b - c.m_b) * c.z; }
+// info_bit_count_5g (src/qc_ldpc.rs:736-739); n_b >= m_b for every code the
+// 5G entry points accept (code_5g_ok).
+inline size_t code_k(const SynCode &c) { return c.n_b >= c.m_b ? (size_t)(c.n_b - c.m_b) * c.z : 0; }
+// init_5g_llr (src/qc_ldpc.rs:711-732) is defined on the code: the filler
+// range needs n_b >= m_b and the 2Z punctured positions N >= 2Z.  The
+// reference panics on any other code.
+inline bool code_5g_ok(const SynCode &c) { return c.n_b >= c.m_b && c.n_b >= 2; }
 inline size_t code_ez(const SynCode &c) { return (size_t)c.n_edges * c.z; }
 inline size_t code_layer_buf(const SynCode &c) { return (size_t)c.max_deg * c.z; }
 

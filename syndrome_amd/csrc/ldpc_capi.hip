@@ -25,6 +25,9 @@ namespace syn {
 int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, const SynLaunch &a, int block,
                   size_t lds, hipStream_t stream);
 int launch_quantize(const float *in, int16_t *out, size_t n, float scale, hipStream_t stream);
+WideShape wide_shape(const SynWideCode &w, bool i8, int lds_limit);
+int launch_wide(bool i8, const SynWideCode &w, const SynLaunch &a, const WideShape &s, char *work,
+                hipStream_t stream);
 size_t encode_lds_bytes(const SynCode &c, const SynEncCode &e, const void *info, const void *cw);
 int launch_encode(const SynCode *d_code, const SynEncCode *d_enc, int z, int k, int n, size_t lds,
                   const uint8_t *info, uint8_t *cw, long long count, hipStream_t stream);
@@ -36,6 +39,11 @@ struct SyndromeLdpcDecoder {
     int device = 0;
     int lds_limit = 0;
     SynCode *d_code = nullptr;
+    // a raw code outside the inline tables (code.kind == SYN_KIND_WIDE):
+    // host tables, and the header whose table pointers address d_wide
+    std::unique_ptr<syn::WideCode> wide;
+    SynWideCode wide_dev{};
+    void *d_wide = nullptr;
     // host-buffer entry points: serialised per handle, one staging area
     std::mutex mu;
     hipStream_t stream = nullptr;
@@ -84,19 +92,63 @@ struct DeviceScope {
     }
 };
 
-int32_t make_decoder(const SynCode &code, float beta, SyndromeLdpcDecoder **out)
+void free_decoder(SyndromeLdpcDecoder *d)
 {
-    std::unique_ptr<SyndromeLdpcDecoder> d(new SyndromeLdpcDecoder());
+    DeviceScope ds(d->device);
+    if (d->d_code) (void)hipFree(d->d_code);
+    if (d->d_wide) (void)hipFree(d->d_wide);
+    if (d->d_stage) (void)hipFree(d->d_stage);
+    if (d->d_enc) (void)hipFree(d->d_enc);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+}
+
+struct DecoderOwner {
+    void operator()(SyndromeLdpcDecoder *d) const
+    {
+        free_decoder(d);
+        delete d;
+    }
+};
+
+// The wide form's tables in one device allocation: layer offsets, columns,
+// shifts, then the dup flags.
+int32_t upload_wide(SyndromeLdpcDecoder *d)
+{
+    const syn::WideCode &w = *d->wide;
+    const size_t b_off = w.layer_off.size() * 4, b_cols = w.cols.size() * 4, b_dup = w.dup.size();
+    std::vector<char> h(b_off + 2 * b_cols + b_dup + 16);
+    memcpy(h.data(), w.layer_off.data(), b_off);
+    if (b_cols) {
+        memcpy(h.data() + b_off, w.cols.data(), b_cols);
+        memcpy(h.data() + b_off + b_cols, w.shifts.data(), b_cols);
+    }
+    if (b_dup) memcpy(h.data() + b_off + 2 * b_cols, w.dup.data(), b_dup);
+    if (hipMalloc(&d->d_wide, h.size()) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    if (hipMemcpy(d->d_wide, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    char *base = (char *)d->d_wide;
+    d->wide_dev = w.hdr;
+    d->wide_dev.layer_off = (const int *)base;
+    d->wide_dev.cols = (const int *)(base + b_off);
+    d->wide_dev.shifts = (const int *)(base + b_off + b_cols);
+    d->wide_dev.dup = (const unsigned char *)(base + b_off + 2 * b_cols);
+    return SYNDROME_OK;
+}
+
+int32_t make_decoder(const SynCode &code, float beta, SyndromeLdpcDecoder **out,
+                     std::unique_ptr<syn::WideCode> wide = nullptr)
+{
+    std::unique_ptr<SyndromeLdpcDecoder, DecoderOwner> d(new SyndromeLdpcDecoder());
     d->code = code;
     d->beta = beta;
+    d->wide = std::move(wide);
     if (hipGetDevice(&d->device) != hipSuccess) return SYNDROME_DEVICE_ERROR;
     if (hipDeviceGetAttribute(&d->lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, d->device) != hipSuccess)
         return SYNDROME_DEVICE_ERROR;
     if (hipMalloc((void **)&d->d_code, sizeof(SynCode)) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-    if (hipMemcpy(d->d_code, &d->code, sizeof(SynCode), hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(d->d_code);
+    if (hipMemcpy(d->d_code, &d->code, sizeof(SynCode), hipMemcpyHostToDevice) != hipSuccess)
         return SYNDROME_DEVICE_ERROR;
-    }
+    if (d->code.kind == SYN_KIND_WIDE)
+        if (int32_t rc = upload_wide(d.get())) return rc;
     *out = d.release();
     return SYNDROME_OK;
 }
@@ -121,6 +173,13 @@ bool use_fast(const SynCode &c, bool plain)
 int32_t launch_shape(const SyndromeLdpcDecoder *d, bool i8, bool plain, Shape *s)
 {
     const SynCode &c = d->code;
+    if (c.kind == SYN_KIND_WIDE) {
+        const WideShape w = syn::wide_shape(d->wide_dev, i8, d->lds_limit);
+        s->cws = 1;
+        s->block = w.block;
+        s->lds = w.lds;
+        return SYNDROME_OK;
+    }
     const size_t n = syn::code_n(c), elem = i8 ? 2 : 4;
     const int max_lanes = use_fast(c, plain) ? 256 : 1024;
     // fast kernels keep a Z*size pad in front of the posterior (ldpc_fast.h)
@@ -137,10 +196,11 @@ int32_t launch_shape(const SyndromeLdpcDecoder *d, bool i8, bool plain, Shape *s
 
 int32_t run_batch(const SyndromeLdpcDecoder *d, bool i8, const void *in, void *post, void *edge, uint8_t *hard,
                   int32_t *iters, size_t batch, size_t iterations, size_t n_filler, float scale, int16_t app_clamp,
-                  uint32_t flags, hipStream_t stream, float *trace = nullptr)
+                  uint32_t flags, hipStream_t stream, float *trace = nullptr, char *work = nullptr)
 {
     if (batch == 0) return SYNDROME_OK;
     if (batch > (size_t)INT_MAX / 2) return SYNDROME_INVALID_PARAM;
+    if ((flags & SYNDROME_BATCH_INIT_5G) && !syn::code_5g_ok(d->code)) return SYNDROME_INVALID_PARAM;
     // a traced decode runs on the plain kernel (same bits; the trace store
     // stays out of the register-resident kernels' register budget)
     const bool plain = (flags & SYNDROME_BATCH_PLAIN) != 0 || (trace && !i8);
@@ -182,7 +242,17 @@ int32_t run_batch(const SyndromeLdpcDecoder *d, bool i8, const void *in, void *p
         a.beta_q = !(b - b == 0.f) ? 127 : (int)std::min(127.f, std::max(0.f, roundf(b)));
     }
     DeviceScope ds(d->device);
-    return syn::launch_decode(d->code.kind, i8, fast, d->d_code, d->code.z, a, s.block, s.lds, stream);
+    if (d->code.kind != SYN_KIND_WIDE)
+        return syn::launch_decode(d->code.kind, i8, fast, d->d_code, d->code.z, a, s.block, s.lds, stream);
+    // the wide form: its workspace is the caller's (staged decodes) or
+    // stream-ordered memory of this call's own
+    const WideShape w = syn::wide_shape(d->wide_dev, i8, d->lds_limit);
+    const size_t ws = w.work_stride * batch;
+    const bool own = ws && !work;
+    if (own && hipMallocAsync((void **)&work, ws, stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    const int rc = syn::launch_wide(i8, d->wide_dev, a, w, work, stream);
+    if (own && hipFreeAsync(work, stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    return rc ? SYNDROME_DEVICE_ERROR : SYNDROME_OK;
 }
 
 int32_t ensure_stage(SyndromeLdpcDecoder *d, size_t bytes)
@@ -199,21 +269,25 @@ int32_t ensure_stage(SyndromeLdpcDecoder *d, size_t bytes)
 }
 
 // Where one codeword's staging lives in a device buffer: posterior (elem
-// bytes per variable), R messages, hard decisions, iteration count, then
-// the optional trace; each part 256-byte aligned.
+// bytes per variable), R messages, hard decisions, iteration count, the
+// workspace of a wide-form code, then the optional trace; each part
+// 256-byte aligned.
 struct StageLayout {
-    size_t off_edge, off_hard, off_iter, off_trace, total;
+    size_t off_edge, off_hard, off_iter, off_work, off_trace, total;
 };
 
-StageLayout stage_layout(const SynCode &c, bool i8, size_t trace_floats)
+StageLayout stage_layout(const SyndromeLdpcDecoder *d, bool i8, size_t trace_floats)
 {
+    const SynCode &c = d->code;
     const size_t n = syn::code_n(c), ez = syn::code_ez(c);
     const size_t pe = i8 ? 2 : 4, me = i8 ? 1 : 4;
+    const size_t work = c.kind == SYN_KIND_WIDE ? syn::wide_shape(d->wide_dev, i8, d->lds_limit).work_stride : 0;
     StageLayout L;
     L.off_edge = (n * pe + 255) & ~(size_t)255;
     L.off_hard = L.off_edge + ((ez * me + 255) & ~(size_t)255);
     L.off_iter = L.off_hard + ((n + 255) & ~(size_t)255);
-    L.off_trace = L.off_iter + 256;
+    L.off_work = L.off_iter + 256;
+    L.off_trace = L.off_work + work;
     L.total = L.off_trace + trace_floats * 4;
     return L;
 }
@@ -230,13 +304,13 @@ int32_t host_decode_at(SyndromeLdpcDecoder *d, char *base, hipStream_t stream, b
     const SynCode &c = d->code;
     const size_t n = syn::code_n(c), ez = syn::code_ez(c);
     const size_t pe = i8 ? 2 : 4, me = i8 ? 1 : 4;
-    const StageLayout L = stage_layout(c, i8, trace ? iterations * n : 0);
+    const StageLayout L = stage_layout(d, i8, trace ? iterations * n : 0);
     const uint32_t flags = (init5g ? SYNDROME_BATCH_INIT_5G : 0u) | SYNDROME_BATCH_WRITE_POST |
                            SYNDROME_BATCH_WRITE_HARD | SYNDROME_BATCH_WRITE_EDGE;
     if (hipMemcpyAsync(base, llr, n * pe, hipMemcpyHostToDevice, stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
     int32_t rc = run_batch(d, i8, base, base, base + L.off_edge, (uint8_t *)(base + L.off_hard),
                            (int32_t *)(base + L.off_iter), 1, iterations, n_filler, scale, app_clamp, flags, stream,
-                           trace ? (float *)(base + L.off_trace) : nullptr);
+                           trace ? (float *)(base + L.off_trace) : nullptr, base + L.off_work);
     if (rc) return rc;
     int32_t used = 0;
     if (hipMemcpyAsync(llr, base, n * pe, hipMemcpyDeviceToHost, stream) != hipSuccess ||
@@ -261,7 +335,7 @@ int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, si
                     size_t *out_used, float *trace = nullptr)
 {
     (void)edge_elems;
-    const StageLayout L = stage_layout(d->code, i8, trace ? iterations * syn::code_n(d->code) : 0);
+    const StageLayout L = stage_layout(d, i8, trace ? iterations * syn::code_n(d->code) : 0);
     std::lock_guard<std::mutex> lk(d->mu);
     DeviceScope ds(d->device);
     if (int32_t rc = ensure_stage(d, L.total)) return rc;
@@ -306,16 +380,23 @@ ScratchSlot *scratch_on(int device)
     return &sl;
 }
 
-// init_5g_llr on host memory (src/qc_ldpc.rs:711-732): applied when a later
-// validation step fails, so the caller's buffer is left as the reference
-// leaves it (the prologue runs before the decode's own checks).
+// init_5g_llr on host memory (src/qc_ldpc.rs:711-732, 1505-1514): applied
+// when a later validation step fails, so the caller's buffer is left as the
+// reference leaves it (the prologue runs before the decode's own checks),
+// and on a code the prologue is not defined on.  Returns false where the
+// reference panics: n_b < m_b (the filler range is out of bounds, nothing
+// written) or N < 2Z (the fillers are written, then the punctured range is
+// out of bounds).
 template <class T>
-void init_5g_host(const SynCode &c, T *buf, size_t n_filler, T filler)
+bool init_5g_host(const SynCode &c, T *buf, size_t n_filler, T filler)
 {
+    if (c.n_b < c.m_b) return false;
     const size_t k = syn::code_k(c);
     const size_t kp = n_filler >= k ? 0 : k - n_filler;
     for (size_t i = kp; i < k; ++i) buf[i] = filler;
+    if (syn::code_n(c) < 2 * (size_t)c.z) return false;
     for (size_t i = 0; i < (size_t)(2 * c.z); ++i) buf[i] = 0;
+    return true;
 }
 
 }  // namespace
@@ -342,11 +423,7 @@ void syndrome_ldpc_decoder_destroy(SyndromeLdpcDecoder *d)
 {
     if (!d) return;
     try {
-        DeviceScope ds(d->device);
-        if (d->d_code) (void)hipFree(d->d_code);
-        if (d->d_stage) (void)hipFree(d->d_stage);
-        if (d->d_enc) (void)hipFree(d->d_enc);
-        if (d->stream) (void)hipStreamDestroy(d->stream);
+        free_decoder(d);
     } catch (...) {
     }
     delete d;
@@ -369,6 +446,10 @@ int32_t syndrome_ldpc_decode_5g(SyndromeLdpcDecoder *d, float *llr, size_t llr_l
         const SynCode &c = d->code;
         const size_t n = syn::code_n(c);
         if (llr_len != n) return SYNDROME_INVALID_PARAM;  // init_5g_llr
+        if (!syn::code_5g_ok(c)) {
+            (void)init_5g_host<float>(c, llr, n_filler, 1000000.0f);
+            return SYNDROME_PANIC;
+        }
         int32_t rc = SYNDROME_OK;
         if (hard_len != n) rc = SYNDROME_INVALID_PARAM;
         else if (edge_r_len < syn::code_ez(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
@@ -391,6 +472,10 @@ int32_t syndrome_ldpc_decode_5g_static(SyndromeLdpcDecoder *d, float *llr, size_
         const SynCode &c = d->code;
         const size_t n = syn::code_n(c);
         if (llr_len != n) return SYNDROME_INVALID_PARAM;  // init_5g_llr
+        if (!syn::code_5g_ok(c)) {
+            (void)init_5g_host<float>(c, llr, n_filler, 1000000.0f);
+            return SYNDROME_PANIC;
+        }
         int32_t rc = SYNDROME_OK;
         if (hard_len != n) rc = SYNDROME_INVALID_PARAM;
         else if (edge_r_len < syn::code_ez(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
@@ -400,7 +485,7 @@ int32_t syndrome_ldpc_decode_5g_static(SyndromeLdpcDecoder *d, float *llr, size_
             return rc;
         }
         // the codeword's staging must fit the array
-        if (stage_layout(c, false, 0).total > SYNDROME_STATIC_SCRATCH_BYTES) return SYNDROME_INVALID_PARAM;
+        if (stage_layout(d, false, 0).total > SYNDROME_STATIC_SCRATCH_BYTES) return SYNDROME_INVALID_PARAM;
         ScratchSlot *sl = scratch_on(d->device);
         if (!sl) return SYNDROME_DEVICE_ERROR;
         if (sl->status) return sl->status;
@@ -460,8 +545,7 @@ int32_t syndrome_ldpc_decoder_create_raw(const uint8_t *rows, const uint8_t *col
         if (!out) return SYNDROME_NULL_POINTER;
         *out = nullptr;
         if (n > 0 && (!rows || !cols || !shifts)) return SYNDROME_NULL_POINTER;
-        if (n > SYN_MAX_EDGES || m_b > SYN_MAX_ROWS || n_b > SYN_MAX_COLS || z > SYN_Z_MAX)
-            return SYNDROME_INVALID_PARAM;
+        if (z > SYN_Z_MAX || m_b > (size_t)INT_MAX || n_b > (size_t)INT_MAX) return SYNDROME_INVALID_PARAM;
         std::vector<int> r(n), c(n), s(n);
         for (size_t i = 0; i < n; ++i) {
             r[i] = rows[i];
@@ -469,9 +553,12 @@ int32_t syndrome_ldpc_decoder_create_raw(const uint8_t *rows, const uint8_t *col
             s[i] = shifts[i];
         }
         SynCode code;
-        if (syn::build_code_raw(r.data(), c.data(), s.data(), n, (long long)m_b, (long long)n_b, (long long)z, &code))
+        std::unique_ptr<syn::WideCode> wide(new syn::WideCode());
+        if (syn::build_code_raw(r.data(), c.data(), s.data(), n, (long long)m_b, (long long)n_b, (long long)z, &code,
+                                wide.get()))
             return SYNDROME_INVALID_PARAM;
-        return make_decoder(code, offset_beta, out);
+        if (code.kind != SYN_KIND_WIDE) wide.reset();
+        return make_decoder(code, offset_beta, out, std::move(wide));
     });
 }
 
@@ -548,6 +635,10 @@ int32_t syndrome_ldpc_decode_5g_i8(SyndromeLdpcDecoder *d, int16_t *app, size_t 
         const SynCode &c = d->code;
         const size_t n = syn::code_n(c);
         if (app_len != n) return SYNDROME_INVALID_PARAM;  // qc_ldpc.rs:1499-1503
+        if (!syn::code_5g_ok(c)) {
+            (void)init_5g_host<int16_t>(c, app, n_filler, app_clamp < 0 ? (int16_t)0 : app_clamp);
+            return SYNDROME_PANIC;
+        }
         int32_t rc = SYNDROME_OK;
         if (hard_len != n) rc = SYNDROME_INVALID_PARAM;
         else if (edge_r_len < syn::code_ez(c)) rc = SYNDROME_BUFFER_TOO_SMALL;
@@ -622,6 +713,7 @@ int32_t syndrome_ldpc_encode_batch(const SyndromeLdpcDecoder *d, const uint8_t *
     return guard([&]() -> int32_t {
         if (!d || ((!info || !cws) && count)) return SYNDROME_NULL_POINTER;
         SyndromeLdpcDecoder *md = const_cast<SyndromeLdpcDecoder *>(d);
+        if (d->code.kind == SYN_KIND_WIDE) return SYNDROME_INVALID_PARAM;  // no encoder for the wide form
         std::call_once(md->enc_once, [&] { md->enc.reset(new syn::Encoder(md->code)); });
         if (!md->enc || !md->enc->ok()) return SYNDROME_INVALID_PARAM;
         const size_t k = syn::code_k(d->code), n = syn::code_n(d->code);
@@ -642,6 +734,7 @@ int32_t syndrome_ldpc_encode_batch_device(const SyndromeLdpcDecoder *d, const ui
 {
     return guard([&]() -> int32_t {
         if (!d || ((!d_info || !d_codewords) && count)) return SYNDROME_NULL_POINTER;
+        if (d->code.kind == SYN_KIND_WIDE) return SYNDROME_INVALID_PARAM;  // no encoder for the wide form
         if (count > (size_t)LLONG_MAX / syn::code_n(d->code)) return SYNDROME_INVALID_PARAM;
         SyndromeLdpcDecoder *md = const_cast<SyndromeLdpcDecoder *>(d);
         DeviceScope ds(d->device);

@@ -7,6 +7,7 @@
 //   hard decisions          : u8 [N], 0/1
 // A batch stacks codewords contiguously: codeword b starts at b*N (b*E*Z for R).
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #define SYN_MAX_ROWS 46
@@ -31,6 +32,7 @@
 #define SYN_KIND_BG1 0
 #define SYN_KIND_BG2 1
 #define SYN_KIND_RAW 2
+#define SYN_KIND_WIDE 3    // a raw code outside the inline tables below (SynWideCode)
 #define SYN_KIND_WIFI0 16  // + matrix index: the 12 IEEE 802.11 matrices (src/wifi_ldpc_tables.rs)
 
 // Runtime description of one lifted code, QcLdpcParams (src/qc_ldpc.rs:233-253).
@@ -45,6 +47,40 @@ struct SynCode {
     int layer_off[SYN_MAX_ROWS + 1];
     int cols[SYN_MAX_EDGES];
     int shifts[SYN_MAX_EDGES];  // reduced mod z
+};
+
+// A raw-edge code of any size (QcLdpcParams::from_raw_edges,
+// src/qc_ldpc.rs:412-455, whose domain is u8 row and column indices, any
+// number of edges and repeated (row, col) pairs): the tables live in device
+// memory of their own.  Only the layers holding edges are listed (an empty
+// layer changes nothing and every one of its checks holds).  dup[l] marks a
+// layer in which one column block appears more than once, so that two check
+// rows of the layer share a variable; such a layer is decoded edge by edge
+// (src/qc_ldpc.rs:1181-1204 applies the posterior deltas in edge order).
+// m_b * z, n_b * z and n_edges * z are below 2^31.
+struct SynWideCode {
+    int z;
+    int m_b;          // num_row_blocks (layers with no edges included)
+    int n_b;          // num_col_blocks
+    int n_layers;     // layers holding edges
+    int n_edges;
+    int max_deg;
+    int any_dup;
+    int pad;
+    const int *layer_off;      // [n_layers + 1] into cols / shifts
+    const int *cols;           // [n_edges]
+    const int *shifts;         // [n_edges], reduced mod z
+    const unsigned char *dup;  // [n_layers]
+};
+
+// Launch geometry of a wide decode: one codeword per workgroup of `block`
+// lanes; per codeword `work_stride` bytes of HBM workspace, holding the
+// posterior at 0 unless it sits in LDS (`lds` bytes), and from `q_off` the Q
+// messages of a dup layer.
+struct WideShape {
+    int block = 0;
+    bool p_in_lds = false;
+    size_t lds = 0, q_off = 0, work_stride = 0;
 };
 
 // Decode-call flags.

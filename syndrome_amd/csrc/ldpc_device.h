@@ -2,9 +2,10 @@
 // (MI355X).  Kernel families:
 //
 // Two kernel families, both bit-exact with the reference's layered offset
-// min-sum decode (src/qc_ldpc.rs:962-1252 f32, 1530-1787 int8):
+// min-sum decode (src/qc_ldpc.rs:962-1252 f32, 1530-1787 int8), plus the
+// wide kernels of ldpc_wide.hip for raw codes outside the inline tables:
 //
-//  * plain_*  -- runtime-structured: any (raw-edge) code, the layer loop
+//  * plain_*  -- runtime-structured: any inline-table code, the layer loop
 //                reads the code tables at run time, the check-to-variable
 //                messages R live uncompressed in HBM (the edge_r buffer),
 //                the posterior of each codeword is staged in LDS.
@@ -23,8 +24,9 @@
 // the same code; lane t handles codeword t / Z, lifting index z = t % Z.
 // All lanes of a workgroup walk the layers in the reference's strict order
 // with a workgroup barrier between dependent layers; z positions of one
-// layer touch disjoint posteriors (each variable appears at most once per
-// layer), so no two lanes ever write the same LDS word between barriers.
+// layer touch disjoint posteriors (in these codes each variable appears at
+// most once per layer), so no two lanes ever write the same LDS word between
+// barriers.
 //
 // Exactness: every f32 operation is the reference's single IEEE operation
 // in the reference's order (q = L - R_old; L' = L + (R_new - R_old)); the

@@ -36,14 +36,35 @@ def test_decoder_create_rejects_bad_params_before_touching_the_device(L):
         assert out.value is None
     assert L.syndrome_ldpc_decoder_create(0, 384, 0.5, None) == _lib.NULL_POINTER
     assert L.syndrome_ldpc_decoder_create_wifi(27, 1, 3, 0.5, C.byref(out)) == _lib.INVALID_PARAM
-    # raw edges: shift >= z, row out of range, duplicate (row, col)
-    for rows, cols, sh in (([0], [0], [4]), ([2], [0], [0]), ([0, 0], [1, 1], [0, 1])):
+    # raw edges (from_raw_edges, qc_ldpc.rs:418-440): z outside 1..=384, shift
+    # >= z or < 0, row or column out of range are refused before any device
+    # call; so are sizes whose element counts reach 2^31 (this library's
+    # limit, the reference has none)
+    def raw(rows, cols, sh, m_b, n_b, z):
         r = np.array(rows, np.uint8)
         c = np.array(cols, np.uint8)
         s = np.array(sh, np.int16)
+        o = C.c_void_p(77)
         rc = L.syndrome_ldpc_decoder_create_raw(r.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p),
-                                                s.ctypes.data_as(C.c_void_p), len(r), 2, 3, 4, 0.5, C.byref(out))
-        assert rc == _lib.INVALID_PARAM
+                                                s.ctypes.data_as(C.c_void_p), len(r), m_b, n_b, z, 0.5, C.byref(o))
+        if rc == _lib.OK:
+            L.syndrome_ldpc_decoder_destroy(o)
+        else:
+            assert o.value is None
+        return rc
+
+    for args in (([0], [0], [4], 2, 3, 4), ([2], [0], [0], 2, 3, 4), ([0], [3], [0], 2, 3, 4),
+                 ([0], [0], [-1], 2, 3, 4), ([0], [0], [0], 2, 3, 0), ([0], [0], [0], 2, 3, 385),
+                 ([0], [0], [0], 1, 2 ** 31 // 4, 4), ([0], [0], [0], 2 ** 40, 3, 4)):
+        assert raw(*args) == _lib.INVALID_PARAM, args
+    # the rest of the domain is accepted: a repeated (row, col), more rows,
+    # columns and edges than any 3GPP graph, empty codes; without a GPU the
+    # call then fails at the device, not at validation
+    rng = np.random.default_rng(3)
+    big = (rng.integers(0, 256, 700), rng.integers(0, 256, 700), rng.integers(0, 8, 700))
+    for args in (([0, 0], [1, 1], [0, 1], 2, 3, 4), (*big, 300, 300, 8), ([], [], [], 0, 0, 5),
+                 ([], [], [], 3, 0, 5), ([0], [0], [0], 1, 1, 384)):
+        assert raw(*args) != _lib.INVALID_PARAM, args
 
 
 def test_null_decoder_is_reported_not_dereferenced(L):

@@ -19,8 +19,14 @@ Pins, by part of the oracle:
   int8 decoder must return the same posteriors, messages, hard decisions
   and iteration counts as the (reference-pinned) f32 decoder.  A dropped
   term, a wrong sign or a misplaced clamp breaks that identity.  The
-  clamp paths are pinned by saturation invariants and by the reference's
-  own documented behaviour (strong channel -> all-zero, filler pinned).
+  clamp paths are pinned by a brute-force one-row pass with every message
+  and the posterior in their clamps (it tells MSG_MAX 127 from 128), by
+  saturation invariants and by the reference's own documented behaviour
+  (strong channel -> all-zero, filler pinned).
+* raw-edge codes (from_raw_edges, qc_ldpc.rs:412-455): a hand-worked layer
+  with a repeated column (Q from the pre-layer posterior, deltas applied in
+  edge order), f32 and int8; empty rows and unused columns are inert (a 3GPP
+  code padded with them decodes bit for bit as the code itself).
 """
 import json
 import os
@@ -266,3 +272,94 @@ def test_f32_min_sum_by_brute_force(oracle):
             sign = -1 if sum(o < 0 for o in others) % 2 else 1
             assert er[e * z + zi] == np.float32(sign * mag)
             assert post[vs[e]] == np.float32(x[vs[e]] + np.float32(sign * mag))
+
+
+def test_i8_min_sum_by_brute_force_at_saturation(oracle):
+    """One layered int8 pass on a one-row toy code, every message and the
+    posterior driven into their clamps, equals the check update computed by
+    brute force (qc_ldpc.rs:1619-1744 read as the textbook rule): Q clamped
+    to +-127 (MSG_MAX), excluded-edge minimum of |Q| and sign product, offset
+    beta_q, floor at zero, then the posterior clamped to +-app_clamp."""
+    z = 4
+    rows, cols, shifts = [0, 0, 0, 0, 0], [0, 1, 2, 3, 4], [0, 1, 2, 3, 1]
+    c = oracle.code_raw(rows, cols, shifts, 1, 5, z)
+    rng = np.random.default_rng(8)
+    beta_q = 4  # round(0.5 * 8)
+    for clamp in (32767, 200, 127):
+        # mostly saturating inputs: |app| >= 127 on most edges, some small
+        x = rng.choice([-1000, -300, -128, -127, -126, 5, 126, 127, 128, 190, 640], c.n).astype(np.int16)
+        x = np.clip(x, -clamp, clamp).astype(np.int16)
+        it, post, er, hard = oracle.decode_i8(c, 0.5, x, 1, 8.0, clamp)
+        assert it == 1
+        for zi in range(z):
+            vs = [cols[e] * z + (zi + shifts[e]) % z for e in range(5)]
+            q = [max(-127, min(127, int(x[v]))) for v in vs]
+            for e in range(5):
+                others = [q[j] for j in range(5) if j != e]
+                mag = max(min(abs(o) for o in others) - beta_q, 0)
+                r = -mag if sum(o < 0 for o in others) % 2 else mag
+                assert er[e * z + zi] == r, (clamp, zi, e)
+                assert post[vs[e]] == max(-clamp, min(clamp, int(x[vs[e]]) + r)), (clamp, zi, e)
+
+
+def test_i8_saturation_brute_force_catches_a_wrong_message_clamp(oracle):
+    """The brute force above separates MSG_MAX = 127 from 128: with every
+    input past the clamp the excluded minimum is 127, so R = 127 - 4."""
+    z = 1
+    c = oracle.code_raw([0, 0, 0], [0, 1, 2], [0, 0, 0], 1, 3, z)
+    _, post, er, _ = oracle.decode_i8(c, 0.5, np.array([1000, 1000, -1000], np.int16), 1, 8.0, 32767)
+    assert list(er) == [-123, -123, 123]
+    assert list(post) == [877, 877, -877]
+
+
+def test_raw_repeated_column_worked_example(oracle):
+    """from_raw_edges keeps a repeated (row, col) (qc_ldpc.rs:412-455 has no
+    check against it).  Worked by hand from qc_ldpc.rs:1072-1204 on a Z=2
+    layer holding column 0 twice (shifts 0 and 1) and column 1 once: row 0
+    reads variables (0, 1, 2), row 1 reads (1, 0, 3).  Every Q comes from the
+    posterior before the layer, and the deltas land edge by edge, so
+    variable 0 gets 1 + (-1.5) + 0.25 and variable 1 gets 2 + 0.25 - 0.5.
+    Reading the posterior as it is being updated would flip row 1's Q on
+    edge 1 (to -0.5), and letting the last writer win would leave variable 1
+    at 1.5: both fail here."""
+    c = oracle.code_raw([0, 0, 0], [0, 0, 1], [0, 1, 0], 1, 2, 2)
+    x = np.array([1.0, 2.0, -3.0, 0.75], np.float32)
+    it, post, er, hard = oracle.decode_f32(c, 0.5, x, 1)
+    # row 0: Q = (1, 2, -3), one negative, min1 1, min2 2; row 1: Q = (2, 1, 0.75), min1 0.75, min2 1
+    assert it == 1
+    assert list(er) == [-1.5, 0.25, -0.5, 0.25, 0.5, 0.5]
+    assert list(post) == [-0.25, 1.75, -2.5, 1.25]
+    assert list(hard) == [1, 0, 1, 0]
+    # the int8 twin, app = 8 * x, beta_q = 4
+    it, app, er8, hard8 = oracle.decode_i8(c, 0.5, np.array([8, 16, -24, 6], np.int16), 1)
+    assert it == 1
+    assert list(er8) == [-12, 2, -4, 2, 4, 4]
+    assert list(app) == [-2, 14, -20, 10]
+
+
+def test_raw_empty_rows_and_columns_change_nothing(oracle):
+    """Rows past the last used one and columns no edge names are inert
+    (an empty layer has no Q, no update, and its checks hold): BG2 Z=4 given
+    as raw edges with 7 extra rows and 3 extra columns decodes exactly as
+    the 3GPP code on its variables, and leaves the extra ones untouched."""
+    base = oracle.code_3gpp(1, 4)
+    rows, cols, shifts = [], [], []
+    for r in range(base.m_b):
+        for e in range(base.layer_off[r], base.layer_off[r + 1]):
+            rows.append(r)
+            cols.append(base.cols[e])
+            shifts.append(base.shifts[e])
+    wide = oracle.code_raw(rows, cols, shifts, base.m_b + 7, base.n_b + 3, 4)
+    rng = np.random.default_rng(5)
+    for it_max in (1, 6):
+        x = rng.normal(1.0, 2.0, wide.n).astype(np.float32)
+        i0, p0, e0, h0 = oracle.decode_f32(base, 0.5, x[: base.n], it_max)
+        i1, p1, e1, h1 = oracle.decode_f32(wide, 0.5, x, it_max)
+        assert i0 == i1
+        assert np.array_equal(f32_bits(p1[: base.n]), f32_bits(p0)) and np.array_equal(p1[base.n:], x[base.n:])
+        assert np.array_equal(f32_bits(e1), f32_bits(e0))
+        a = oracle.quantize_i16(x, 8.0)
+        j0, q0, r0, _ = oracle.decode_i8(base, 0.5, a[: base.n], it_max)
+        j1, q1, r1, _ = oracle.decode_i8(wide, 0.5, a, it_max)
+        assert j0 == j1 and np.array_equal(q1[: base.n], q0) and np.array_equal(q1[base.n:], a[base.n:])
+        assert np.array_equal(r1, r0)
