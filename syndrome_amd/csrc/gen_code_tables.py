@@ -56,7 +56,7 @@ def wifi_matrices(src):
     return mats
 
 
-def main(ref_root):
+def main(ref_root, dest=None):
     with open(os.path.join(ref_root, "data", "bg_tables.json")) as f:
         tables = json.load(f)
     with open(os.path.join(ref_root, "src", "wifi_ldpc_tables.rs")) as f:
@@ -101,7 +101,8 @@ def main(ref_root):
         out.append(f"  {{{z}, {rn}, {rd}, {mb}, {nb}, {len(ents)}, SYN_WIFI_{i}_EDGES}},\n")
     out.append("};\n")
 
-    dest = os.path.join(os.path.dirname(os.path.abspath(__file__)), "code_tables.inc")
+    if dest is None:
+        dest = os.path.join(os.path.dirname(os.path.abspath(__file__)), "code_tables.inc")
     with open(dest, "w") as f:
         f.write("".join(out))
     print("wrote", dest)
