@@ -97,6 +97,18 @@ def _cpu_model():
                 "Thread(s) per core", "CPU(s)", "CPU max MHz")
         got = {k: kv[k] for k in keys if k in kv}
         if got:
+            # the kernel's own name for the part, and the AMD family's core
+            # generation (family 0x19 = Zen 3 / Zen 4, 0x1A = Zen 5)
+            try:
+                for line in open("/proc/cpuinfo"):
+                    if line.startswith("model name"):
+                        got["/proc/cpuinfo model name"] = line.split(":", 1)[1].strip()
+                        break
+            except OSError:
+                pass
+            fam = {"25": "Zen 3 / Zen 4", "26": "Zen 5"}.get(got.get("CPU family", ""))
+            if got.get("Vendor ID") == "AuthenticAMD" and fam:
+                got["core generation (from CPU family)"] = fam
             return got
     except (OSError, subprocess.SubprocessError):
         pass
@@ -508,7 +520,7 @@ def bench_uneven(torch, args):
     t8 = _timed(lambda: plan8.run(d_llr, d_post8, d_hard, d_it, args.iters, f8, quant=q), 5, torch)
     bits = float(k.sum())
     edges = np.where(bg == 0, 316, 197) * z
-    roof = valu_roofline("syn::persist_decode<false> (both launches of the plan)", "f32",
+    roof = valu_roofline("syn::persist_decode<false, *> (both launches of the plan)", "f32",
                          float((edges * its).sum()), t_dec * 1e3, float(n.sum()) * 9, "persist_f32_uneven_b32768")
     cpu = cpu8 = None
     if not args.no_cpu:

@@ -13,8 +13,9 @@
 // compile-time-Z instances for BG1 Z=384 and BG2 Z=208), its inputs read
 // straight from each block's source pointer (HBM or pinned host memory),
 // its hard decisions and iteration counts written straight into pinned host
-// memory; then a system-scope fence and the item's completion word
-// done[w % cap] = w + 1.
+// memory; then every wave waits for its own stores to complete, the
+// workgroup meets at a barrier, and lane 0 stores the item's completion word
+// done[w % cap] = w + 1 with a system-scope release (resident_protocol.h).
 //
 // A workgroup waiting for work leaves once no workgroup has claimed an item
 // for `idle_ticks` of the device wall clock (counted from the grid's start
