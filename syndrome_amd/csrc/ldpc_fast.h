@@ -80,6 +80,30 @@ constexpr int layer_begin(int l)
     return s;
 }
 
+// Edges of column c, and whether edge e's column has no other edge (a
+// degree-1 variable: BG1 columns 26..67, BG2 columns 14..51).
+template <class G>
+constexpr int col_degree(int c)
+{
+    int d = 0;
+    for (int e = 0; e < G::EDGES; ++e) d += G::col(e) == c;
+    return d;
+}
+
+template <class G>
+constexpr bool edge_deg1(int e)
+{
+    return col_degree<G>(G::col(e)) == 1;
+}
+
+template <class G>
+constexpr bool any_deg1()
+{
+    for (int e = 0; e < G::EDGES; ++e)
+        if (edge_deg1<G>(e)) return true;
+    return false;
+}
+
 // A barrier is needed after layer l unless layer l+1 touches none of the
 // columns touched since the last barrier: layers over disjoint variables
 // commute exactly, so running them without a barrier between them changes
@@ -372,7 +396,18 @@ __device__ __forceinline__ void layer_f32(const Lane &ln, const Geo<G, ZC, WM> &
 // One layer of the int8 decode (src/qc_ldpc.rs:1619-1745).  KEEPA keeps
 // |q| in registers for pass 2, which then finds the argmin instead of
 // pass 1 (one operation less per edge, D more live registers).
-template <class G, int ZC, int L, bool KEEPA, bool WM>
+//
+// D1: the degree-1 variables hold their channel value c for the whole
+// decode.  Such a variable's only edge is this row's, so (integer
+// arithmetic, no clamp reached) its posterior is A = c + R after every
+// pass and its Q = A - R_old is c itself; the decode then needs neither its
+// Q-build nor its posterior update, and c + R is written once at the end
+// (fast_group).  The caller takes this path only when every degree-1 c of
+// the workgroup has |c| <= C - 127: then |c + R| <= C for every message R
+// (|R| <= 127), so neither the saturating add nor the clamp to [-C, C] of
+// qc_ldpc.rs:1738-1742 ever changes the reference's posterior, which
+// stays c + R exactly.
+template <class G, int ZC, int L, bool KEEPA, bool WM, bool D1 = false>
 __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &geo, StateI8<G> &st, int bq,
                                          int lo, int hi, bool act)
 {
@@ -398,7 +433,9 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
         // m1 = m2 = 127 at the start an |q| above 127 moves neither min1 nor
         // min2.  It could only change "a == m1" / "a <= m1" when min1 ends
         // at 127, and then min2 == min1, so the argmin does not matter.
-        const int q = add_r_i8<D>(Av[e], o1, o2, ~os, oam, e);
+        int q;
+        if constexpr (D1 && edge_deg1<G>(B + e)) q = Av[e];  // Q = c (D1 above)
+        else q = add_r_i8<D>(Av[e], o1, o2, ~os, oam, e);
         qv[e] = q;
         qs = __builtin_amdgcn_alignbit(qs, (uint32_t)q, 31);
         const int a = q < 0 ? -q : q;
@@ -422,7 +459,11 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
             // A + (R_new - R_old) = q + R_new; the saturating add and the
             // clamp after it equal the clamp alone.
             int nv;
-            if constexpr (KEEPA) {
+            if constexpr (D1 && edge_deg1<G>(B + e)) {
+                // no posterior update (D1 above): only the argmin
+                if constexpr (KEEPA) am = (av[e] == m1) ? (uint32_t)e : am;
+                return;
+            } else if constexpr (KEEPA) {
                 const bool at = av[e] == m1;
                 am = at ? (uint32_t)e : am;
                 const int mag = at ? g2 : g1;
@@ -441,8 +482,21 @@ __device__ __forceinline__ void layer_i8(const Lane &ln, const Geo<G, ZC, WM> &g
     }
 }
 
-template <class G, int ZC, int L, class T, bool WM>
-__device__ __forceinline__ bool row_parity(const Lane &ln, const Geo<G, ZC, WM> &geo)
+// The row's R on edge e from the int8 state (the layer's g1, g2, signs and
+// argmin).
+template <class G, int L, int e>
+__device__ __forceinline__ int state_r_i8(const StateI8<G> &st)
+{
+    constexpr int D = G::deg(L);
+    const uint32_t mw = mg_get<G, L>(st), sw = sg_get<G, L>(st.sg);
+    return r_i8<D>((int)(mw & 0xffu), (int)((mw >> 8) & 0xffu), sw, (sw >> D) & 31u, e);
+}
+
+// Parity of the hard decisions of row L.  With a D1 state (the int8 layer's
+// degree-1 bypass), a degree-1 variable's posterior is its stored channel
+// value plus the row's own R on that edge.
+template <class G, int ZC, int L, class T, bool WM, class S = void>
+__device__ __forceinline__ bool row_parity(const Lane &ln, const Geo<G, ZC, WM> &geo, const S *st = nullptr)
 {
     constexpr int D = G::deg(L), B = layer_begin<G>(L);
     T v[D];  // every load of the row issued before the first compare
@@ -451,16 +505,25 @@ __device__ __forceinline__ bool row_parity(const Lane &ln, const Geo<G, ZC, WM> 
         v[e] = *(const T *)geo.template addr<B + e, sizeof(T)>(ln);
     });
     bool p = false;
-    SFor<0, D>::run([&](auto ec) { p ^= v[decltype(ec)::value] < (T)0; });
+    SFor<0, D>::run([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        if constexpr (!std::is_void_v<S>) {
+            if constexpr (edge_deg1<G>(B + e)) {
+                p ^= ((int)v[e] + state_r_i8<G, L, e>(*st)) < 0;
+                return;
+            }
+        }
+        p ^= v[e] < (T)0;
+    });
     return p;
 }
 
-template <class G, int ZC, int L0, int L1, class T, bool WM>
-__device__ __forceinline__ bool rows_parity(const Lane &ln, const Geo<G, ZC, WM> &geo)
+template <class G, int ZC, int L0, int L1, class T, bool WM, class S = void>
+__device__ __forceinline__ bool rows_parity(const Lane &ln, const Geo<G, ZC, WM> &geo, const S *st = nullptr)
 {
     bool p = false;
     // every row of the segment, without a branch between rows
-    SFor<L0, L1>::run([&](auto lc) { p = p | row_parity<G, ZC, decltype(lc)::value, T>(ln, geo); });
+    SFor<L0, L1>::run([&](auto lc) { p = p | row_parity<G, ZC, decltype(lc)::value, T, WM, S>(ln, geo, st); });
     return p;
 }
 
@@ -472,8 +535,9 @@ __device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballo
 // skips the remaining segments once none of its lanes is still checking
 // (wave-uniform branches, so no exec-mask bookkeeping).  Returns whether
 // this lane found a failed check.
-template <class G, int ZC, class T, bool WM>
-__device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC, WM> &geo, bool going, volatile int *cw_fail)
+template <class G, int ZC, class T, bool WM, class S = void>
+__device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC, WM> &geo, bool going, volatile int *cw_fail,
+                                           const S *st = nullptr)
 {
     constexpr int R = G::ROWS;
     constexpr int S0 = R < 1 ? R : 1, S1 = R < 4 ? R : 4, S2 = R < 10 ? R : 10, S3 = R < 20 ? R : 20,
@@ -482,7 +546,7 @@ __device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC, WM> 
     auto segment = [&](auto lo, auto hi) {
         constexpr int A = decltype(lo)::value, B = decltype(hi)::value;
         if (!wave_any(going)) return;
-        const bool f = going && rows_parity<G, ZC, A, B, T>(ln, geo);
+        const bool f = going && rows_parity<G, ZC, A, B, T, WM, S>(ln, geo, st);
         if (f) *cw_fail = 1;
         fail = fail || f;
         going = going && !f && !*cw_fail;
@@ -499,8 +563,9 @@ __device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC, WM> 
 // Decode one workgroup's codewords: `cws` codeword slots laid out in LDS
 // and over the lanes, `io` saying where the live ones come from and go to.
 // I8 selects the int8 path.  Waves without a live lane keep only the
-// barriers.
-template <int KIND, int ZC, bool I8, bool KEEPQ = true, bool WM = true>
+// barriers.  D1 (int8 only) lets the workgroup take the degree-1 bypass of
+// layer_i8 when every live codeword's degree-1 channel values allow it.
+template <int KIND, int ZC, bool I8, bool KEEPQ = true, bool WM = true, bool D1 = false>
 __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int Z, int cws, const GroupIO &io,
                                            const SynLaunch &a)
 {
@@ -543,31 +608,85 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
     const int glo = beta != beta ? INT_MIN : 0;  // the floor of |R|'s bits (none for a NaN beta)
     const int g0 = max(__float_as_int(0.f - beta), glo);  // |R| where the min is the FLT_MAX sentinel
     const int bq = a.beta_q, C = a.app_clamp;
-    for (int it = 0; it < a.iterations; ++it) {
-        if (all_done(fl, cws)) break;
-        const bool act = lane_ok && !fl.done[cl];
-        if (t < cws && !fl.done[t]) fl.used[t] = it + 1;
-        // Opaque per-iteration copies keep the 316 address computations
-        // from being hoisted out of the loop into registers; the masks
-        // re-establish that the bases are small and non-negative.
-        unsigned char *pb = smem + (opaque_v(zb0) & 0x3ffff);
-        const Lane ln{pb, pb - Z * ESZ, pb + 65536, pb - Z * ESZ + 65536, opaque_v(z) & 0x3ff};
+    // this lane's LDS addressing (the loop below rebuilds it per iteration)
+    auto lane_at = [&](unsigned char *pb) {
+        return Lane{pb, pb - Z * ESZ, pb + 65536, pb - Z * ESZ + 65536, z};
+    };
+    auto geo_at = [&]() {
         const unsigned long long *wm = nullptr;
-        if constexpr (ZC > 0 && WM) wm = kWrap<G, ZC>.m[opaque_s(wave)];
-        const Geo<G, ZC, WM> geo{Z, shifts + opaque_s(0), wm};
-        const int clo = opaque_v(-C), chi = opaque_v(C);  // the int8 posterior clamp's bounds, in VGPRs
-        SFor<0, G::ROWS>::run([&](auto lc) {
-            constexpr int L = decltype(lc)::value;
-            if (wave_live) {
-                if constexpr (I8) layer_i8<G, ZC, L, KEEPQ && KIND != SYN_KIND_BG1>(ln, geo, st, bq, clo, chi, act);
-                else layer_f32<G, ZC, L, KEEPQ>(ln, geo, st, beta, glo, g0, act);
-            }
-            if constexpr (barrier_after<G>(L)) __syncthreads();
-        });
-        check_rows<G, ZC, std::conditional_t<I8, int16_t, float>>(ln, geo, act, fl.fail + cl);
-        __syncthreads();
-        flags_settle(fl, cws);
-        __syncthreads();
+        if constexpr (ZC > 0 && WM) wm = kWrap<G, ZC>.m[wave];
+        return Geo<G, ZC, WM>{Z, shifts, wm};
+    };
+    auto iterate = [&](auto d1c) {
+        constexpr bool BYPASS = decltype(d1c)::value;
+        for (int it = 0; it < a.iterations; ++it) {
+            if (all_done(fl, cws)) break;
+            const bool act = lane_ok && !fl.done[cl];
+            if (t < cws && !fl.done[t]) fl.used[t] = it + 1;
+            // Opaque per-iteration copies keep the 316 address computations
+            // from being hoisted out of the loop into registers; the masks
+            // re-establish that the bases are small and non-negative.
+            unsigned char *pb = smem + (opaque_v(zb0) & 0x3ffff);
+            const Lane ln{pb, pb - Z * ESZ, pb + 65536, pb - Z * ESZ + 65536, opaque_v(z) & 0x3ff};
+            const unsigned long long *wm = nullptr;
+            if constexpr (ZC > 0 && WM) wm = kWrap<G, ZC>.m[opaque_s(wave)];
+            const Geo<G, ZC, WM> geo{Z, shifts + opaque_s(0), wm};
+            const int clo = opaque_v(-C), chi = opaque_v(C);  // the int8 posterior clamp's bounds, in VGPRs
+            SFor<0, G::ROWS>::run([&](auto lc) {
+                constexpr int L = decltype(lc)::value;
+                if (wave_live) {
+                    if constexpr (I8)
+                        layer_i8<G, ZC, L, KEEPQ && KIND != SYN_KIND_BG1, WM, BYPASS>(ln, geo, st, bq, clo, chi, act);
+                    else layer_f32<G, ZC, L, KEEPQ>(ln, geo, st, beta, glo, g0, act);
+                }
+                if constexpr (barrier_after<G>(L)) __syncthreads();
+            });
+            if constexpr (BYPASS) check_rows<G, ZC, int16_t, WM, State>(ln, geo, act, fl.fail + cl, &st);
+            else check_rows<G, ZC, std::conditional_t<I8, int16_t, float>>(ln, geo, act, fl.fail + cl);
+            __syncthreads();
+            flags_settle(fl, cws);
+            __syncthreads();
+        }
+    };
+    if constexpr (I8 && D1) {
+        // every degree-1 channel value of the live codewords within C - 127
+        const bool live = lane_ok && cl < io.count;
+        const Lane ln = lane_at(smem + zb0);
+        const Geo<G, ZC, WM> geo = geo_at();
+        bool ok = true;
+        if (live)
+            SFor<0, G::ROWS>::run([&](auto lc) {
+                constexpr int L = decltype(lc)::value;
+                constexpr int D = G::deg(L), B = layer_begin<G>(L);
+                SFor<0, D>::run([&](auto ec) {
+                    constexpr int e = decltype(ec)::value;
+                    if constexpr (edge_deg1<G>(B + e)) {
+                        const int c = *(const int16_t *)geo.template addr<B + e, 2>(ln);
+                        ok = ok && (c < 0 ? -c : c) <= C - 127;
+                    }
+                });
+            });
+        if (__syncthreads_and(ok ? 1 : 0)) {
+            iterate(std::true_type{});
+            // the degree-1 posteriors: c + R (no clamp reached)
+            if (live)
+                SFor<0, G::ROWS>::run([&](auto lc) {
+                    constexpr int L = decltype(lc)::value;
+                    constexpr int D = G::deg(L), B = layer_begin<G>(L);
+                    SFor<0, D>::run([&](auto ec) {
+                        constexpr int e = decltype(ec)::value;
+                        if constexpr (edge_deg1<G>(B + e)) {
+                            int16_t *v = (int16_t *)geo.template addr<B + e, 2>(ln);
+                            *v = (int16_t)((int)*v + state_r_i8<G, L, e>(st));
+                        }
+                    });
+                });
+            __syncthreads();
+        } else {
+            iterate(std::false_type{});
+        }
+    } else {
+        iterate(std::false_type{});
     }
     store_output<I8>(a, N, io, P0);
     store_iters(a, fl, io);
@@ -597,7 +716,9 @@ __device__ __forceinline__ void fast_body(const SynCode *__restrict__ code, cons
 {
     const int Z = (ZC > 0) ? ZC : code->z;
     const GroupIO io = contiguous_group(blockIdx.x * a.cws, a.cws, a.batch);
-    fast_group<KIND, ZC, I8>(code, Z, a.cws, io, a);
+    // the per-config int8 kernels carry the degree-1 bypass where the graph
+    // has degree-1 columns
+    fast_group<KIND, ZC, I8, true, true, I8 && any_deg1<Graph<KIND>>()>(code, Z, a.cws, io, a);
 }
 
 template <int KIND, int ZC>

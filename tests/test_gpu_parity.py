@@ -289,3 +289,49 @@ def test_capi_i8_host_buffers(dev, oracle):
             ri, ra, re, rh = oracle.decode_i8(code, 0.5, app0, 8)
         assert used == ri
         assert np.array_equal(app, ra) and np.array_equal(er, re) and np.array_equal(hd, rh)
+
+
+@pytest.mark.parametrize("bg,z", [(1, 208), (0, 384), (1, 52), (0, 26)])
+def test_i8_degree1_bypass_and_its_fallback(dev, oracle, bg, z):
+    """The int8 kernels' degree-1 bypass (a degree-1 variable's posterior
+    kept as channel value + R) holds only while every degree-1 |c| of a
+    workgroup is at most C - 127; a workgroup with one value past it runs
+    the full update.  One batch mixes both kinds of workgroup, at an app
+    clamp of 300 (bound 173), with values up to the clamp on degree-1
+    columns of every other codeword; every codeword equals the oracle."""
+    import syndrome_amd._lib as L
+    from syndrome_amd import QcLdpcDecoder, QuantParams
+    code = oracle.code_3gpp(bg, z)
+    dec = QcLdpcDecoder.with_lifting_size(bg, z, 0.5)
+    cws_per_group, _, _ = dec.launch_shape(int8=True)
+    B = 4 * cws_per_group
+    say(f"[i8 degree-1] BG{bg + 1} Z={z}: {B} codewords, {cws_per_group} per workgroup")
+    cw = codewords(oracle, code, B, seed=91 * z + bg)
+    rng = np.random.default_rng(7 * z + bg)
+    mag = rng.integers(1, 120, size=cw.shape)
+    app_in = np.where(cw == 0, mag, -mag).astype(np.int16)
+    first_deg1 = (26 if bg == 0 else 14) * z  # degree-1 columns: BG1 26..67, BG2 14..51
+    for g in range(1, 4, 2):  # workgroups 1 and 3: one degree-1 value past the bound each codeword
+        for b in range(g * cws_per_group, (g + 1) * cws_per_group):
+            v = first_deg1 + int(rng.integers(0, code.n - first_deg1))
+            app_in[b, v] = np.int16(250 if cw[b, v] == 0 else -250)
+    flip = rng.choice(code.n, size=(B, 6))
+    for b in range(B):
+        app_in[b, flip[b]] = -app_in[b, flip[b]] // 2
+    for clamp in (300, 32767):
+        d_in = torch.from_numpy(app_in).cuda()
+        d_post = torch.empty((B, code.n), dtype=torch.int16, device="cuda")
+        d_hard = torch.empty((B, code.n), dtype=torch.uint8, device="cuda")
+        d_edge = torch.empty((B, dec.required_edge_buffer()), dtype=torch.int8, device="cuda")
+        d_it = torch.empty(B, dtype=torch.int32, device="cuda")
+        flags = L.WRITE_POST | L.WRITE_HARD | L.WRITE_EDGE
+        dec.decode_batch_i8(d_in, d_post, d_hard, d_it, 8, flags=flags, d_edge=d_edge, quant=QuantParams(8.0, clamp))
+        torch.cuda.synchronize()
+        post, hard, edge, it = (d_post.cpu().numpy(), d_hard.cpu().numpy(), d_edge.cpu().numpy(),
+                                d_it.cpu().numpy())
+        for b in range(B):
+            ri, ra, re, rh = oracle.decode_i8(code, 0.5, app_in[b], 8, 8.0, clamp)
+            assert it[b] == ri, f"clamp {clamp} codeword {b}: iterations {it[b]} != {ri}"
+            assert np.array_equal(post[b], ra), f"clamp {clamp} codeword {b}: app differs"
+            assert np.array_equal(edge[b], re), f"clamp {clamp} codeword {b}: edge_r differs"
+            assert np.array_equal(hard[b], rh), f"clamp {clamp} codeword {b}: hard differs"
