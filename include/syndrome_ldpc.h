@@ -59,7 +59,13 @@ size_t syndrome_ldpc_decoder_info_bit_count(const SyndromeLdpcDecoder *decoder);
 size_t syndrome_ldpc_decoder_required_edge_buffer(const SyndromeLdpcDecoder *decoder);
 size_t syndrome_ldpc_decoder_required_layer_buffer(const SyndromeLdpcDecoder *decoder);
 
-/* capi/src/lib.rs:290-343 -> QcLdpcDecoder::decode_5g (qc_ldpc.rs:644-655).
+/* Host-buffer decodes (decode, decode_traced, decode_5g, decode_i8,
+ * decode_5g_i8) may be called concurrently on one handle, as the reference's
+ * decoder is Send + Sync (qc_ldpc.rs:480-484): each call takes one of up to 8
+ * staging areas of the handle, each with its own stream, so up to 8 calls run
+ * side by side and further ones wait for a free area.
+ *
+ * capi/src/lib.rs:290-343 -> QcLdpcDecoder::decode_5g (qc_ldpc.rs:644-655).
  * llr [N] in/out (channel LLRs in, a-posteriori out), edge_r [>= E*Z] out
  * (final check-to-variable messages), layer_scratch [>= maxdeg*Z] validated
  * only (its contents are left unspecified), hard [N] out, iterations = max

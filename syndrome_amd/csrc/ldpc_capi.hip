@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -22,6 +23,12 @@
 #include "ldpc_common.h"
 
 namespace syn {
+// One host-buffer call's device staging and its stream.
+struct Stage {
+    hipStream_t stream = nullptr;
+    void *d = nullptr;
+    size_t bytes = 0;
+};
 int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, const SynLaunch &a, int block,
                   size_t lds, hipStream_t stream);
 int launch_quantize(const float *in, int16_t *out, size_t n, float scale, hipStream_t stream);
@@ -44,11 +51,13 @@ struct SyndromeLdpcDecoder {
     std::unique_ptr<syn::WideCode> wide;
     SynWideCode wide_dev{};
     void *d_wide = nullptr;
-    // host-buffer entry points: serialised per handle, one staging area
-    std::mutex mu;
-    hipStream_t stream = nullptr;
-    void *d_stage = nullptr;
-    size_t stage_bytes = 0;
+    // host-buffer entry points: a pool of staging areas, each with its own
+    // stream, so concurrent calls on one handle run side by side (the
+    // reference's decoder is Send + Sync, qc_ldpc.rs:480-484)
+    std::mutex pool_mu;
+    std::condition_variable pool_cv;
+    std::vector<std::unique_ptr<syn::Stage>> stages;  // every staging area made
+    std::vector<syn::Stage *> free_stages;
     // lazily built encoder for traffic generation
     std::once_flag enc_once;
     std::unique_ptr<syn::Encoder> enc;
@@ -97,9 +106,11 @@ void free_decoder(SyndromeLdpcDecoder *d)
     DeviceScope ds(d->device);
     if (d->d_code) (void)hipFree(d->d_code);
     if (d->d_wide) (void)hipFree(d->d_wide);
-    if (d->d_stage) (void)hipFree(d->d_stage);
+    for (auto &st : d->stages) {
+        if (st->d) (void)hipFree(st->d);
+        if (st->stream) (void)hipStreamDestroy(st->stream);
+    }
     if (d->d_enc) (void)hipFree(d->d_enc);
-    if (d->stream) (void)hipStreamDestroy(d->stream);
 }
 
 struct DecoderOwner {
@@ -255,17 +266,49 @@ int32_t run_batch(const SyndromeLdpcDecoder *d, bool i8, const void *in, void *p
     return rc ? SYNDROME_DEVICE_ERROR : SYNDROME_OK;
 }
 
-int32_t ensure_stage(SyndromeLdpcDecoder *d, size_t bytes)
+// Staging areas of a handle: up to STAGES_MAX, made on demand; a call takes
+// a free one (waiting when all are busy), grows it if needed, and returns
+// it.
+constexpr size_t STAGES_MAX = 8;
+
+int32_t take_stage(SyndromeLdpcDecoder *d, size_t bytes, syn::Stage **out)
 {
-    if (!d->stream && hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
+    syn::Stage *st = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(d->pool_mu);
+        d->pool_cv.wait(lk, [&] { return !d->free_stages.empty() || d->stages.size() < STAGES_MAX; });
+        if (!d->free_stages.empty()) {
+            st = d->free_stages.back();
+            d->free_stages.pop_back();
+        } else {
+            d->stages.emplace_back(new syn::Stage());
+            st = d->stages.back().get();
+        }
+    }
+    if (!st->stream && hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) {
+        st->stream = nullptr;
+    } else if (st->bytes < bytes) {
+        if (st->d) (void)hipFree(st->d);
+        st->d = nullptr;
+        st->bytes = 0;
+        if (hipMalloc(&st->d, bytes) == hipSuccess) st->bytes = bytes;
+        else st->d = nullptr;
+    }
+    if (!st->stream || st->bytes < bytes) {
+        std::lock_guard<std::mutex> lk(d->pool_mu);
+        d->free_stages.push_back(st);
+        d->pool_cv.notify_one();
         return SYNDROME_DEVICE_ERROR;
-    if (d->stage_bytes >= bytes) return SYNDROME_OK;
-    if (d->d_stage) (void)hipFree(d->d_stage);
-    d->d_stage = nullptr;
-    d->stage_bytes = 0;
-    if (hipMalloc(&d->d_stage, bytes) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-    d->stage_bytes = bytes;
+    }
+    *out = st;
     return SYNDROME_OK;
+}
+
+void give_stage(SyndromeLdpcDecoder *d, syn::Stage *st)
+{
+    std::lock_guard<std::mutex> lk(d->pool_mu);
+    d->free_stages.push_back(st);
+    d->pool_cv.notify_one();
 }
 
 // Where one codeword's staging lives in a device buffer: posterior (elem
@@ -328,19 +371,21 @@ int32_t host_decode_at(SyndromeLdpcDecoder *d, char *base, hipStream_t stream, b
     return SYNDROME_OK;
 }
 
-// The handle's own staging: allocated on first use (and grown), one call at
-// a time per handle.
+// The handle's staging pool: each call takes a staging area of its own
+// (allocated on first use and grown), so calls on one handle overlap.
 int32_t host_decode(SyndromeLdpcDecoder *d, bool i8, void *llr, void *edge_r, size_t edge_elems, uint8_t *hard,
                     size_t iterations, size_t n_filler, bool init5g, float scale, int16_t app_clamp,
                     size_t *out_used, float *trace = nullptr)
 {
     (void)edge_elems;
     const StageLayout L = stage_layout(d, i8, trace ? iterations * syn::code_n(d->code) : 0);
-    std::lock_guard<std::mutex> lk(d->mu);
     DeviceScope ds(d->device);
-    if (int32_t rc = ensure_stage(d, L.total)) return rc;
-    return host_decode_at(d, (char *)d->d_stage, d->stream, i8, llr, edge_r, hard, iterations, n_filler, init5g,
-                          scale, app_clamp, out_used, trace);
+    syn::Stage *st = nullptr;
+    if (int32_t rc = take_stage(d, L.total, &st)) return rc;
+    const int32_t rc = host_decode_at(d, (char *)st->d, st->stream, i8, llr, edge_r, hard, iterations, n_filler,
+                                      init5g, scale, app_clamp, out_used, trace);
+    give_stage(d, st);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

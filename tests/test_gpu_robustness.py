@@ -125,38 +125,54 @@ def test_workspace_path_equals_raw_path(oracle):
 
 
 def test_concurrent_decodes_share_one_handle(oracle):
-    from syndrome_amd import QcLdpcDecoder
-    say("[threads] 4 threads x 5 decode_5g calls on one BG2 Z=104 handle")
+    """12 threads on one handle, more than its 8 staging areas, mixing the
+    f32 and int8 host-buffer decodes: every call returns the oracle's result
+    (the calls overlap on the device, each on its own staging and stream)."""
+    from syndrome_amd import QcLdpcDecoder, QuantParams
+    T, K = 12, 36
+    say(f"[threads] {T} threads x {K // T} decode_5g / decode_5g_i8 calls on one BG2 Z=104 handle")
     code = oracle.code_3gpp(1, 104)
     dec = QcLdpcDecoder.with_lifting_size(1, 104, 0.5)
-    cws = codewords(oracle, code, 20, seed=33)
-    llrs = channel(oracle, code, cws, list(np.linspace(0.0, 2.5, 20)), seed=8)
-    results = [None] * 20
+    cws = codewords(oracle, code, K, seed=33)
+    llrs = channel(oracle, code, cws, list(np.linspace(0.0, 2.5, K)), seed=8)
+    results = [None] * K
     errors = []
 
     def work(tid):
         try:
-            for j in range(tid, 20, 4):
-                a = llrs[j].copy()
-                er = np.zeros(dec.required_edge_buffer(), np.float32)
-                sc = np.zeros(dec.required_layer_buffer(), np.float32)
+            for j in range(tid, K, T):
                 hd = np.zeros(dec.variable_node_count(), np.uint8)
-                used = dec.decode_5g(a, 0, er, sc, hd, 8)
+                if j % 3 == 2:
+                    a = oracle.quantize_i16(llrs[j], 8.0)
+                    er = np.zeros(dec.required_edge_buffer(), np.int8)
+                    sc = np.zeros(dec.required_layer_buffer(), np.int8)
+                    used = dec.decode_5g_i8(a, 0, er, sc, hd, 8, QuantParams())
+                else:
+                    a = llrs[j].copy()
+                    er = np.zeros(dec.required_edge_buffer(), np.float32)
+                    sc = np.zeros(dec.required_layer_buffer(), np.float32)
+                    used = dec.decode_5g(a, 0, er, sc, hd, 8)
                 results[j] = (used, a, er, hd)
         except Exception as e:  # surfaced below
             errors.append(e)
 
-    th = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    th = [threading.Thread(target=work, args=(t,)) for t in range(T)]
     for t in th:
         t.start()
     for t in th:
         t.join(timeout=60)
     assert not errors and all(r is not None for r in results)
-    for j in range(20):
-        ri, rp, re, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, llrs[j], 0), 8)
+    for j in range(K):
         used, a, er, hd = results[j]
-        assert used == ri and np.array_equal(f32_bits(a), f32_bits(rp)), f"decode {j}"
-        assert np.array_equal(f32_bits(er), f32_bits(re)) and np.array_equal(hd, rh), f"decode {j}"
+        if j % 3 == 2:
+            app = oracle.init_5g_i8(code, oracle.quantize_i16(llrs[j], 8.0), 0)
+            ri, ra, re, rh = oracle.decode_i8(code, 0.5, app, 8)
+            assert used == ri and np.array_equal(a, ra), f"decode {j}"
+            assert np.array_equal(er, re) and np.array_equal(hd, rh), f"decode {j}"
+        else:
+            ri, rp, re, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, llrs[j], 0), 8)
+            assert used == ri and np.array_equal(f32_bits(a), f32_bits(rp)), f"decode {j}"
+            assert np.array_equal(f32_bits(er), f32_bits(re)) and np.array_equal(hd, rh), f"decode {j}"
 
 
 def test_device_quantizer_known_answers_and_oracle(oracle):
