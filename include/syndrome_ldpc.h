@@ -196,6 +196,12 @@ int32_t syndrome_ldpc_decode_traced(SyndromeLdpcDecoder *decoder, float *llr, si
                                     size_t trace_len, size_t *out_iterations_used);
 /* quantize_llr_i16 (quantize.rs:331-341) on device: f32 [n] -> i16 [n]. */
 int32_t syndrome_ldpc_quantize_i16_device(const float *d_llr, int16_t *d_out, size_t n, float scale, void *stream);
+/* quantize_llr (quantize.rs:148-154) on device: f32 [n] -> i8 [n],
+ * round half away from zero, clamp to +-127 (never -128), NaN -> 0. */
+int32_t syndrome_ldpc_quantize_i8_device(const float *d_llr, int8_t *d_out, size_t n, float scale, void *stream);
+/* dequantize_llr (quantize.rs:163-169) on device: i8 [n] -> f32 [n] as
+ * q * (1 / scale). */
+int32_t syndrome_ldpc_dequantize_i8_device(const int8_t *d_q, float *d_out, size_t n, float scale, void *stream);
 
 /* ======================================================================
  * Part 3: host helpers for traffic generation

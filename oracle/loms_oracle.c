@@ -260,6 +260,12 @@ void orc_quantize_llr_i8(const float *llr, int8_t *out, size_t n, float scale)
     }
 }
 
+void orc_dequantize_llr_i8(const int8_t *q, float *out, size_t n, float scale)
+{
+    const float inv = 1.0f / scale; /* quantize.rs:163-169 */
+    for (size_t i = 0; i < n; ++i) out[i] = (float)q[i] * inv;
+}
+
 int orc_init_5g_i8(const orc_code_t *c, int16_t *app, size_t app_len, size_t n_filler, int16_t app_clamp)
 {
     /* qc_ldpc.rs:1495-1514 */

@@ -85,6 +85,8 @@ int8_t orc_beta_q(float beta, float scale);
 void orc_quantize_llr_i16(const float *llr, int16_t *out, size_t n, float scale);
 /* quantize.rs:148-154 */
 void orc_quantize_llr_i8(const float *llr, int8_t *out, size_t n, float scale);
+/* quantize.rs:163-169 */
+void orc_dequantize_llr_i8(const int8_t *q, float *out, size_t n, float scale);
 
 /* Syndrome of hard bits (0/1 bytes) against H: 1 iff every check holds. */
 int orc_syndrome_ok_bits(const orc_code_t *c, const uint8_t *bits);

@@ -71,6 +71,7 @@ def lib():
         L.orc_beta_q.restype = C.c_int8
         L.orc_quantize_llr_i16.argtypes = [P, P, C.c_size_t, C.c_float]
         L.orc_quantize_llr_i8.argtypes = [P, P, C.c_size_t, C.c_float]
+        L.orc_dequantize_llr_i8.argtypes = [P, P, C.c_size_t, C.c_float]
         L.orc_syndrome_ok_bits.argtypes = [C.POINTER(Code), P]
         L.orc_fnv1a_hard.argtypes = [P, C.c_size_t]
         L.orc_fnv1a_hard.restype = C.c_uint64
@@ -172,6 +173,13 @@ def quantize_i8(llr, scale=8.0):
     llr = np.ascontiguousarray(llr, np.float32)
     out = np.zeros(llr.size, np.int8)
     lib().orc_quantize_llr_i8(_p(llr), _p(out), llr.size, scale)
+    return out
+
+
+def dequantize_i8(q, scale=8.0):
+    q = np.ascontiguousarray(q, np.int8)
+    out = np.zeros(q.size, np.float32)
+    lib().orc_dequantize_llr_i8(_p(q), _p(out), q.size, scale)
     return out
 
 

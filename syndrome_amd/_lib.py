@@ -65,6 +65,8 @@ SIGNATURES = {
     "syndrome_ldpc_decode_batch_i8": (I32, [P, P, P, P, P, P, SZ, SZ, SZ, C.c_float, C.c_int16, C.c_uint32, P]),
     "syndrome_ldpc_launch_shape": (I32, [P, I32, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
     "syndrome_ldpc_quantize_i16_device": (I32, [P, P, SZ, C.c_float, P]),
+    "syndrome_ldpc_quantize_i8_device": (I32, [P, P, SZ, C.c_float, P]),
+    "syndrome_ldpc_dequantize_i8_device": (I32, [P, P, SZ, C.c_float, P]),
     "syndrome_awgn_transmit": (I32, [C.c_float, C.c_float, C.c_uint64, P, SZ, P]),
     "syndrome_ldpc_encode_batch": (I32, [P, P, P, SZ, I32]),
     "syndrome_ldpc_encode_batch_device": (I32, [P, P, P, SZ, P]),

@@ -32,6 +32,8 @@ struct Stage {
 int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, const SynLaunch &a, int block,
                   size_t lds, hipStream_t stream);
 int launch_quantize(const float *in, int16_t *out, size_t n, float scale, hipStream_t stream);
+int launch_quantize_i8(const float *in, int8_t *out, size_t n, float scale, hipStream_t stream);
+int launch_dequantize_i8(const int8_t *in, float *out, size_t n, float scale, hipStream_t stream);
 WideShape wide_shape(const SynWideCode &w, bool i8, int lds_limit);
 int launch_wide(bool i8, const SynWideCode &w, const SynLaunch &a, const WideShape &s, char *work,
                 hipStream_t stream);
@@ -739,6 +741,22 @@ int32_t syndrome_ldpc_quantize_i16_device(const float *d_llr, int16_t *d_out, si
     return guard([&]() -> int32_t {
         if ((!d_llr || !d_out) && n) return SYNDROME_NULL_POINTER;
         return syn::launch_quantize(d_llr, d_out, n, scale, (hipStream_t)stream);
+    });
+}
+
+int32_t syndrome_ldpc_quantize_i8_device(const float *d_llr, int8_t *d_out, size_t n, float scale, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if ((!d_llr || !d_out) && n) return SYNDROME_NULL_POINTER;
+        return syn::launch_quantize_i8(d_llr, d_out, n, scale, (hipStream_t)stream);
+    });
+}
+
+int32_t syndrome_ldpc_dequantize_i8_device(const int8_t *d_q, float *d_out, size_t n, float scale, void *stream)
+{
+    return guard([&]() -> int32_t {
+        if ((!d_q || !d_out) && n) return SYNDROME_NULL_POINTER;
+        return syn::launch_dequantize_i8(d_q, d_out, n, scale, (hipStream_t)stream);
     });
 }
 

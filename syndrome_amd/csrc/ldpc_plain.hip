@@ -181,9 +181,13 @@ __global__ void __launch_bounds__(1024) plain_i8(const SynCode *__restrict__ cod
     store_iters(a, fl, io);
 }
 
-// quantize_llr_i16 (quantize.rs:331-341) as a standalone pass.
-__global__ void __launch_bounds__(256) quantize_i16(const float *__restrict__ in, int16_t *__restrict__ out, size_t n,
-                                                    float scale)
+// quantize_llr_i16 (quantize.rs:331-341) and quantize_llr (quantize.rs:148-154)
+// as standalone passes: v = l * scale; NaN -> 0 (the i16 form's test, and
+// Rust's saturating `as i32` for the i8 form); round half away from zero;
+// clamp to +-127 (which also takes +-inf to +-127).
+template <class T>
+__global__ void __launch_bounds__(256) quantize_pass(const float *__restrict__ in, T *__restrict__ out, size_t n,
+                                                     float scale)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         float f = in[i] * scale;
@@ -193,10 +197,18 @@ __global__ void __launch_bounds__(256) quantize_i16(const float *__restrict__ in
             f = f < -127.f ? -127.f : (f > 127.f ? 127.f : f);
             v = (int)f;
         }
-        out[i] = (int16_t)v;
+        out[i] = (T)v;
     }
 }
 
+// dequantize_llr (quantize.rs:163-169): q * (1 / scale), the reciprocal
+// computed once on the host, as the reference computes it once per call.
+__global__ void __launch_bounds__(256) dequantize_i8(const int8_t *__restrict__ in, float *__restrict__ out, size_t n,
+                                                     float inv)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i] * inv;
+}
 
 const void *fast_kernel_f32_bg1(int z);
 const void *fast_kernel_f32_bg2(int z);
@@ -258,7 +270,24 @@ int launch_quantize(const float *in, int16_t *out, size_t n, float scale, hipStr
 {
     if (n == 0) return SYN_OK;
     const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(quantize_i16, dim3((unsigned)blocks), dim3(256), 0, stream, in, out, n, scale);
+    hipLaunchKernelGGL(quantize_pass<int16_t>, dim3((unsigned)blocks), dim3(256), 0, stream, in, out, n, scale);
+    return hipGetLastError() == hipSuccess ? SYN_OK : SYN_DEVICE_ERROR;
+}
+
+int launch_quantize_i8(const float *in, int8_t *out, size_t n, float scale, hipStream_t stream)
+{
+    if (n == 0) return SYN_OK;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(quantize_pass<int8_t>, dim3((unsigned)blocks), dim3(256), 0, stream, in, out, n, scale);
+    return hipGetLastError() == hipSuccess ? SYN_OK : SYN_DEVICE_ERROR;
+}
+
+int launch_dequantize_i8(const int8_t *in, float *out, size_t n, float scale, hipStream_t stream)
+{
+    if (n == 0) return SYN_OK;
+    const float inv = 1.0f / scale;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(dequantize_i8, dim3((unsigned)blocks), dim3(256), 0, stream, in, out, n, inv);
     return hipGetLastError() == hipSuccess ? SYN_OK : SYN_DEVICE_ERROR;
 }
 

@@ -346,6 +346,28 @@ def quantize_llr_i16_device(d_llr, d_out, scale=DEFAULT_SCALE, stream=None):
           "quantize")
 
 
+def quantize_llr_i8_device(d_llr, d_out, scale=DEFAULT_SCALE, stream=None):
+    """quantize_llr (quantize.rs:148-154) on device: f32 tensor -> i8 tensor."""
+    import torch
+    _need(d_llr, "d_llr", "quantize_llr_i8_device", (torch.float32,), 0)
+    _need(d_out, "d_out", "quantize_llr_i8_device", (torch.int8,), d_llr.numel())
+    _resident("quantize_llr_i8_device", d_llr=d_llr, d_out=d_out)
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().syndrome_ldpc_quantize_i8_device(d_llr.data_ptr(), d_out.data_ptr(), d_llr.numel(), scale, s),
+          "quantize_i8")
+
+
+def dequantize_llr_i8_device(d_q, d_out, scale=DEFAULT_SCALE, stream=None):
+    """dequantize_llr (quantize.rs:163-169) on device: i8 tensor -> f32 tensor."""
+    import torch
+    _need(d_q, "d_q", "dequantize_llr_i8_device", (torch.int8,), 0)
+    _need(d_out, "d_out", "dequantize_llr_i8_device", (torch.float32,), d_q.numel())
+    _resident("dequantize_llr_i8_device", d_q=d_q, d_out=d_out)
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().syndrome_ldpc_dequantize_i8_device(d_q.data_ptr(), d_out.data_ptr(), d_q.numel(), scale, s),
+          "dequantize_i8")
+
+
 def awgn_transmit(ebno_db, code_rate, seed, bits):
     """AwgnChannel::new(ebno_db, code_rate, seed).transmit(bits) (channel_sim.rs:139-215)."""
     bits = np.ascontiguousarray(bits, np.uint8)

@@ -207,3 +207,35 @@ def test_device_quantizer_known_answers_and_oracle(oracle):
             quantize_llr_i16_device(d_x, d_q, scale, stream=s.cuda_stream)
         s.synchronize()
         assert np.array_equal(d_q.cpu().numpy(), oracle.quantize_i16(x, scale))
+
+
+def test_device_quantize_llr_i8_and_dequantize(oracle):
+    """syndrome_ldpc_quantize_i8_device / dequantize_i8_device: quantize_llr
+    and dequantize_llr (quantize.rs:148-169) on the doc example's known
+    answers and on a large fuzz vector against the oracle, two scales."""
+    import syndrome_amd
+    from syndrome_amd.ldpc import dequantize_llr_i8_device, quantize_llr_i8_device
+    syndrome_amd.lib()
+    d_x = torch.tensor([1.5, -3.0, 0.0, 200.0, float("nan"), float("-inf")], dtype=torch.float32, device="cuda")
+    d_q = torch.full((6,), 99, dtype=torch.int8, device="cuda")
+    quantize_llr_i8_device(d_x, d_q, 8.0)
+    d_back = torch.empty(6, dtype=torch.float32, device="cuda")
+    dequantize_llr_i8_device(d_q, d_back, 8.0)
+    torch.cuda.synchronize()
+    assert d_q.cpu().tolist() == [12, -24, 0, 127, 0, -127]
+    assert d_back.cpu().tolist() == [1.5, -3.0, 0.0, 15.875, 0.0, -15.875]
+    rng = np.random.default_rng(29)
+    n = (1 << 20) + 13
+    x = _fuzz_llrs(rng, n)
+    x[: n // 2] = rng.normal(0, 20, n // 2).astype(np.float32)
+    for scale in (8.0, 3.0):
+        say(f"[quantize i8] {n} LLRs, scale {scale}")
+        d_x = torch.from_numpy(x).cuda()
+        d_q = torch.empty(n, dtype=torch.int8, device="cuda")
+        d_f = torch.empty(n, dtype=torch.float32, device="cuda")
+        quantize_llr_i8_device(d_x, d_q, scale)
+        dequantize_llr_i8_device(d_q, d_f, scale)
+        torch.cuda.synchronize()
+        q = oracle.quantize_i8(x, scale)
+        assert np.array_equal(d_q.cpu().numpy(), q)
+        assert np.array_equal(f32_bits(d_f.cpu().numpy()), f32_bits(oracle.dequantize_i8(q, scale)))

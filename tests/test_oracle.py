@@ -155,6 +155,15 @@ def test_quantizer_known_answers(oracle):
     assert list(oracle.quantize_i16(np.array([1.0, -0.25, 99.0], np.float32), 8.0)) == [8, -2, 127]
     # ties away from zero (f32::round), not to even
     assert list(oracle.quantize_i16(np.array([0.0625, -0.0625, 0.1875], np.float32), 8.0)) == [1, -1, 2]
+    # quantize_llr / dequantize_llr doc example (quantize.rs:123-133): 200 saturates at 127
+    q = oracle.quantize_i8(np.array([1.5, -3.0, 0.0, 200.0], np.float32), 8.0)
+    assert list(q) == [12, -24, 0, 127]
+    assert list(oracle.dequantize_i8(q, 8.0)) == [1.5, -3.0, 0.0, 15.875]
+    # round trip on the grid: every multiple of 1/8 in [-15.875, 15.875] comes back
+    grid = (np.arange(-127, 128) / 8.0).astype(np.float32)
+    assert np.array_equal(oracle.dequantize_i8(oracle.quantize_i8(grid, 8.0), 8.0), grid)
+    # 1/scale is one f32 division, then one multiply: scale 3 gives q * 0.33333334f
+    assert oracle.dequantize_i8(np.array([3], np.int8), 3.0)[0] == np.float32(3) * (np.float32(1) / np.float32(3))
     assert oracle.beta_q(0.5) == 4 and oracle.beta_q(0.125) == 1 and oracle.beta_q(0.0) == 0
     assert oracle.beta_q(-1.0) == 0 and oracle.beta_q(1e6) == 127
     assert oracle.beta_q(float("inf")) == 127 and oracle.beta_q(float("nan")) == 127
