@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsyndrome_amd.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["ldpc_fast_f32_bg1.hip", "ldpc_fast_i8_bg1.hip", "ldpc_fast_f32_bg2.hip", "ldpc_fast_i8_bg2.hip",
-           "ldpc_fast_wifi.hip",
+           "ldpc_fast_wifi.hip", "ldpc_pack_i8_bg2.hip",
            "ldpc_persist_f32.hip", "ldpc_persist_f32_small.hip", "ldpc_persist_i8.hip", "ldpc_resident.hip", "ldpc_tb.hip", "ldpc_encode.hip", "ldpc_dematch.hip", "ldpc_plain.hip", "ldpc_wide.hip", "ldpc_plan.hip", "ldpc_serve.hip", "ldpc_capi.hip",
            "code_host.cpp"]
 ARCH = os.environ.get("SYNDROME_OFFLOAD_ARCH", "gfx950")
@@ -24,6 +24,9 @@ COMMON = [
     "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
+# per-source extras: the pair-packed kernels schedule for ILP, which keeps
+# dependent packed instructions apart (each needs a wait state after it)
+EXTRA = {"ldpc_pack_i8_bg2.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 DEVICE = [
     f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -39,7 +42,7 @@ def hipcc():
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    extra = DEVICE if src.endswith(".hip") else []
+    extra = (DEVICE + EXTRA.get(src, [])) if src.endswith(".hip") else []
     cmd = [hipcc()] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     errs = "\n".join(l for l in (r.stderr or "").splitlines() if "packed-fp32-ops" not in l)

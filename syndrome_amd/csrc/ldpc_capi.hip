@@ -193,7 +193,21 @@ int32_t launch_shape(const SyndromeLdpcDecoder *d, bool i8, bool plain, Shape *s
         s->lds = w.lds;
         return SYNDROME_OK;
     }
-    const size_t n = syn::code_n(c), elem = i8 ? 2 : 4;
+    const size_t n = syn::code_n(c);
+    if (i8 && use_fast(c, plain) && c.kind == SYN_KIND_BG2) {
+        // pair-packed kernel (ldpc_pack.h): codeword pairs, 32-bit words of
+        // two interleaved i16 posteriors, up to 256 lanes
+        const size_t pad = (size_t)(((c.z * 4) + 15) & ~15);
+        auto bytes = [&](int k) { return pad + (size_t)k * n * 4 + 3 * sizeof(int) * (size_t)(2 * k); };
+        int k = std::max(1, 256 / c.z);
+        while (k > 1 && bytes(k) > (size_t)d->lds_limit) --k;
+        if (bytes(k) > (size_t)d->lds_limit) return SYNDROME_INVALID_PARAM;
+        s->cws = 2 * k;
+        s->block = ((k * c.z + 63) / 64) * 64;
+        s->lds = bytes(k);
+        return SYNDROME_OK;
+    }
+    const size_t elem = i8 ? 2 : 4;
     const int max_lanes = use_fast(c, plain) ? 256 : 1024;
     // fast kernels keep a Z*size pad in front of the posterior (ldpc_fast.h)
     const size_t pad = use_fast(c, plain) ? (size_t)(((c.z * (int)elem) + 15) & ~15) : 0;

@@ -215,6 +215,7 @@ const void *fast_kernel_f32_bg2(int z);
 const void *fast_kernel_i8_bg1(int z);
 const void *fast_kernel_i8_bg2(int z);
 const void *fast_kernel_wifi(int kind, bool i8);
+const void *pack_kernel_i8_bg2(int z);
 
 // The kernel a decode of (kind, format, Z) runs: plain, or the
 // register-resident one for the base graph (its Z-specialised instance when
@@ -223,7 +224,7 @@ const void *decode_kernel(int kind, bool i8, bool fast, int z)
 {
     if (!fast) return i8 ? (const void *)plain_i8 : (const void *)plain_f32;
     if (kind == SYN_KIND_BG1) return i8 ? fast_kernel_i8_bg1(z) : fast_kernel_f32_bg1(z);
-    if (kind == SYN_KIND_BG2) return i8 ? fast_kernel_i8_bg2(z) : fast_kernel_f32_bg2(z);
+    if (kind == SYN_KIND_BG2) return i8 ? pack_kernel_i8_bg2(z) : fast_kernel_f32_bg2(z);
     if (kind >= SYN_KIND_WIFI0) return fast_kernel_wifi(kind, i8);
     return nullptr;
 }
