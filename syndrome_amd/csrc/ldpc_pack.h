@@ -426,7 +426,7 @@ __device__ __forceinline__ void pack_group(const SynCode *__restrict__ code, int
 }
 
 template <int KIND, int ZC>
-__global__ void __launch_bounds__(384) pack_i8(const SynCode *__restrict__ code, SynLaunch a)
+__global__ void __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(3))) pack_i8(const SynCode *__restrict__ code, SynLaunch a)
 {
     const int Z = (ZC > 0) ? ZC : code->z;
     const GroupIO io = contiguous_group(blockIdx.x * a.cws, a.cws, a.batch);
