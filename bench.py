@@ -215,6 +215,10 @@ VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 #   exclusive sign xor, a == m1, select, - beta_q, max 0, sign as neg =
 #   s >> 7, xor, sub (3), delta, saturating add, clamp max, min (13): 22.
 ALG_OPS_PER_EDGE_SLOT = {"f32": 15, "i8": 22}
+# The int8 kernel for BG2 runs two codewords per lane in the 16-bit halves
+# of packed instructions (v_pk_*_i16/u16): one lane-instruction performs two
+# of the operations above, so its roof is twice the lane-instruction peak.
+PACKED_OPS_PER_LANE_INSTRUCTION = 2
 
 
 def _profile_json(name, suffix):
@@ -227,22 +231,26 @@ def _profile_json(name, suffix):
     return None
 
 
-def valu_roofline(kernel, dtype, edge_slot_iters, kernel_ms, compulsory_bytes, profile):
+def valu_roofline(kernel, dtype, edge_slot_iters, kernel_ms, compulsory_bytes, profile, packed=False):
     """Roofline of one decode launch: achieved algorithmic lane-operations
     (ALG_OPS_PER_EDGE_SLOT x executed edge-slot-iterations) over the
     launch's average duration against the VALU peak, which is the binding
     roof; beside it the VALU issue share and VALU instructions per
     edge-slot-iteration that the SQ counters of profiles/<profile>.md read,
     and HBM: the compulsory bytes (input, posterior, hard decisions) over the
-    same time, with the counter traffic of that profile."""
+    same time, with the counter traffic of that profile.  packed: the kernel
+    does its arithmetic in packed 16-bit instructions, whose peak is
+    PACKED_OPS_PER_LANE_INSTRUCTION operations per lane-instruction."""
     ops = ALG_OPS_PER_EDGE_SLOT[dtype] * float(edge_slot_iters)
     t = kernel_ms * 1e-3
     tops = ops / t / 1e12
+    peak = VALU_PEAK_TOPS * (PACKED_OPS_PER_LANE_INSTRUCTION if packed else 1)
     traffic = (_profile_json(profile, "traffic") or {}).get("hbm_bytes_per_launch")
     sq = _profile_json(profile, "sq") or {}
     hbm = compulsory_bytes / t / 1e9
-    return {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tlane-op/s",
-            "frac": round(tops / VALU_PEAK_TOPS, 4), "traffic": traffic, "kernel": kernel,
+    return {"bound": "valu", "achieved": round(tops, 3), "peak": round(peak, 2),
+            "unit": "Top/s (16-bit lanes of packed instructions)" if packed else "Tlane-op/s",
+            "frac": round(tops / peak, 4), "traffic": traffic, "kernel": kernel,
             "kernel_ms": round(kernel_ms, 4), "alg_ops_per_edge_slot_iteration": ALG_OPS_PER_EDGE_SLOT[dtype],
             "edge_slot_iterations_per_launch": float(edge_slot_iters),
             "valu_issue_frac_profiled": sq.get("valu_issue_frac"),
@@ -403,8 +411,9 @@ def bench_int8(torch, args):
            "ms_per_launch": round(t * 1e3, 4), "plain_value": round(B * k / tp / 1e9, 4),
            "mean_iterations": round(float(its.mean()), 3), "bler": bler,
            # compulsory bytes per codeword: f32 input, i16 posterior, u8 hard decisions
-           "roofline": valu_roofline("syn::fast_i8<1, 208> (BG2, Z=208)", "i8", float(its.sum()) * 197 * z,
-                                     float(np.mean(per)), float(B) * n * 7, "fast_i8_bg2_z208_b16384")}
+           "roofline": valu_roofline("syn::pack_i8<1, 208> (BG2, Z=208, two codewords per lane)", "i8",
+                                     float(its.sum()) * 197 * z, float(np.mean(per)), float(B) * n * 7,
+                                     "pack_i8_bg2_z208_b16384", packed=True)}
     if not args.no_cpu:
         from oracle import orc
         code = orc.code_3gpp(1, z)

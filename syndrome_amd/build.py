@@ -24,9 +24,12 @@ COMMON = [
     "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
-# per-source extras: the pair-packed kernels schedule for ILP, which keeps
-# dependent packed instructions apart (each needs a wait state after it)
-EXTRA = {"ldpc_pack_i8_bg2.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+# per-source extras, scheduling for ILP: the pair-packed kernels, whose
+# dependent packed instructions need a wait state between them (max-ilp
+# interleaves independent work there instead of s_nop), and the BG1 f32
+# kernels (BG1 Z=384: 2.94 -> 2.91 ms per launch, no added spills)
+EXTRA = {"ldpc_pack_i8_bg2.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+         "ldpc_fast_f32_bg1.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 DEVICE = [
     f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -142,14 +142,23 @@ struct GroupIO {
     const void *const *srcs;
     const int *nfill_pos;
     uint32_t no_hard = 0;  // bit j: codeword j's hard decisions are not written (its consumer reads LDS)
+    // Optional LDS copies of offs[id(j)] and nfill[id(j)] for j < count.
+    // Without them every input and output chunk of the indirect form looks
+    // its codeword up through ids, then offs (a chain of dependent global
+    // loads that the output stores, which may alias them, force again on
+    // every chunk).
+    const long long *t_off = nullptr;
+    const int *t_nf = nullptr;
 
     __device__ __forceinline__ int id(int j) const { return ids ? ids[j] : base + j; }
     __device__ __forceinline__ long long off(int j, int N) const
     {
+        if (t_off) return t_off[j];
         return offs ? offs[id(j)] : (long long)id(j) * N;
     }
     __device__ __forceinline__ long long n_filler(int j, long long dflt) const
     {
+        if (t_nf) return t_nf[j];
         return nfill_pos ? nfill_pos[j] : (nfill ? nfill[id(j)] : dflt);
     }
     // first input byte of codeword j (elements of esz bytes)

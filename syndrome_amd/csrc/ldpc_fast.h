@@ -565,9 +565,15 @@ __device__ __forceinline__ bool check_rows(const Lane &ln, const Geo<G, ZC, WM> 
 // I8 selects the int8 path.  Waves without a live lane keep only the
 // barriers.  D1 (int8 only) lets the workgroup take the degree-1 bypass of
 // layer_i8 when every live codeword's degree-1 channel values allow it.
-template <int KIND, int ZC, bool I8, bool KEEPQ = true, bool WM = true, bool D1 = false>
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+// `done` runs once the iterations are over, before the outputs are stored
+// (a work-queue workgroup claims its next item there).
+template <int KIND, int ZC, bool I8, bool KEEPQ = true, bool WM = true, bool D1 = false, class Hook = NoHook>
 __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int Z, int cws, const GroupIO &io,
-                                           const SynLaunch &a)
+                                           const SynLaunch &a, Hook done = {})
 {
     using G = Graph<KIND>;
     using State = std::conditional_t<I8, StateI8<G>, StateF32<G>>;
@@ -688,6 +694,7 @@ __device__ __forceinline__ void fast_group(const SynCode *__restrict__ code, int
     } else {
         iterate(std::false_type{});
     }
+    done();
     store_output<I8>(a, N, io, P0);
     store_iters(a, fl, io);
     if ((a.flags & SYN_F_WRITE_EDGE) && lane_ok && cl < io.count) {
@@ -743,26 +750,26 @@ namespace syn {
 template <int... ZS>
 struct ZSet {};
 
-template <int KIND, bool I8, int Z0, int... ZS>
+template <int KIND, bool I8, int Z0, int... ZS, class Hook>
 __device__ __forceinline__ bool static_group(int z, const SynCode *__restrict__ code, int count, const GroupIO &io,
-                                             const SynLaunch &a)
+                                             const SynLaunch &a, Hook done)
 {
     if (z == Z0) {
-        fast_group<KIND, Z0, I8, !I8>(code, Z0, count, io, a);
+        fast_group<KIND, Z0, I8, !I8>(code, Z0, count, io, a, done);
         return true;
     }
-    if constexpr (sizeof...(ZS) > 0) return static_group<KIND, I8, ZS...>(z, code, count, io, a);
+    if constexpr (sizeof...(ZS) > 0) return static_group<KIND, I8, ZS...>(z, code, count, io, a, done);
     else return false;
 }
 
-template <int KIND, bool I8, int... ZS>
+template <int KIND, bool I8, int... ZS, class Hook>
 __device__ __forceinline__ void item_group(ZSet<ZS...>, const SynCode *__restrict__ code, int count, const GroupIO &io,
-                                           const SynLaunch &a)
+                                           const SynLaunch &a, Hook done)
 {
     const int z = code->z;
     if constexpr (sizeof...(ZS) > 0)
-        if (static_group<KIND, I8, ZS...>(z, code, count, io, a)) return;
-    fast_group<KIND, 0, I8, !I8>(code, z, count, io, a);
+        if (static_group<KIND, I8, ZS...>(z, code, count, io, a, done)) return;
+    fast_group<KIND, 0, I8, !I8>(code, z, count, io, a, done);
 }
 
 // Persistent work-queue decoder: a grid sized to the resident workgroups
@@ -774,6 +781,8 @@ __device__ __forceinline__ void item_group(ZSet<ZS...>, const SynCode *__restric
 // different base graphs, lifting sizes and filler counts share one launch.
 // The int8 body keeps its register-lean form: both base graphs share one
 // kernel, whose VGPR budget the BG1 body sets.
+constexpr int PERSIST_MAX_CWS = 256;  // codewords of one work item: 512 lanes over Z >= 2
+
 template <bool I8, class Z1, class Z2>
 __device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, const SynItem *__restrict__ items,
                                              int n_items, int *head, const int *__restrict__ ids,
@@ -781,17 +790,38 @@ __device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, 
                                              const SynLaunch &a)
 {
     __shared__ int s_item;
+    // the item's per-codeword element offsets and filler counts, looked up
+    // once per item
+    __shared__ long long s_off[PERSIST_MAX_CWS];
+    __shared__ int s_nf[PERSIST_MAX_CWS];
+    if (threadIdx.x == 0) s_item = atomicAdd(head, 1);
     for (;;) {
-        __syncthreads();  // the previous item is finished with LDS and s_item
-        if (threadIdx.x == 0) s_item = atomicAdd(head, 1);
-        __syncthreads();
+        __syncthreads();  // s_item is written
         const int w = s_item;
         if (w >= n_items) return;
         const SynItem it = items[w];
         const SynCode *code = codes + it.code;
-        const GroupIO io{it.count, 0, ids + it.first, offs, nfill, nullptr, nullptr};
-        if (it.kind == SYN_KIND_BG1) item_group<SYN_KIND_BG1, I8>(Z1{}, code, it.count, io, a);
-        else item_group<SYN_KIND_BG2, I8>(Z2{}, code, it.count, io, a);
+        if ((int)threadIdx.x < it.count) {
+            const int id = ids[it.first + threadIdx.x];
+            s_off[threadIdx.x] = offs[id];
+            s_nf[threadIdx.x] = nfill ? nfill[id] : (int)a.n_filler;
+        }
+        __syncthreads();
+        GroupIO io{it.count, 0, ids + it.first, offs, nfill, nullptr, nullptr};
+        io.t_off = s_off;
+        io.t_nf = s_nf;
+        // the next item is claimed once this one's iterations are over, so
+        // the atomic's round trip overlaps the output stores (claiming it
+        // any earlier would let one workgroup hold two items at the end of
+        // the queue while others idle)
+        int next = 0;
+        const auto claim = [&]() {
+            if (threadIdx.x == 0) next = atomicAdd(head, 1);
+        };
+        if (it.kind == SYN_KIND_BG1) item_group<SYN_KIND_BG1, I8>(Z1{}, code, it.count, io, a, claim);
+        else item_group<SYN_KIND_BG2, I8>(Z2{}, code, it.count, io, a, claim);
+        __syncthreads();  // this item is finished with LDS, s_item and the tables
+        if (threadIdx.x == 0) s_item = next;
     }
 }
 
