@@ -213,7 +213,6 @@ __global__ void __launch_bounds__(256) dequantize_i8(const int8_t *__restrict__ 
 const void *fast_kernel_f32_bg1(int z);
 const void *fast_kernel_f32_bg2(int z);
 const void *fast_kernel_i8_bg1(int z);
-const void *fast_kernel_i8_bg2(int z);
 const void *fast_kernel_wifi(int kind, bool i8);
 const void *pack_kernel_i8_bg2(int z);
 
