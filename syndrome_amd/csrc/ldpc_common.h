@@ -112,15 +112,20 @@ struct SynLaunch {
     float *trace;          // SYN_F_TRACE: f32 [B][iterations][N], row it = posterior after pass it+1
 };
 
+// Most codewords one work item of a work-queue launch holds (the size of
+// the kernel's per-item tables in LDS).
+#define SYN_ITEM_MAX_CWS 256
+
 // One work item of a work-queue launch: `count` codewords (ids
 // ids[first .. first+count)) of code `code` (an index into the launch's code
 // table), whose base graph is `kind`.
 struct SynItem {
     int code;
-    int kind;
+    int kind;  // the code's base-graph kind, | SYN_ITEM_PACKED for an int8 BG2 item decoded pair-packed
     int first;
     int count;
 };
+#define SYN_ITEM_PACKED 0x100
 
 // Resident serving decoder: one queued item = up to SYN_WORK_MAX code
 // blocks of one code, named by their server slots.  Items live in a ring in

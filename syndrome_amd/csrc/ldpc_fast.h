@@ -772,57 +772,4 @@ __device__ __forceinline__ void item_group(ZSet<ZS...>, const SynCode *__restric
     fast_group<KIND, 0, I8, !I8>(code, z, count, io, a, done);
 }
 
-// Persistent work-queue decoder: a grid sized to the resident workgroups
-// pulls items off a device counter until the queue is drained (every
-// workgroup reaches the exit test with the same counter, so all of them
-// leave), decoding each item with the register-resident body for its base
-// graph at its lifting size: a compile-time-Z body for the sizes in Z1
-// (BG1) and Z2 (BG2), the run-time-Z body for the rest.  Items of
-// different base graphs, lifting sizes and filler counts share one launch.
-// The int8 body keeps its register-lean form: both base graphs share one
-// kernel, whose VGPR budget the BG1 body sets.
-constexpr int PERSIST_MAX_CWS = 256;  // codewords of one work item: 512 lanes over Z >= 2
-
-template <bool I8, class Z1, class Z2>
-__device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, const SynItem *__restrict__ items,
-                                             int n_items, int *head, const int *__restrict__ ids,
-                                             const long long *__restrict__ offs, const int *__restrict__ nfill,
-                                             const SynLaunch &a)
-{
-    __shared__ int s_item;
-    // the item's per-codeword element offsets and filler counts, looked up
-    // once per item
-    __shared__ long long s_off[PERSIST_MAX_CWS];
-    __shared__ int s_nf[PERSIST_MAX_CWS];
-    if (threadIdx.x == 0) s_item = atomicAdd(head, 1);
-    for (;;) {
-        __syncthreads();  // s_item is written
-        const int w = s_item;
-        if (w >= n_items) return;
-        const SynItem it = items[w];
-        const SynCode *code = codes + it.code;
-        if ((int)threadIdx.x < it.count) {
-            const int id = ids[it.first + threadIdx.x];
-            s_off[threadIdx.x] = offs[id];
-            s_nf[threadIdx.x] = nfill ? nfill[id] : (int)a.n_filler;
-        }
-        __syncthreads();
-        GroupIO io{it.count, 0, ids + it.first, offs, nfill, nullptr, nullptr};
-        io.t_off = s_off;
-        io.t_nf = s_nf;
-        // the next item is claimed once this one's iterations are over, so
-        // the atomic's round trip overlaps the output stores (claiming it
-        // any earlier would let one workgroup hold two items at the end of
-        // the queue while others idle)
-        int next = 0;
-        const auto claim = [&]() {
-            if (threadIdx.x == 0) next = atomicAdd(head, 1);
-        };
-        if (it.kind == SYN_KIND_BG1) item_group<SYN_KIND_BG1, I8>(Z1{}, code, it.count, io, a, claim);
-        else item_group<SYN_KIND_BG2, I8>(Z2{}, code, it.count, io, a, claim);
-        __syncthreads();  // this item is finished with LDS, s_item and the tables
-        if (threadIdx.x == 0) s_item = next;
-    }
-}
-
 }  // namespace syn

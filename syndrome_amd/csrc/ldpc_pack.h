@@ -313,10 +313,11 @@ __device__ __forceinline__ void store_output_pk(const SynLaunch &a, int N, int p
 }
 
 // Decode one workgroup's `cws` codewords (cws / 2 pairs, cws even) of one
-// compile-time base graph, pair-packed.  Same contract as fast_group<.., I8>.
-template <int KIND, int ZC, bool WM = true>
+// compile-time base graph, pair-packed.  Same contract as fast_group<.., I8>
+// (`done` runs before the outputs are stored).
+template <int KIND, int ZC, bool WM = true, class Hook = NoHook>
 __device__ __forceinline__ void pack_group(const SynCode *__restrict__ code, int Z, int cws, const GroupIO &io,
-                                           const SynLaunch &a)
+                                           const SynLaunch &a, Hook done = {})
 {
     using G = Graph<KIND>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -413,6 +414,7 @@ __device__ __forceinline__ void pack_group(const SynCode *__restrict__ code, int
             iterate(std::false_type{});
         }
     }
+    done();
     store_output_pk(a, N, pairs, io, P0);
     store_iters(a, fl, io);
     if ((a.flags & SYN_F_WRITE_EDGE) && lane_ok) {

@@ -119,9 +119,18 @@ constexpr int SPLIT_LANES = 256;
 constexpr int HEAD_STRIDE = 32;
 constexpr unsigned RUN_RING = 16;
 
-size_t group_bytes(const SynCode &c, int cws, bool i8)
+// An int8 BG2 item normally runs pair-packed (ldpc_pack.h): its codewords
+// go two to a lane, in 32-bit LDS words that interleave a pair.
+bool can_pack(const SynCode &c, bool i8) { return i8 && c.kind == SYN_KIND_BG2; }
+
+size_t group_bytes(const SynCode &c, int cws, bool i8, bool packed = false)
 {
-    const size_t esz = i8 ? 2 : 4, n = syn::code_n(c);
+    const size_t n = syn::code_n(c);
+    if (packed) {
+        const size_t pairs = (size_t)(cws + 1) / 2;
+        return (size_t)(((c.z * 4) + 15) & ~15) + pairs * n * 4 + 3 * sizeof(int) * 2 * pairs;
+    }
+    const size_t esz = i8 ? 2 : 4;
     const size_t pad = (size_t)(((c.z * (int)esz) + 15) & ~15);
     return pad + ((cws * n * esz + 3) & ~(size_t)3) + 3 * sizeof(int) * (size_t)cws;
 }
@@ -141,16 +150,55 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     const bool split = max_z > SPLIT_LANES && block > SPLIT_LANES;
     std::vector<int> cws(nc), part(nc, 0);
     size_t lds[2] = {0, 0};
+    // the dynamic LDS an item may take: the device's per-workgroup limit less
+    // the largest static LDS of the kernels this plan may launch (their item
+    // tables)
+    size_t lds_cap = (size_t)p->lds_limit, static_max = 0;
+    for (const void *fn : {syn::persist_kernel_i8(), syn::persist_kernel_f32_big(), syn::persist_kernel_f32_small()}) {
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, fn) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        static_max = std::max(static_max, (size_t)fa.sharedSizeBytes);
+    }
+    lds_cap -= std::min(lds_cap, static_max);
+    // The small codes' launch reserves its largest item's LDS for every
+    // workgroup, so an item there takes no more than lets as many
+    // SPLIT_LANES-lane workgroups share a CU as the kernel's registers allow;
+    // a code that cannot stay under that goes to the main launch.
+    size_t small_cap = lds_cap;
+    if (split) {
+        const void *fn_s = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32_small();
+        int wgs = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&wgs, fn_s, SPLIT_LANES, 0) != hipSuccess || wgs < 1) wgs = 1;
+        small_cap = std::min(lds_cap, (size_t)p->lds_limit / (size_t)wgs - std::min((size_t)p->lds_limit / wgs, static_max));
+    }
+    const int max_k = SYN_ITEM_MAX_CWS;
+    auto fit = [&](const SynCode &code, bool pk, int lanes, size_t cap) {
+        int k = std::min(max_k, std::max(1, lanes / code.z) * (pk ? 2 : 1));
+        while (k > 1 && group_bytes(code, k, p->i8, pk) > cap) --k;
+        return k;
+    };
+    auto fits = [&](const SynCode &code, bool pk, int lanes, size_t cap) {
+        return group_bytes(code, fit(code, pk, lanes, cap), p->i8, pk) <= cap;
+    };
+    std::vector<char> packed(nc, 0);
     for (size_t c = 0; c < nc; ++c) {
         const SynCode &code = p->codes[c];
+        bool pk = can_pack(code, p->i8);
         part[c] = split && code.z <= SPLIT_LANES ? 1 : 0;
+        if (part[c] && !fits(code, pk, SPLIT_LANES, small_cap)) {
+            // one pair of a large Z overfills the small launch's share of a
+            // CU: such a code runs one codeword per lane there, or else in
+            // the main launch
+            if (pk && fits(code, false, SPLIT_LANES, small_cap)) pk = false;
+            else part[c] = 0;
+        }
         const int lanes = part[c] ? SPLIT_LANES : block;
         if (code.z > lanes) return SYNDROME_INVALID_PARAM;
-        int k = std::max(1, lanes / code.z);
-        while (k > 1 && group_bytes(code, k, p->i8) > (size_t)p->lds_limit) --k;
-        if (group_bytes(code, k, p->i8) > (size_t)p->lds_limit) return SYNDROME_INVALID_PARAM;
+        const int k = fit(code, pk, lanes, part[c] ? small_cap : lds_cap);
+        if (group_bytes(code, k, p->i8, pk) > lds_cap) return SYNDROME_INVALID_PARAM;
         cws[c] = k;
-        lds[part[c]] = std::max(lds[part[c]], group_bytes(code, k, p->i8));
+        packed[c] = pk;
+        lds[part[c]] = std::max(lds[part[c]], group_bytes(code, k, p->i8, pk));
     }
     std::vector<std::vector<int>> members(nc);
     for (size_t i = 0; i < p->cb_code.size(); ++i) members[p->cb_code[i]].push_back((int)i);
@@ -164,7 +212,7 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         for (size_t j = 0; j < members[c].size(); j += (size_t)cws[c]) {
             const int cnt = (int)std::min(members[c].size() - j, (size_t)cws[c]);
             Item x;
-            x.it = SynItem{(int)c, p->codes[c].kind, 0, cnt};
+            x.it = SynItem{(int)c, p->codes[c].kind | (packed[c] ? SYN_ITEM_PACKED : 0), 0, cnt};
             x.cost = (long long)p->codes[c].n_edges * p->codes[c].z * ((cnt + 0LL));
             x.it.first = (int)ids.size();
             for (int q = 0; q < cnt; ++q) ids.push_back(members[c][j + q]);
