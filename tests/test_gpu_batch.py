@@ -105,6 +105,51 @@ def test_plan_i8_mixed_matches_oracle(dev_ok, oracle):
         assert np.array_equal(hard[o:o + n], rh), f"block {i}: hard differs"
 
 
+def test_plan_i8_packed_and_one_per_lane_items(dev_ok, oracle):
+    """int8 BG2 items run pair-packed, except where one pair would overfill
+    the small launch's share of a CU (BG2 Z=256 beside a Z > 256 code):
+    those run one codeword per lane.  Odd counts leave a pair with one live
+    codeword.  Every block equals the oracle."""
+    import syndrome_amd._lib as L
+    from syndrome_amd import DecodePlan, QuantParams
+    codes = [(1, 256)] * 3 + [(0, 384)] + [(1, 13)] * 5 + [(1, 104)] * 3 + [(1, 384)] * 3
+    rng = np.random.default_rng(17)
+    bg, z, nf, blocks = [], [], [], []
+    for i, (b, zz) in enumerate(codes):
+        code = oracle.code_3gpp(b, zz)
+        f = int(rng.integers(0, min(32, code.k // 4) + 1))
+        info = oracle.xorshift_bits(4242 + i, code.k)
+        info[code.k - f:] = 0
+        llr = oracle.awgn(float(rng.uniform(0.0, 3.0)), code.k / code.n, 99 + 13 * i, oracle.encode(code, info))
+        bg.append(b)
+        z.append(zz)
+        nf.append(f)
+        blocks.append(llr)
+    offs = np.concatenate([[0], np.cumsum([x.size for x in blocks])[:-1]]).astype(np.int64)
+    arena = np.concatenate(blocks).astype(np.float32)
+    bg, z, nf = np.array(bg), np.array(z), np.array(nf)
+    plan = DecodePlan(bg, z, nf, offs, 0.5, int8=True)
+    say(f"[plan i8 packed] {len(bg)} blocks, shape {plan.shape()}")
+    d_in = torch.from_numpy(arena).cuda()
+    d_post = torch.empty(arena.size, dtype=torch.int16, device="cuda")
+    d_hard = torch.empty(arena.size, dtype=torch.uint8, device="cuda")
+    d_it = torch.empty(len(bg), dtype=torch.int32, device="cuda")
+    for clamp in (32767, 300):
+        plan.run(d_in, d_post, d_hard, d_it, 8, L.INIT_5G | L.WRITE_POST | L.WRITE_HARD | L.QUANTIZE,
+                 quant=QuantParams(8.0, clamp))
+        torch.cuda.synchronize()
+        post, hard, its = d_post.cpu().numpy(), d_hard.cpu().numpy(), d_it.cpu().numpy()
+        for i in range(len(bg)):
+            code = oracle.code_3gpp(int(bg[i]), int(z[i]))
+            app = oracle.init_5g_i8(code, oracle.quantize_i16(blocks[i], 8.0), int(nf[i]), clamp)
+            ri, ra, _, rh = oracle.decode_i8(code, 0.5, app, 8, 8.0, clamp)
+            o, n = offs[i], code.n
+            tag = f"clamp {clamp} block {i} (BG{bg[i] + 1} Z={z[i]})"
+            assert its[i] == ri, f"{tag}: iterations {its[i]} != {ri}"
+            assert np.array_equal(post[o:o + n], ra), f"{tag}: app differs"
+            assert np.array_equal(hard[o:o + n], rh), f"{tag}: hard differs"
+
+
 def test_device_dematch_matches_oracle(dev_ok, oracle):
     from syndrome_amd import DecodePlan
     say("[dematch] 20 code blocks, Qm in {1,2,4,6}, rv 0..3, repetition and puncturing")
