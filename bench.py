@@ -1012,12 +1012,16 @@ def main():
     mean_it = float(its.mean())
     value = B * k / (ms * 1e-3) / 1e9
 
-    # roofline of the dominant kernel, fast_f32<BG1, 384>: VALU-bound (the
-    # check-to-variable messages stay in registers, so HBM sees only the
-    # compulsory N*4 input + N*4 posterior + N*1 hard-decision bytes)
+    # roofline of the dominant kernel, fastx_f32<BG1, 384>: VALU-bound (the
+    # check-to-variable messages stay in registers, so HBM sees the
+    # compulsory N*4 input + N*4 posterior + N*1 hard-decision bytes, and
+    # the 42 extension columns' posteriors, which this kernel keeps in the
+    # posterior buffer: one 4-byte read and one write per extension
+    # variable and executed iteration, mostly served by L2)
     kern_ms = float(per.mean())
-    roof = valu_roofline("syn::fast_f32<0, 384> (BG1, Z=384)", "f32", float(its.sum()) * ez, kern_ms,
-                         float(B) * n * 9, "fast_f32_bg1_z384_b4096")
+    roof = valu_roofline("syn::fastx_f32<0, 384> (BG1, Z=384)", "f32", float(its.sum()) * ez, kern_ms,
+                         float(B) * n * 9, "fastx_f32_bg1_z384_b4096")
+    roof["hbm"]["extension_bytes_per_launch"] = float(its.sum()) * 42 * 384 * 8
 
     out = {}
     if args.copy_calibrate:

@@ -215,6 +215,8 @@ const void *fast_kernel_f32_bg2(int z);
 const void *fast_kernel_i8_bg1(int z);
 const void *fast_kernel_wifi(int kind, bool i8);
 const void *pack_kernel_i8_bg2(int z);
+const void *fastx_kernel_f32_bg1(int z);
+int fastx_shape_f32_bg1(int z, int *lanes, int *cws);
 
 // The kernel a decode of (kind, format, Z) runs: plain, or the
 // register-resident one for the base graph (its Z-specialised instance when
@@ -255,10 +257,23 @@ int allow_max_dynamic_lds(const void *fn)
 int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, const SynLaunch &a, int block,
                   size_t lds, hipStream_t stream)
 {
-    const int grid = (a.batch + a.cws - 1) / a.cws;
+    int grid = (a.batch + a.cws - 1) / a.cws;
     if (grid <= 0) return SYN_OK;
     const void *fn = decode_kernel(kind, i8, fast, z);
     if (!fn) return SYN_INVALID_PARAM;
+    // BG1 f32 with one codeword per workgroup, the posterior written back
+    // and at least one iteration: the kernel that keeps the extension
+    // columns in that posterior buffer (ldpc_fastx.h), two per CU
+    if (fast && !i8 && kind == SYN_KIND_BG1 && a.cws == 1 && (a.flags & SYN_F_WRITE_POST) && a.post &&
+        a.iterations >= 1 && !(a.flags & SYN_F_TRACE)) {
+        if (const void *fx = fastx_kernel_f32_bg1(z)) {
+            int lanes = 0, per = 1;
+            lds = (size_t)fastx_shape_f32_bg1(z, &lanes, &per);
+            fn = fx;
+            block = lanes;
+            grid = (a.batch + per - 1) / per;
+        }
+    }
     if (int rc = allow_max_dynamic_lds(fn)) return rc;
     SynLaunch arg = a;
     void *args[] = {(void *)&d_code, (void *)&arg};
