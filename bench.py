@@ -1056,7 +1056,8 @@ def main():
                                 args.warmup, torch)
     qms = qwall * 1e3 / args.steps
     out["persistent"] = {"value": round(B * k / (qms * 1e-3) / 1e9, 4), "unit": "Gbit/s", "ms_per_step": round(qms, 4),
-                         "kernel": "syn::persist_decode<false, 0>", "shape": plan.shape(),
+                         "kernel": "syn::persistx_f32<0, 384> (the two-codeword kernel's work-queue form)",
+                         "shape": plan.shape(),
                          "vs_per_config_kernel": round(B * k / (qms * 1e-3) / 1e9 / value, 4),
                          "identical_decoded_bits": bool(torch.equal(d_hard3, d_hard) and torch.equal(d_it3, d_it) and
                                                         torch.equal(d_post3.view(torch.int32), d_post.view(torch.int32)))}

@@ -451,15 +451,26 @@ __host__ __device__ constexpr int fastx_group_lds_bytes()
     return FASTX_CPW * fastx_region_bytes<KIND, ZC>();
 }
 
-// FASTX_CPW codewords per workgroup of FASTX_CPW * Z lanes, codeword
-// blockIdx.x * FASTX_CPW + t / Z.  The launch needs SYN_F_WRITE_POST (the
-// posterior buffer holds the extension columns during the decode) and at
-// least one iteration.  The codewords share the layer barriers; a
-// converged one stops (its waves skip the layers) while its partner goes
-// on, as in qc_ldpc.rs:1211 each stops at its own iteration.
-template <int KIND, int ZC>
-__global__ void __launch_bounds__(FASTX_CPW * ZC) __attribute__((amdgpu_waves_per_eu(3, 3)))
-fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
+// Where a workgroup's two codewords come from and go to: per lane, its own
+// codeword's slot (whether it exists, element offset of its input /
+// posterior / hard decisions, filler count, index for iterations and
+// edge_r) and whether the partner slot exists.
+struct XSlot {
+    bool live, live_o;
+    long long off, nf;
+    int id;
+};
+
+// FASTX_CPW codewords per workgroup of FASTX_CPW * Z lanes, the lane's
+// codeword t / Z.  The launch needs SYN_F_WRITE_POST (the posterior buffer
+// holds the extension columns during the decode) and at least one
+// iteration.  The codewords share the layer barriers; a converged one stops
+// (its waves skip the layers) while its partner goes on, as in
+// qc_ldpc.rs:1211 each stops at its own iteration.  `done` runs once the
+// iterations are over, before the outputs are stored (a work-queue
+// workgroup claims its next item there).
+template <int KIND, int ZC, class Hook>
+__device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl, Hook done_hook)
 {
     using G = Graph<KIND>;
     static_assert(kX<G>.ok, "degree-1 columns: one per row, shift 0");
@@ -474,16 +485,15 @@ fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
     const int cwl = t >= ZC ? 1 : 0;  // FASTX_CPW == 2
     static_assert(FASTX_CPW == 2, "two codewords per workgroup");
     const int z = t - cwl * ZC;
-    const int cw = blockIdx.x * FASTX_CPW + cwl;
-    const bool live = cw < a.batch;
+    const bool live = sl.live;
     unsigned char *R0 = smem + cwl * RB;  // this codeword's LDS region
     float *P0 = (float *)(R0 + pad);
     volatile int *fail = (volatile int *)(P0 + NC);
     // the partner codeword's fail flags
     volatile int *fail_o = (volatile int *)((float *)(smem + (1 - cwl) * RB + pad) + NC);
-    const long long off = (long long)(live ? cw : 0) * N;
+    const long long off = sl.off;
     const bool init5g = (a.flags & SYN_F_INIT_5G) != 0;
-    const long long nf = a.n_filler;
+    const long long nf = sl.nf;
     const int kp = nf >= K ? 0 : K - (int)nf;
     const float *in = (const float *)a.in + off;
     float *post = (float *)a.post + off;
@@ -557,7 +567,7 @@ fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
     // done: this codeword (wave-uniform) and the partner; a codeword past
     // the batch starts done
     bool done = !live;
-    bool done_o = (blockIdx.x * FASTX_CPW + (1 - cwl)) >= a.batch;
+    bool done_o = !sl.live_o;
     int used = 0;
     for (int it = 0; it < a.iterations; ++it) {
         if (!done) used = it + 1;
@@ -603,6 +613,7 @@ fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
         done_o = done_o || fail_o[it & 1] == 0;
         if (done && done_o) break;
     }
+    done_hook();
     __syncthreads();
     if (!live) return;
 
@@ -630,9 +641,9 @@ fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
             constexpr int k = decltype(kc)::value;
             a.hard[off + x_col_of<G>(k) * ZC + z] = (uint8_t)((rec[k / 32] >> x_bit<G>(k)) & 1u);
         });
-    if (z == 0) a.iters[cw] = used;
+    if (z == 0) a.iters[sl.id] = used;
     if (a.flags & SYN_F_WRITE_EDGE) {
-        const long long ebase = (long long)cw * G::EDGES * ZC;
+        const long long ebase = (long long)sl.id * G::EDGES * ZC;
         SFor<0, G::ROWS>::run([&](auto lc) {
             constexpr int L = decltype(lc)::value;
             constexpr int D = G::deg(L), B = layer_begin<G>(L);
@@ -650,6 +661,57 @@ fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
                 ((float *)a.edge)[ebase + (long long)(B + e) * ZC + z] = r_f32<D>(g1, g2, sw, (sw >> D) & 31u, e);
             });
         });
+    }
+}
+
+
+// The per-config kernel: codewords blockIdx.x * FASTX_CPW + t / Z of a
+// contiguous batch.
+template <int KIND, int ZC>
+__global__ void __launch_bounds__(FASTX_CPW * ZC) __attribute__((amdgpu_waves_per_eu(3, 3)))
+fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
+{
+    constexpr int N = Graph<KIND>::COLS * ZC;
+    const int cwl = (int)threadIdx.x >= ZC ? 1 : 0;
+    const int cw = blockIdx.x * FASTX_CPW + cwl;
+    XSlot sl;
+    sl.live = cw < a.batch;
+    sl.live_o = blockIdx.x * FASTX_CPW + (1 - cwl) < a.batch;
+    sl.id = sl.live ? cw : 0;
+    sl.off = (long long)sl.id * N;
+    sl.nf = a.n_filler;
+    fastx_group<KIND, ZC>(a, sl, NoHook{});
+}
+
+// The work-queue form: workgroups pull items of one or two BG1 Z=384
+// codewords (a plan's arena offsets and filler counts) off a device
+// counter until the queue is drained, as persist_body does.
+template <int KIND, int ZC>
+__global__ void __launch_bounds__(FASTX_CPW * ZC) __attribute__((amdgpu_waves_per_eu(3, 3)))
+persistx_f32(const SynCode *__restrict__ codes, const SynItem *__restrict__ items, int n_items, int *head,
+             const int *__restrict__ ids, const long long *__restrict__ offs, const int *__restrict__ nfill,
+             SynLaunch a)
+{
+    __shared__ int s_item;
+    if (threadIdx.x == 0) s_item = atomicAdd(head, 1);
+    const int cwl = (int)threadIdx.x >= ZC ? 1 : 0;
+    for (;;) {
+        __syncthreads();  // s_item is written; the previous item is done with LDS
+        const int w = s_item;
+        if (w >= n_items) return;
+        const SynItem it = items[w];
+        XSlot sl;
+        sl.live = cwl < it.count;
+        sl.live_o = (1 - cwl) < it.count;
+        sl.id = sl.live ? ids[it.first + cwl] : 0;
+        sl.off = sl.live ? offs[sl.id] : 0;
+        sl.nf = sl.live ? (nfill ? nfill[sl.id] : a.n_filler) : 0;
+        int next = 0;
+        fastx_group<KIND, ZC>(a, sl, [&]() {
+            if (threadIdx.x == 0) next = atomicAdd(head, 1);
+        });
+        __syncthreads();  // every lane has read this item
+        if (threadIdx.x == 0) s_item = next;
     }
 }
 

@@ -10,6 +10,12 @@ const void *fastx_kernel_f32_bg1(int z)
     return nullptr;
 }
 
+const void *persistx_kernel_f32_bg1(int z)
+{
+    if (z == 384) return (const void *)persistx_f32<SYN_KIND_BG1, 384>;
+    return nullptr;
+}
+
 // LDS bytes and lanes of one workgroup, and codewords per workgroup
 int fastx_shape_f32_bg1(int z, int *lanes, int *cws)
 {

@@ -20,6 +20,8 @@ namespace syn {
 const void *persist_kernel_f32_big();
 const void *persist_kernel_f32_small();
 const void *persist_kernel_i8();
+const void *persistx_kernel_f32_bg1(int z);
+int fastx_shape_f32_bg1(int z, int *lanes, int *cws);
 int launch_reset(int *head, hipStream_t stream);
 int allow_max_dynamic_lds(const void *fn);
 void dematch_derive(SynDematchCb &cb);
@@ -53,10 +55,20 @@ struct SyndromeLdpcPlan {
     // the kernel of each launch: f32 codes with Z > 256 in a launch take the
     // kernel whose compile-time-Z bodies are the large sizes
     const void *fn_main = nullptr, *fn_small = nullptr;
+    // f32 BG1 Z=384 codes: a third launch of the two-codeword kernel that
+    // keeps the extension columns in the posterior buffer (ldpc_fastx.h),
+    // for runs that write the posterior back; other runs decode them in the
+    // main launch, one codeword per workgroup: the *_mx list is the main
+    // list with them (the same as the main list when the plan has none)
+    int block_x = 0, grid_x = 0, n_items_x = 0;
+    size_t lds_x = 0;
+    int grid_mx = 0, n_items_mx = 0;
+    size_t lds_mx = 0;
+    const void *fn_x = nullptr, *fn_mx = nullptr;
     std::vector<int> cws_of_code;
     // device buffers
     SynCode *d_codes = nullptr;
-    SynItem *d_items = nullptr, *d_items_s = nullptr;
+    SynItem *d_items = nullptr, *d_items_s = nullptr, *d_items_x = nullptr, *d_items_mx = nullptr;
     int *d_ids = nullptr, *d_nfill = nullptr, *d_head = nullptr;
     long long *d_offs = nullptr;
     SynDematchCb *d_dm = nullptr;
@@ -64,7 +76,7 @@ struct SyndromeLdpcPlan {
     int n_dm_chunks = 0;
     SynTb *d_tbs = nullptr;
     int n_tbs = 0;
-    // queue counters: RUN_RING runs, two launches each, one 128-byte line
+    // queue counters: RUN_RING runs, three launches each, one 128-byte line
     // per counter; run k uses ring entry k % RUN_RING
     unsigned run_seq = 0;
     // device tables replaced by tune / set_dematch / set_tbs: kept until the
@@ -118,6 +130,10 @@ int32_t upload(SyndromeLdpcPlan *p, T **dst, const std::vector<T> &src)
 constexpr int SPLIT_LANES = 256;
 constexpr int HEAD_STRIDE = 32;
 constexpr unsigned RUN_RING = 16;
+constexpr int LAUNCHES = 3;
+
+// A code of the two-codeword extension-in-HBM kernel's launch.
+bool x_code(const SynCode &c, bool i8) { return !i8 && c.kind == SYN_KIND_BG1 && syn::persistx_kernel_f32_bg1(c.z); }
 
 // An int8 BG2 item normally runs pair-packed (ldpc_pack.h): its codewords
 // go two to a lane, in 32-bit LDS words that interleave a pair.
@@ -148,8 +164,8 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     int max_z = 0;
     for (const SynCode &code : p->codes) max_z = std::max(max_z, code.z);
     const bool split = max_z > SPLIT_LANES && block > SPLIT_LANES;
-    std::vector<int> cws(nc), part(nc, 0);
-    size_t lds[2] = {0, 0};
+    std::vector<int> cws(nc), part(nc, 0), cws_mx(nc, 0);
+    size_t lds[3] = {0, 0, 0};
     // the dynamic LDS an item may take: the device's per-workgroup limit less
     // the largest static LDS of the kernels this plan may launch (their item
     // tables)
@@ -198,38 +214,65 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         if (group_bytes(code, k, p->i8, pk) > lds_cap) return SYNDROME_INVALID_PARAM;
         cws[c] = k;
         packed[c] = pk;
+        if (x_code(code, p->i8)) {
+            // the extension-in-HBM launch; cws_mx: its main-launch form
+            int xl = 0, xc = 1;
+            const size_t xb = (size_t)syn::fastx_shape_f32_bg1(code.z, &xl, &xc);
+            part[c] = 2;
+            cws_mx[c] = k;
+            cws[c] = xc;
+            lds[2] = std::max(lds[2], xb);
+            p->block_x = xl;
+            continue;
+        }
         lds[part[c]] = std::max(lds[part[c]], group_bytes(code, k, p->i8, pk));
     }
+    size_t lds_mx = lds[0];
+    for (size_t c = 0; c < nc; ++c)
+        if (part[c] == 2) lds_mx = std::max(lds_mx, group_bytes(p->codes[c], cws_mx[c], false));
     std::vector<std::vector<int>> members(nc);
     for (size_t i = 0; i < p->cb_code.size(); ++i) members[p->cb_code[i]].push_back((int)i);
     struct Item {
         SynItem it;
         long long cost;
     };
-    std::vector<Item> items[2];
+    // items[3]: the main list with the extension-launch codes in main-launch
+    // form (its *_mx list)
+    std::vector<Item> items[4];
     std::vector<int> ids;
-    for (size_t c = 0; c < nc; ++c)
-        for (size_t j = 0; j < members[c].size(); j += (size_t)cws[c]) {
-            const int cnt = (int)std::min(members[c].size() - j, (size_t)cws[c]);
+    auto add = [&](size_t c, size_t j0, int k, int q, bool pk) {
+        for (size_t j = 0; j < members[c].size(); j += (size_t)k) {
+            const int cnt = (int)std::min(members[c].size() - j, (size_t)k);
             Item x;
-            x.it = SynItem{(int)c, p->codes[c].kind | (packed[c] ? SYN_ITEM_PACKED : 0), 0, cnt};
+            x.it = SynItem{(int)c, p->codes[c].kind | (pk ? SYN_ITEM_PACKED : 0), (int)(j0 + j), cnt};
             x.cost = (long long)p->codes[c].n_edges * p->codes[c].z * ((cnt + 0LL));
-            x.it.first = (int)ids.size();
-            for (int q = 0; q < cnt; ++q) ids.push_back(members[c][j + q]);
-            items[part[c]].push_back(x);
+            items[q].push_back(x);
         }
-    std::vector<SynItem> flat[2];
-    for (int q = 0; q < 2; ++q) {
+    };
+    for (size_t c = 0; c < nc; ++c) {
+        const size_t j0 = ids.size();
+        for (int m : members[c]) ids.push_back(m);
+        add(c, j0, cws[c], part[c], packed[c]);
+        if (part[c] == 2) add(c, j0, cws_mx[c], 3, false);
+        else if (part[c] == 0) add(c, j0, cws[c], 3, packed[c]);
+    }
+    std::vector<SynItem> flat[4];
+    for (int q = 0; q < 4; ++q) {
         // longest-processing-time first: the queue hands out big items early
         std::stable_sort(items[q].begin(), items[q].end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
         for (auto &x : items[q]) flat[q].push_back(x.it);
     }
+    const bool has_x = !flat[2].empty();
     if (int32_t rc = upload(p, &p->d_items, flat[0])) return rc;
     if (int32_t rc = upload(p, &p->d_items_s, flat[1])) return rc;
+    if (int32_t rc = upload(p, &p->d_items_x, flat[2])) return rc;
+    if (int32_t rc = upload(p, &p->d_items_mx, has_x ? flat[3] : std::vector<SynItem>())) return rc;
     if (int32_t rc = upload(p, &p->d_ids, ids)) return rc;
-    int main_z = 0;  // largest Z in the main launch
-    for (size_t c = 0; c < nc; ++c)
+    int main_z = 0, mx_z = 0;  // largest Z in the main launch, without and with the extension-launch codes
+    for (size_t c = 0; c < nc; ++c) {
         if (!part[c]) main_z = std::max(main_z, p->codes[c].z);
+        if (part[c] != 1) mx_z = std::max(mx_z, p->codes[c].z);
+    }
     const void *fn = p->i8 ? syn::persist_kernel_i8()
                            : (main_z > SPLIT_LANES ? syn::persist_kernel_f32_big() : syn::persist_kernel_f32_small());
     const void *fn_s = p->i8 ? syn::persist_kernel_i8() : syn::persist_kernel_f32_small();
@@ -253,6 +296,31 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         p->grid_s = std::max(1, std::min(p->n_items_s, per_cu * p->cus));
     }
     p->cws_of_code = cws;
+    p->n_items_x = (int)flat[2].size();
+    p->lds_x = lds[2];
+    p->grid_x = 0;
+    p->fn_x = nullptr;
+    if (has_x) {
+        const void *fx = syn::persistx_kernel_f32_bg1(384);
+        if (syn::allow_max_dynamic_lds(fx)) return SYNDROME_DEVICE_ERROR;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fx, p->block_x, lds[2]) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        p->fn_x = fx;
+        p->grid_x = std::max(1, std::min(p->n_items_x, per_cu * p->cus));
+        const void *fm = mx_z > SPLIT_LANES ? syn::persist_kernel_f32_big() : syn::persist_kernel_f32_small();
+        if (syn::allow_max_dynamic_lds(fm)) return SYNDROME_DEVICE_ERROR;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fm, block, lds_mx) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        p->fn_mx = fm;
+        p->n_items_mx = (int)flat[3].size();
+        p->lds_mx = lds_mx;
+        p->grid_mx = std::max(1, std::min(p->n_items_mx, per_cu * p->cus));
+    } else {
+        p->fn_mx = p->fn_main;
+        p->n_items_mx = p->n_items;
+        p->lds_mx = p->lds;
+        p->grid_mx = p->grid;
+    }
     return SYNDROME_OK;
 }
 
@@ -284,19 +352,32 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
     const int *ids = p->d_ids;
     const long long *offs = p->d_offs;
     const int *nf = p->d_nfill;
-    // this run's pair of counters: runs in flight at once (on different
-    // streams, or a graph replay beside an eager run) never share one
-    int *heads = p->d_head + (size_t)(p->run_seq++ % RUN_RING) * 2 * HEAD_STRIDE;
-    // the main launch, then the small codes' (each with its own counter)
-    for (int q = 0; q < 2; ++q) {
-        const SynItem *items = q ? p->d_items_s : p->d_items;
-        int n_items = q ? p->n_items_s : p->n_items;
+    // this run's counters: runs in flight at once (on different streams, or
+    // a graph replay beside an eager run) never share one
+    int *heads = p->d_head + (size_t)(p->run_seq++ % RUN_RING) * LAUNCHES * HEAD_STRIDE;
+    // the extension-in-HBM launch needs the posterior buffer and an
+    // iteration; without them its codes go in the main launch
+    const bool use_x = p->n_items_x > 0 && (a.flags & SYN_F_WRITE_POST) && a.iterations >= 1;
+    struct L {
+        const void *fn;
+        const SynItem *items;
+        int n, grid, block;
+        size_t lds;
+    } ls[LAUNCHES] = {
+        use_x ? L{p->fn_main, p->d_items, p->n_items, p->grid, p->block, p->lds}
+              : L{p->fn_mx, p->n_items_x ? p->d_items_mx : p->d_items, p->n_items_mx, p->grid_mx, p->block, p->lds_mx},
+        L{p->fn_small, p->d_items_s, p->n_items_s, p->grid_s, p->block_s, p->lds_s},
+        L{p->fn_x, p->d_items_x, use_x ? p->n_items_x : 0, p->grid_x, p->block_x, p->lds_x}};
+    // the main launch, the small codes', the extension launch (each with
+    // its own counter)
+    for (int q = 0; q < LAUNCHES; ++q) {
+        const SynItem *items = ls[q].items;
+        int n_items = ls[q].n;
         if (n_items == 0) continue;
         int *head = heads + q * HEAD_STRIDE;
         if (syn::launch_reset(head, stream)) return SYNDROME_DEVICE_ERROR;
         void *args[] = {&codes, &items, &n_items, &head, &ids, &offs, &nf, &a};
-        if (hipLaunchKernel(q ? p->fn_small : p->fn_main, dim3(q ? p->grid_s : p->grid), dim3(q ? p->block_s : p->block), args,
-                            q ? p->lds_s : p->lds, stream) != hipSuccess)
+        if (hipLaunchKernel(ls[q].fn, dim3(ls[q].grid), dim3(ls[q].block), args, ls[q].lds, stream) != hipSuccess)
             return SYNDROME_DEVICE_ERROR;
     }
     return SYNDROME_OK;
@@ -347,7 +428,7 @@ int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int
         if (int32_t rc = upload(p.get(), &p->d_codes, p->codes)) return rc;
         if (int32_t rc = upload(p.get(), &p->d_offs, p->offs)) return rc;
         if (int32_t rc = upload(p.get(), &p->d_nfill, p->nfill)) return rc;
-        if (hipMalloc((void **)&p->d_head, RUN_RING * 2 * HEAD_STRIDE * sizeof(int)) != hipSuccess)
+        if (hipMalloc((void **)&p->d_head, RUN_RING * LAUNCHES * HEAD_STRIDE * sizeof(int)) != hipSuccess)
             return SYNDROME_DEVICE_ERROR;
         const int min_block = ((max_z + 63) / 64) * 64;
         const int b = block > 0 ? block : std::max(min_block, 512);
@@ -362,7 +443,8 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *p)
     if (!p) return;
     try {
         DeviceScope ds(p->device);
-        for (void *q : {(void *)p->d_codes, (void *)p->d_items, (void *)p->d_items_s, (void *)p->d_ids, (void *)p->d_nfill,
+        for (void *q : {(void *)p->d_codes, (void *)p->d_items, (void *)p->d_items_s, (void *)p->d_items_x,
+                        (void *)p->d_items_mx, (void *)p->d_ids, (void *)p->d_nfill,
                         (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm, (void *)p->d_dm_chunks,
                         (void *)p->d_tbs})
             if (q) (void)hipFree(q);
@@ -391,10 +473,13 @@ int32_t syndrome_ldpc_plan_shape(const SyndromeLdpcPlan *p, int32_t *block, int3
 {
     return guard([&]() -> int32_t {
         if (!p) return SYNDROME_NULL_POINTER;
-        if (block) *block = p->block;
-        if (grid) *grid = p->grid;
-        if (items) *items = p->n_items + p->n_items_s;  // both launches' items
-        if (lds_bytes) *lds_bytes = (int32_t)p->lds;
+        // the main launch's shape, or the extension launch's when the main
+        // launch has no items
+        const bool x = p->n_items == 0 && p->n_items_x > 0;
+        if (block) *block = x ? p->block_x : p->block;
+        if (grid) *grid = x ? p->grid_x : p->grid;
+        if (items) *items = p->n_items + p->n_items_s + p->n_items_x;  // every launch's items
+        if (lds_bytes) *lds_bytes = (int32_t)(x ? p->lds_x : p->lds);
         return SYNDROME_OK;
     });
 }

@@ -151,3 +151,47 @@ def test_fastx_one_iteration_and_cap(dev, oracle):
         post, hard, its, edge = _run(dec, d_in, torch.empty_like(d_in), iters, L.WRITE_POST | L.WRITE_HARD)
         _check_oracle(oracle, 0.25, llr, post, hard, its, edge, iters, init5g=False, label=f"{iters} it")
     assert oracle.fnv1a(hard[0]) == 0xA2448E8CF5272825  # the reference's golden checksum
+
+
+def test_plan_extension_launch_with_and_without_posterior(dev, oracle):
+    """A work-queue plan decodes its f32 BG1 Z=384 blocks in a launch of
+    the two-codeword kernel when the run writes the posterior back, and in
+    the main launch, one codeword per workgroup, when it does not: the same
+    bits both ways, and each block as the oracle decodes it alone (fillers
+    included).  Three Z=384 blocks: the last item holds one codeword."""
+    import syndrome_amd._lib as L
+    from syndrome_amd import DecodePlan
+    codes = [(0, 384), (1, 208), (0, 384), (0, 52), (0, 384), (1, 384)]
+    rng = np.random.default_rng(21)
+    bg, z, nf, blocks = [], [], [], []
+    for i, (b, zz) in enumerate(codes):
+        code = oracle.code_3gpp(b, zz)
+        f = int(rng.integers(0, 40))
+        info = oracle.xorshift_bits(3100 + i, code.k)
+        info[code.k - f:] = 0
+        blocks.append(oracle.awgn(float(rng.uniform(0.5, 2.5)), code.k / code.n, 900 + i, oracle.encode(code, info)))
+        bg.append(b)
+        z.append(zz)
+        nf.append(f)
+    offs = np.concatenate([[0], np.cumsum([x.size for x in blocks])[:-1]]).astype(np.int64)
+    arena = np.concatenate(blocks).astype(np.float32)
+    plan = DecodePlan(np.array(bg), np.array(z), np.array(nf), offs, 0.5)
+    say("[fastx plan] shape", plan.shape())
+    d_in = torch.from_numpy(arena).cuda()
+    out = []
+    for flags in (L.INIT_5G | L.WRITE_POST | L.WRITE_HARD, L.INIT_5G | L.WRITE_HARD):
+        d_post = torch.zeros_like(d_in)
+        d_hard = torch.empty(arena.size, dtype=torch.uint8, device="cuda")
+        d_it = torch.empty(len(bg), dtype=torch.int32, device="cuda")
+        plan.run(d_in, d_post, d_hard, d_it, 8, flags)
+        torch.cuda.synchronize()
+        out.append((d_post.cpu().numpy(), d_hard.cpu().numpy(), d_it.cpu().numpy()))
+    (post, hard, its), (_, hard2, its2) = out
+    assert np.array_equal(its, its2) and np.array_equal(hard, hard2)
+    for i in range(len(bg)):
+        code = oracle.code_3gpp(bg[i], z[i])
+        ri, rp, _, rh = oracle.decode_f32(code, 0.5, oracle.init_5g_f32(code, blocks[i], nf[i]), 8)
+        o, n = offs[i], code.n
+        assert its[i] == ri, f"block {i}: iterations {its[i]} != {ri}"
+        assert np.array_equal(hard[o:o + n], rh), f"block {i}: hard differs"
+        assert np.array_equal(f32_bits(post[o:o + n]), f32_bits(rp)), f"block {i}: posterior differs"
