@@ -485,13 +485,18 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
     const int cwl = t >= ZC ? 1 : 0;  // FASTX_CPW == 2
     static_assert(FASTX_CPW == 2, "two codewords per workgroup");
     const int z = t - cwl * ZC;
-    const bool live = sl.live;
+    // a wave's lanes all belong to one codeword: its slot values are
+    // wave-uniform, and read as such (scalar registers; the buffer
+    // resources below would otherwise be per-lane values, which the
+    // compiler serves with a waterfall loop around every access)
+    const bool live = __builtin_amdgcn_readfirstlane(sl.live ? 1 : 0) != 0;
     unsigned char *R0 = smem + cwl * RB;  // this codeword's LDS region
     float *P0 = (float *)(R0 + pad);
     volatile int *fail = (volatile int *)(P0 + NC);
     // the partner codeword's fail flags
     volatile int *fail_o = (volatile int *)((float *)(smem + (1 - cwl) * RB + pad) + NC);
-    const long long off = sl.off;
+    const long long off = (long long)(((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(sl.off >> 32)) << 32) |
+                                      (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)sl.off));
     const bool init5g = (a.flags & SYN_F_INIT_5G) != 0;
     const long long nf = sl.nf;
     const int kp = nf >= K ? 0 : K - (int)nf;
@@ -609,7 +614,7 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
         lds_barrier();
         // zero syndrome: the codeword stops (qc_ldpc.rs:1211); a done
         // codeword never raises its flag again
-        done = done || fail[it & 1] == 0;
+        done = done || __builtin_amdgcn_readfirstlane(fail[it & 1]) == 0;
         done_o = done_o || fail_o[it & 1] == 0;
         if (done && done_o) break;
     }
