@@ -469,8 +469,8 @@ struct XSlot {
 // qc_ldpc.rs:1211 each stops at its own iteration.  `done` runs once the
 // iterations are over, before the outputs are stored (a work-queue
 // workgroup claims its next item there).
-template <int KIND, int ZC, class Hook>
-__device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl, Hook done_hook)
+template <int KIND, int ZC, class Hook, class IdFn>
+__device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl, Hook done_hook, IdFn id_of)
 {
     using G = Graph<KIND>;
     static_assert(kX<G>.ok, "degree-1 columns: one per row, shift 0");
@@ -646,9 +646,12 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
             constexpr int k = decltype(kc)::value;
             a.hard[off + x_col_of<G>(k) * ZC + z] = (uint8_t)((rec[k / 32] >> x_bit<G>(k)) & 1u);
         });
-    if (z == 0) a.iters[sl.id] = used;
+    // the codeword's index, fetched again here rather than held through the
+    // decode
+    const int id = id_of();
+    if (z == 0) a.iters[id] = used;
     if (a.flags & SYN_F_WRITE_EDGE) {
-        const long long ebase = (long long)sl.id * G::EDGES * ZC;
+        const long long ebase = (long long)id * G::EDGES * ZC;
         SFor<0, G::ROWS>::run([&](auto lc) {
             constexpr int L = decltype(lc)::value;
             constexpr int D = G::deg(L), B = layer_begin<G>(L);
@@ -685,7 +688,7 @@ fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
     sl.id = sl.live ? cw : 0;
     sl.off = (long long)sl.id * N;
     sl.nf = a.n_filler;
-    fastx_group<KIND, ZC>(a, sl, NoHook{});
+    fastx_group<KIND, ZC>(a, sl, NoHook{}, [&]() { return cw; });
 }
 
 // The work-queue form: workgroups pull items of one or two BG1 Z=384
@@ -712,9 +715,12 @@ persistx_f32(const SynCode *__restrict__ codes, const SynItem *__restrict__ item
         sl.off = sl.live ? offs[sl.id] : 0;
         sl.nf = sl.live ? (nfill ? nfill[sl.id] : a.n_filler) : 0;
         int next = 0;
-        fastx_group<KIND, ZC>(a, sl, [&]() {
-            if (threadIdx.x == 0) next = atomicAdd(head, 1);
-        });
+        fastx_group<KIND, ZC>(
+            a, sl,
+            [&]() {
+                if (threadIdx.x == 0) next = atomicAdd(head, 1);
+            },
+            [&]() { return sl.id; });
         __syncthreads();  // every lane has read this item
         if (threadIdx.x == 0) s_item = next;
     }
