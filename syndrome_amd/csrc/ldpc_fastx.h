@@ -438,6 +438,10 @@ constexpr int FASTX_KQ_BELOW = FASTX_KQB;
 // codewords has 12 waves, three on every SIMD.
 constexpr int FASTX_CPW = 2;
 
+// Lanes of one codeword's slot: Z rounded up to whole waves (a wave holds
+// lanes of one codeword only; lanes z >= Z of the last wave idle).
+__host__ __device__ constexpr int fastx_lanes(int z) { return (z + 63) & ~63; }
+
 // LDS of one codeword's region (16-byte multiple) and of a workgroup.
 template <int KIND, int ZC>
 __host__ __device__ constexpr int fastx_region_bytes()
@@ -461,8 +465,10 @@ struct XSlot {
     int id;
 };
 
-// FASTX_CPW codewords per workgroup of FASTX_CPW * Z lanes, the lane's
-// codeword t / Z.  The launch needs SYN_F_WRITE_POST (the posterior buffer
+// FASTX_CPW codewords per workgroup, the lane's codeword slot t / lanes(Z),
+// lifting index z = t % lanes(Z); lanes with z >= Z, and in a wider
+// work-queue workgroup whole waves past the slots, take part in the
+// barriers and the loop's exit only.  The launch needs SYN_F_WRITE_POST (the posterior buffer
 // holds the extension columns during the decode) and at least one
 // iteration.  The codewords share the layer barriers; a converged one stops
 // (its waves skip the layers) while its partner goes on, as in
@@ -474,17 +480,21 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
 {
     using G = Graph<KIND>;
     static_assert(kX<G>.ok, "degree-1 columns: one per row, shift 0");
-    static_assert(ZC % 64 == 0, "a wave holds lanes of one codeword");
     constexpr int C = kX<G>.core, NX = x_count<G>(), PF = FASTX_PF;
     constexpr int NC = C * ZC, N = G::COLS * ZC, K = (G::COLS - G::ROWS) * ZC;
     static_assert(NX >= PF && NX % PF == 0 && NC % 4 == 0, "shape: the prefetch slots repeat every iteration");
     constexpr int RB = fastx_region_bytes<KIND, ZC>();
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int pad = fast_pad(ZC, 4);
+    constexpr int LPC = fastx_lanes(ZC);
     const int t = threadIdx.x;
-    const int cwl = t >= ZC ? 1 : 0;  // FASTX_CPW == 2
     static_assert(FASTX_CPW == 2, "two codewords per workgroup");
-    const int z = t - cwl * ZC;
+    // idle: a wave past both slots (it follows slot 0's flags, so that its
+    // loop ends with the others)
+    const bool idle = t >= FASTX_CPW * LPC;
+    const int cwl = t >= LPC && !idle ? 1 : 0;
+    const int z = idle ? 0 : t - cwl * LPC;
+    const bool lane_ok = !idle && (LPC == ZC || z < ZC);  // a lane with a check row
     // a wave's lanes all belong to one codeword: its slot values are
     // wave-uniform, and read as such (scalar registers; the buffer
     // resources below would otherwise be per-lane values, which the
@@ -505,7 +515,7 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
 
     // core columns -> LDS, with the 5G prologue (it touches no extension
     // column: the punctured 2Z and the fillers lie below K)
-    if (live) {
+    if (live && lane_ok) {
         if ((a.flags & SYN_F_ALIGNED) != 0) {
             constexpr int U = 4, n4 = NC / 4;
             for (int b = z; b < n4; b += U * ZC) {
@@ -531,7 +541,7 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
             for (int i = z; i < NC; i += ZC) P0[i] = prologue_f32(in[i], i, 2 * ZC, kp, K, init5g);
         }
     }
-    if (z == 0) fail[0] = 0;
+    if (!idle && z == 0) fail[0] = 0;
 
     const int wave = __builtin_amdgcn_readfirstlane(z) >> 6;  // the wave's place in its codeword
     const int zb0 = (int)(R0 - smem) + pad + z * 4;
@@ -543,11 +553,12 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
             st.g2[L] = 0.f;
         }
     });
-    SFor<0, FASTX_SPILL>::run([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        P0[spill_offset<ZC, C>(j, 0) / 4 + z] = 0.f;
-        P0[spill_offset<ZC, C>(j, 1) / 4 + z] = 0.f;
-    });
+    if (lane_ok)
+        SFor<0, FASTX_SPILL>::run([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            P0[spill_offset<ZC, C>(j, 0) / 4 + z] = 0.f;
+            P0[spill_offset<ZC, C>(j, 1) / 4 + z] = 0.f;
+        });
     SFor<0, sg_words<G>()>::run([&](auto wc) { st.sg[decltype(wc)::value] = 0u; });
     SFor<0, G::ROWS>::run([&](auto lc) {
         constexpr int L = decltype(lc)::value;
@@ -564,15 +575,17 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
     const __amdgpu_buffer_rsrc_t r_in = __builtin_amdgcn_make_buffer_rsrc((void *)in, 0, N * 4, 0x00020000);
     const __amdgpu_buffer_rsrc_t r_post = __builtin_amdgcn_make_buffer_rsrc((void *)post, 0, N * 4, 0x00020000);
     // the first PF extension values of iteration 0
-    float xq[PF];
-    SFor<0, PF>::run([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        xq[k] = x_load(r_in, z * 4, x_col_of<G>(k) * ZC * 4);
-    });
+    float xq[PF] = {};
+    if (lane_ok)
+        SFor<0, PF>::run([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            xq[k] = x_load(r_in, z * 4, x_col_of<G>(k) * ZC * 4);
+        });
     // done: this codeword (wave-uniform) and the partner; a codeword past
     // the batch starts done
     bool done = !live;
     bool done_o = !sl.live_o;
+    const bool work = lane_ok;  // a lane that decodes (while its codeword is not done)
     int used = 0;
     for (int it = 0; it < a.iterations; ++it) {
         if (!done) used = it + 1;
@@ -586,7 +599,7 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
         const XMem xm{it == 0 ? r_in : r_post, r_post, opaque_v(z) * 4};
         SFor<0, G::ROWS>::run([&](auto lc) {
             constexpr int L = decltype(lc)::value;
-            if (!done) {
+            if (!done && work) {
                 float xv = 0.f;
                 if constexpr (row_has_x<G>(L)) {
                     constexpr int k = x_index<G>(L), kn = k + PF;
@@ -598,8 +611,8 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
             }
             if constexpr (barrier_after<G>(L)) lds_barrier();
         });
-        if (z == 0) fail[(it + 1) & 1] = 0;
-        if (!done) {
+        if (!idle && z == 0) fail[(it + 1) & 1] = 0;
+        if (!done && work) {
             // fresh opaque copies of the lane's bases and mask row for the
             // check: the compiler would otherwise keep the layers' 316 LDS
             // addresses alive (common subexpressions) until the check reads
@@ -620,7 +633,7 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
     }
     done_hook();
     __syncthreads();
-    if (!live) return;
+    if (!live || !lane_ok) return;
 
     // outputs: the core posterior and hard decisions from LDS, the
     // extension hard decisions from the bit record (its posterior is in
@@ -676,11 +689,11 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
 // The per-config kernel: codewords blockIdx.x * FASTX_CPW + t / Z of a
 // contiguous batch.
 template <int KIND, int ZC>
-__global__ void __launch_bounds__(FASTX_CPW * ZC) __attribute__((amdgpu_waves_per_eu(3, 3)))
+__global__ void __launch_bounds__(FASTX_CPW * fastx_lanes(ZC)) __attribute__((amdgpu_waves_per_eu(3, 3)))
 fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
 {
     constexpr int N = Graph<KIND>::COLS * ZC;
-    const int cwl = (int)threadIdx.x >= ZC ? 1 : 0;
+    const int cwl = (int)threadIdx.x >= fastx_lanes(ZC) ? 1 : 0;
     const int cw = blockIdx.x * FASTX_CPW + cwl;
     XSlot sl;
     sl.live = cw < a.batch;
@@ -691,36 +704,53 @@ fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
     fastx_group<KIND, ZC>(a, sl, NoHook{}, [&]() { return cw; });
 }
 
-// The work-queue form: workgroups pull items of one or two BG1 Z=384
-// codewords (a plan's arena offsets and filler counts) off a device
-// counter until the queue is drained, as persist_body does.
-template <int KIND, int ZC>
-__global__ void __launch_bounds__(FASTX_CPW * ZC) __attribute__((amdgpu_waves_per_eu(3, 3)))
+// The work-queue form: workgroups pull items of one or two BG1 codewords
+// of one lifting size in ZS (a plan's arena offsets and filler counts) off
+// a device counter until the queue is drained, as persist_body does.  The
+// workgroup has the lanes of the largest size's two slots; with a smaller
+// size the waves past its slots idle.
+template <int... ZS>
+constexpr int fastx_max_lanes()
+{
+    int m = 0;
+    ((m = m > FASTX_CPW * fastx_lanes(ZS) ? m : FASTX_CPW * fastx_lanes(ZS)), ...);
+    return m;
+}
+
+template <int KIND, int... ZS>
+__global__ void __launch_bounds__(fastx_max_lanes<ZS...>()) __attribute__((amdgpu_waves_per_eu(3, 3)))
 persistx_f32(const SynCode *__restrict__ codes, const SynItem *__restrict__ items, int n_items, int *head,
              const int *__restrict__ ids, const long long *__restrict__ offs, const int *__restrict__ nfill,
              SynLaunch a)
 {
     __shared__ int s_item;
     if (threadIdx.x == 0) s_item = atomicAdd(head, 1);
-    const int cwl = (int)threadIdx.x >= ZC ? 1 : 0;
     for (;;) {
         __syncthreads();  // s_item is written; the previous item is done with LDS
         const int w = s_item;
         if (w >= n_items) return;
         const SynItem it = items[w];
-        XSlot sl;
-        sl.live = cwl < it.count;
-        sl.live_o = (1 - cwl) < it.count;
-        sl.id = sl.live ? ids[it.first + cwl] : 0;
-        sl.off = sl.live ? offs[sl.id] : 0;
-        sl.nf = sl.live ? (nfill ? nfill[sl.id] : a.n_filler) : 0;
+        const int zc = codes[it.code].z;
         int next = 0;
-        fastx_group<KIND, ZC>(
-            a, sl,
-            [&]() {
-                if (threadIdx.x == 0) next = atomicAdd(head, 1);
-            },
-            [&]() { return sl.id; });
+        const auto body = [&](auto zk) {
+            constexpr int ZC = decltype(zk)::value;
+            const int slot = (int)threadIdx.x / fastx_lanes(ZC);  // 2 and up: idle, following slot 0
+            const int cwl = slot == 1 ? 1 : 0;
+            XSlot sl;
+            sl.live = cwl < it.count;
+            sl.live_o = (1 - cwl) < it.count;
+            const bool mine = slot < FASTX_CPW && sl.live;
+            sl.id = mine ? ids[it.first + cwl] : 0;
+            sl.off = mine ? offs[sl.id] : 0;
+            sl.nf = mine ? (nfill ? nfill[sl.id] : a.n_filler) : 0;
+            fastx_group<KIND, ZC>(
+                a, sl,
+                [&]() {
+                    if (threadIdx.x == 0) next = atomicAdd(head, 1);
+                },
+                [&]() { return sl.id; });
+        };
+        ((zc == ZS ? (body(std::integral_constant<int, ZS>{}), 0) : 0), ...);
         __syncthreads();  // every lane has read this item
         if (threadIdx.x == 0) s_item = next;
     }

@@ -21,6 +21,7 @@ const void *persist_kernel_f32_big();
 const void *persist_kernel_f32_small();
 const void *persist_kernel_i8();
 const void *persistx_kernel_f32_bg1(int z);
+int persistx_lanes_f32_bg1();
 int fastx_shape_f32_bg1(int z, int *lanes, int *cws);
 int launch_reset(int *head, hipStream_t stream);
 int allow_max_dynamic_lds(const void *fn);
@@ -55,7 +56,7 @@ struct SyndromeLdpcPlan {
     // the kernel of each launch: f32 codes with Z > 256 in a launch take the
     // kernel whose compile-time-Z bodies are the large sizes
     const void *fn_main = nullptr, *fn_small = nullptr;
-    // f32 BG1 Z=384 codes: a third launch of the two-codeword kernel that
+    // f32 BG1 codes of Z > 256: a third launch of the two-codeword kernel that
     // keeps the extension columns in the posterior buffer (ldpc_fastx.h),
     // for runs that write the posterior back; other runs decode them in the
     // main launch, one codeword per workgroup: the *_mx list is the main
@@ -222,7 +223,8 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
             cws_mx[c] = k;
             cws[c] = xc;
             lds[2] = std::max(lds[2], xb);
-            p->block_x = xl;
+            p->block_x = syn::persistx_lanes_f32_bg1();
+            (void)xl;
             continue;
         }
         lds[part[c]] = std::max(lds[part[c]], group_bytes(code, k, p->i8, pk));
@@ -301,7 +303,9 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     p->grid_x = 0;
     p->fn_x = nullptr;
     if (has_x) {
-        const void *fx = syn::persistx_kernel_f32_bg1(384);
+        const void *fx = nullptr;
+        for (size_t c = 0; c < nc && !fx; ++c)
+            if (part[c] == 2) fx = syn::persistx_kernel_f32_bg1(p->codes[c].z);
         if (syn::allow_max_dynamic_lds(fx)) return SYNDROME_DEVICE_ERROR;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fx, p->block_x, lds[2]) != hipSuccess || per_cu < 1)
             per_cu = 1;

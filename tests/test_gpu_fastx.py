@@ -1,8 +1,10 @@
 """The BG1 f32 kernel that keeps the 42 extension columns' posteriors in the
 posterior buffer (HBM) instead of LDS, two codewords per workgroup
-(syndrome_amd/csrc/ldpc_fastx.h).  It runs whenever a BG1 Z=384 f32 batch
-writes its posterior back; a batch that writes only hard decisions runs the
-all-LDS kernel, so the two can be set against each other on the same input.
+(syndrome_amd/csrc/ldpc_fastx.h).  It runs whenever a BG1 f32 batch of
+Z = 384, 352, 320 or 288 writes its posterior back; a batch that writes only
+hard decisions runs the all-LDS kernel, so the two can be set against each
+other on the same input.  At 352 and 288 the last wave of each codeword
+has idle lanes (the lifting size is not a whole number of waves).
 
 Checked against the oracle (qc_ldpc.rs:962-1252 restated) bit for bit:
 posteriors, final messages (edge_r), hard decisions, iteration counts, over
@@ -32,14 +34,15 @@ def dev():
     return torch.device("cuda:0")
 
 
-def _decoder(beta):
+def _decoder(beta, z=Z):
     import syndrome_amd
-    return syndrome_amd.QcLdpcDecoder.with_lifting_size(syndrome_amd.BG1, Z, beta)
+    return syndrome_amd.QcLdpcDecoder.with_lifting_size(syndrome_amd.BG1, z, beta)
 
 
-def _channel(dec, ebnos, seed):
+def _channel(dec, ebnos, seed, z=Z):
     """5G-style inputs: encoded random info bits, BPSK/AWGN, first 2Z zero."""
     from syndrome_amd import awgn_transmit
+    Z = z
     n, k = dec.variable_node_count(), dec.info_bit_count_5g()
     rng = np.random.default_rng(seed)
     info = rng.integers(0, 2, size=(len(ebnos), k), dtype=np.uint8)
@@ -63,8 +66,8 @@ def _run(dec, d_in, d_post, iters, flags, edge=True):
             d_edge.cpu().numpy() if edge else None)
 
 
-def _check_oracle(oracle, beta, llr_in, post, hard, its, edge, iters, init5g=True, label=""):
-    code = oracle.code_3gpp(0, Z)
+def _check_oracle(oracle, beta, llr_in, post, hard, its, edge, iters, init5g=True, label="", z=Z):
+    code = oracle.code_3gpp(0, z)
     for b in range(llr_in.shape[0]):
         x = oracle.init_5g_f32(code, llr_in[b], 0) if init5g else llr_in[b]
         ri, rp, re, rh = oracle.decode_f32(code, beta, x, iters)
@@ -195,3 +198,17 @@ def test_plan_extension_launch_with_and_without_posterior(dev, oracle):
         assert its[i] == ri, f"block {i}: iterations {its[i]} != {ri}"
         assert np.array_equal(hard[o:o + n], rh), f"block {i}: hard differs"
         assert np.array_equal(f32_bits(post[o:o + n]), f32_bits(rp)), f"block {i}: posterior differs"
+
+
+@pytest.mark.parametrize("z", [352, 320, 288])
+def test_fastx_other_sizes_match_oracle_and_all_lds_kernel(dev, oracle, z):
+    from syndrome_amd import _lib as L
+    dec = _decoder(0.5, z)
+    llr = _channel(dec, [1.5, 0.4, 3.0, 1.0, 1.8], 30 + z, z)  # odd: the last workgroup half empty
+    say(f"[fastx] BG1 Z={z}, B=5")
+    d_in = torch.from_numpy(llr).cuda()
+    post, hard, its, edge = _run(dec, d_in, torch.empty_like(d_in), 8, L.INIT_5G | L.WRITE_POST | L.WRITE_HARD)
+    _check_oracle(oracle, 0.5, llr, post, hard, its, edge, 8, label=f"Z={z}", z=z)
+    _, h2, i2, e2 = _run(dec, d_in, None, 8, L.INIT_5G | L.WRITE_HARD)
+    assert np.array_equal(its, i2) and np.array_equal(hard, h2)
+    assert np.array_equal(f32_bits(edge), f32_bits(e2))
