@@ -679,9 +679,37 @@ def bench_serving(torch, args, peak_ref_gbps):
                                      "transport_blocks": int(len(tbl)), "code_blocks": int(ncb),
                                      "crc_failures": int(fails)})
             out["resident_transport_blocks_inputs_in_hbm"] = res
+        if mode == "resident":
+            # the batched (work-queue) decoder's peak on this same traffic:
+            # its code blocks, copied 16 times into one arena, decoded by one
+            # plan (inputs in HBM), and the server offered 70 % of that
+            R = 16
+            tot = int(offs[-1] + (68 if cb_bg[-1] == 0 else 52) * cb_z[-1])
+            from syndrome_amd import DecodePlan, _lib as L
+            rb = np.tile(cb_bg, R)
+            rplan = DecodePlan(rb, np.tile(cb_z, R), np.tile(cb_nf, R),
+                               (np.tile(offs, R) + np.repeat(np.arange(R, dtype=np.int64) * tot, nb)), args.beta)
+            r_in = d_llr[:tot].repeat(R)
+            r_post = torch.empty_like(r_in)
+            r_hard = torch.empty(r_in.numel(), dtype=torch.uint8, device="cuda")
+            r_it = torch.empty(R * nb, dtype=torch.int32, device="cuda")
+            fl = L.INIT_5G | L.WRITE_POST | L.WRITE_HARD
+            _, rper = time_launches(lambda: rplan.run(r_in, r_post, r_hard, r_it, args.iters, fl), 5, 1, torch)
+            bpk = float(info_bits.sum()) * R / (float(rper.mean()) * 1e-3) / 1e9
+            del r_in, r_post, r_hard, rplan
+            cbl, tbl, bps, waits = serve_loadgen(srv, d_llr, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
+                                                 0.7 * bpk * 1e9 / mean_tb_bits, args.serve_seconds, seed=args.seed)
+            out["batched_peak_same_traffic"] = {"info_gbps": round(bpk, 4), "ms_per_launch": round(float(rper.mean()), 4),
+                                                "code_blocks_per_launch": int(R * nb)}
+            out["resident_inputs_in_hbm_at_70pct_of_batched_peak_same_traffic"] = {
+                "offered_info_gbps": round(0.7 * bpk, 4), "achieved_info_gbps": round(bps / 1e9, 4),
+                "cb_latency_ms_p50": pct(cbl, 50), "cb_latency_ms_p99": pct(cbl, 99),
+                "tb_latency_ms_p99": pct(tbl, 99)}
         if mode == "resident" and peak_ref_gbps is not None:
             # the same traffic offered at 70% of the batched BG1 Z=384
-            # decoder's peak (measured by the headline run)
+            # decoder's peak (measured by the headline run); above this
+            # traffic's own batched peak, its mix of codes being slower to
+            # decode than BG1 Z=384, this load cannot be carried
             cbl, tbl, bps, waits = serve_loadgen(srv, d_llr, offs, cb_bg, cb_z, cb_nf, info_bits, tb_first, tb_num,
                                                  0.7 * peak_ref_gbps * 1e9 / mean_tb_bits, args.serve_seconds, seed=args.seed)
             out["resident_inputs_in_hbm_at_70pct_of_batched_bg1_peak"] = {

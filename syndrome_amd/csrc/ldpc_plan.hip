@@ -132,6 +132,7 @@ constexpr int SPLIT_LANES = 256;
 constexpr int HEAD_STRIDE = 32;
 constexpr unsigned RUN_RING = 16;
 constexpr int LAUNCHES = 3;
+constexpr int X_MIN_ITEMS_PER_CU = 4;
 
 // A code of the two-codeword extension-in-HBM kernel's launch.
 bool x_code(const SynCode &c, bool i8) { return !i8 && c.kind == SYN_KIND_BG1 && syn::persistx_kernel_f32_bg1(c.z); }
@@ -197,6 +198,27 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     auto fits = [&](const SynCode &code, bool pk, int lanes, size_t cap) {
         return group_bytes(code, fit(code, pk, lanes, cap), p->i8, pk) <= cap;
     };
+    // The extension launch runs after the others on the stream.  When its
+    // codes are a small part of the plan (config-4-like mixes: a few items
+    // per size), its tail would leave most CUs idle behind the main launch,
+    // so it is used only where they hold at least half the plan's work
+    // (E*Z per block) or a few items per CU.
+    size_t x_items = 0;
+    double x_work = 0.0, all_work = 0.0;
+    {
+        std::vector<size_t> per_code(nc, 0);
+        for (int cc : p->cb_code) ++per_code[cc];
+        for (size_t c = 0; c < nc; ++c) {
+            const double w = (double)p->codes[c].n_edges * p->codes[c].z * (double)per_code[c];
+            all_work += w;
+            if (x_code(p->codes[c], p->i8)) {
+                x_items += (per_code[c] + 1) / 2;
+                x_work += w;
+            }
+        }
+    }
+    const bool use_x_launch =
+        x_items > 0 && (2.0 * x_work >= all_work || x_items >= (size_t)X_MIN_ITEMS_PER_CU * (size_t)std::max(1, p->cus));
     std::vector<char> packed(nc, 0);
     for (size_t c = 0; c < nc; ++c) {
         const SynCode &code = p->codes[c];
@@ -215,7 +237,7 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         if (group_bytes(code, k, p->i8, pk) > lds_cap) return SYNDROME_INVALID_PARAM;
         cws[c] = k;
         packed[c] = pk;
-        if (x_code(code, p->i8)) {
+        if (use_x_launch && x_code(code, p->i8)) {
             // the extension-in-HBM launch; cws_mx: its main-launch form
             int xl = 0, xc = 1;
             const size_t xb = (size_t)syn::fastx_shape_f32_bg1(code.z, &xl, &xc);
