@@ -48,26 +48,29 @@ __device__ __forceinline__ uint32_t crc24_xpow(long long n, uint32_t poly)
     return result;
 }
 
-// CRC of bits [0, len) of `bit(i)` by the whole workgroup (blockDim.x
-// threads, a power of two); `red` holds blockDim.x words of LDS.  Every
-// thread returns the remainder.  Call from all threads of the workgroup.
+// CRC of bits [0, len) of `bit(i)` by the first `nt` threads of the
+// workgroup (nt a power of two, blockDim.x when 0); `red` holds nt words of
+// LDS.  Every thread returns the remainder.  Call from all threads of the
+// workgroup.
 template <class Bit>
-__device__ uint32_t group_crc24(const Bit &bit, long long len, uint32_t poly, uint32_t *red)
+__device__ uint32_t group_crc24(const Bit &bit, long long len, uint32_t poly, uint32_t *red, int nt_use = 0)
 {
-    const int t = threadIdx.x, nt = blockDim.x;
+    const int t = threadIdx.x, nt = nt_use > 0 ? nt_use : (int)blockDim.x;
+    const bool part = t < nt;
     const long long w = (len + nt - 1) / nt;
     const long long pad = w * nt - len;
     uint32_t reg = 0;
     const long long b0 = t * w - pad;
-    for (long long i = 0; i < w; ++i) {
-        const long long b = b0 + i;
-        reg = crc24_step(reg, b >= 0 ? bit(b) : 0u, poly);
-    }
-    red[t] = reg;
+    if (part)
+        for (long long i = 0; i < w; ++i) {
+            const long long b = b0 + i;
+            reg = crc24_step(reg, b >= 0 ? bit(b) : 0u, poly);
+        }
+    if (part) red[t] = reg;
     uint32_t xl = crc24_xpow(w, poly);
     __syncthreads();
     for (int stride = 1; stride < nt; stride *= 2) {
-        if (t % (2 * stride) == 0) red[t] = crc24_mulmod(red[t], xl, poly) ^ red[t + stride];
+        if (part && t % (2 * stride) == 0) red[t] = crc24_mulmod(red[t], xl, poly) ^ red[t + stride];
         xl = crc24_mulmod(xl, xl, poly);
         __syncthreads();
     }

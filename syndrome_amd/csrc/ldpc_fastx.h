@@ -461,9 +461,21 @@ __host__ __device__ constexpr int fastx_group_lds_bytes()
 // edge_r) and whether the partner slot exists.
 struct XSlot {
     bool live, live_o;
-    long long off, nf;
+    long long nf;
     int id;
+    const float *in;  // the codeword's channel input (N elements)
+    float *xb;        // N elements the decode may write: the extension columns live here
+    float *post;      // the core posterior goes here when not null (xb, when the posterior is wanted)
+    uint8_t *hard;    // hard decisions (N elements) when not null
 };
+
+template <class T>
+__device__ __forceinline__ T *rfl_ptr(T *p)
+{
+    const unsigned long long v = (unsigned long long)p;
+    return (T *)(((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v));
+}
 
 // FASTX_CPW codewords per workgroup, the lane's codeword slot t / lanes(Z),
 // lifting index z = t % lanes(Z); lanes with z >= Z, and in a wider
@@ -505,13 +517,13 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
     volatile int *fail = (volatile int *)(P0 + NC);
     // the partner codeword's fail flags
     volatile int *fail_o = (volatile int *)((float *)(smem + (1 - cwl) * RB + pad) + NC);
-    const long long off = (long long)(((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(sl.off >> 32)) << 32) |
-                                      (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)sl.off));
     const bool init5g = (a.flags & SYN_F_INIT_5G) != 0;
     const long long nf = sl.nf;
     const int kp = nf >= K ? 0 : K - (int)nf;
-    const float *in = (const float *)a.in + off;
-    float *post = (float *)a.post + off;
+    const float *in = rfl_ptr(sl.in);
+    float *xb = rfl_ptr(sl.xb);
+    float *post = rfl_ptr(sl.post);
+    uint8_t *hard = rfl_ptr(sl.hard);
 
     // core columns -> LDS, with the 5G prologue (it touches no extension
     // column: the punctured 2Z and the fillers lie below K)
@@ -573,7 +585,7 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
     const int g0 = max(__float_as_int(0.f - beta), glo);
 
     const __amdgpu_buffer_rsrc_t r_in = __builtin_amdgcn_make_buffer_rsrc((void *)in, 0, N * 4, 0x00020000);
-    const __amdgpu_buffer_rsrc_t r_post = __builtin_amdgcn_make_buffer_rsrc((void *)post, 0, N * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t r_post = __builtin_amdgcn_make_buffer_rsrc((void *)xb, 0, N * 4, 0x00020000);
     // the first PF extension values of iteration 0
     float xq[PF] = {};
     if (lane_ok)
@@ -638,26 +650,26 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
     // outputs: the core posterior and hard decisions from LDS, the
     // extension hard decisions from the bit record (its posterior is in
     // place already)
-    const bool wpost = (a.flags & SYN_F_WRITE_POST) != 0, whard = (a.flags & SYN_F_WRITE_HARD) != 0;
+    const bool wpost = post != nullptr, whard = hard != nullptr;
     if ((a.flags & SYN_F_ALIGNED) != 0) {
         for (int i4 = z; i4 < NC / 4; i4 += ZC) {
             const float4 v = *(const float4 *)(P0 + 4 * i4);
             if (wpost) *(float4 *)(post + 4 * i4) = v;
             if (whard)
-                *(uint32_t *)(a.hard + off + 4 * i4) = (v.x < 0.f ? 1u : 0u) | (v.y < 0.f ? 1u << 8 : 0u) |
+                *(uint32_t *)(hard + 4 * i4) = (v.x < 0.f ? 1u : 0u) | (v.y < 0.f ? 1u << 8 : 0u) |
                                                        (v.z < 0.f ? 1u << 16 : 0u) | (v.w < 0.f ? 1u << 24 : 0u);
         }
     } else {
         for (int i = z; i < NC; i += ZC) {
             const float v = P0[i];
             if (wpost) post[i] = v;
-            if (whard) a.hard[off + i] = v < 0.f ? 1 : 0;
+            if (whard) hard[i] = v < 0.f ? 1 : 0;
         }
     }
     if (whard)
         SFor<0, NX>::run([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            a.hard[off + x_col_of<G>(k) * ZC + z] = (uint8_t)((rec[k / 32] >> x_bit<G>(k)) & 1u);
+            hard[x_col_of<G>(k) * ZC + z] = (uint8_t)((rec[k / 32] >> x_bit<G>(k)) & 1u);
         });
     // the codeword's index, fetched again here rather than held through the
     // decode
@@ -686,6 +698,17 @@ __device__ __forceinline__ void fastx_group(const SynLaunch &a, const XSlot &sl,
 }
 
 
+// A batch launch's buffers for the codeword at element offset `off`: input
+// from `in`, the posterior buffer (required) holds the extension columns
+// and takes the whole posterior, hard decisions when asked for.
+__device__ __forceinline__ void x_slot_io(const SynLaunch &a, XSlot &sl, long long off)
+{
+    sl.in = (const float *)a.in + off;
+    sl.xb = (float *)a.post + off;
+    sl.post = sl.xb;
+    sl.hard = (a.flags & SYN_F_WRITE_HARD) ? a.hard + off : nullptr;
+}
+
 // The per-config kernel: codewords blockIdx.x * FASTX_CPW + t / Z of a
 // contiguous batch.
 template <int KIND, int ZC>
@@ -699,8 +722,8 @@ fastx_f32(const SynCode *__restrict__ code, SynLaunch a)
     sl.live = cw < a.batch;
     sl.live_o = blockIdx.x * FASTX_CPW + (1 - cwl) < a.batch;
     sl.id = sl.live ? cw : 0;
-    sl.off = (long long)sl.id * N;
     sl.nf = a.n_filler;
+    x_slot_io(a, sl, (long long)sl.id * N);
     fastx_group<KIND, ZC>(a, sl, NoHook{}, [&]() { return cw; });
 }
 
@@ -741,8 +764,8 @@ persistx_f32(const SynCode *__restrict__ codes, const SynItem *__restrict__ item
             sl.live_o = (1 - cwl) < it.count;
             const bool mine = slot < FASTX_CPW && sl.live;
             sl.id = mine ? ids[it.first + cwl] : 0;
-            sl.off = mine ? offs[sl.id] : 0;
             sl.nf = mine ? (nfill ? nfill[sl.id] : a.n_filler) : 0;
+            x_slot_io(a, sl, mine ? offs[sl.id] : 0);
             fastx_group<KIND, ZC>(
                 a, sl,
                 [&]() {

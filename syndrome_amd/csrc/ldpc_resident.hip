@@ -10,7 +10,9 @@
 // read a device-memory copy of the tail; one of them at a time (a lock)
 // refreshes the copy from host memory.  Each item is
 // decoded with the register-resident body for its base graph (the
-// compile-time-Z instances for BG1 Z=384 and BG2 Z=208), its inputs read
+// compile-time-Z instances for BG1 Z=384 and BG2 Z=208; BG1 Z=384 items of
+// two blocks with the two-codeword kernel of ldpc_fastx.h, its extension
+// columns in a scratch area of the workgroup's own), its inputs read
 // straight from each block's source pointer (HBM or pinned host memory),
 // its hard decisions and iteration counts written straight into pinned host
 // memory; then every wave waits for its own stores to complete, the
@@ -24,7 +26,7 @@
 // host sets ctl->stop: every wait ends, and the host relaunches the grid
 // when work is published after it has left.
 #include "ldpc_crc.h"
-#include "ldpc_fast.h"
+#include "ldpc_fastx.h"
 #include "resident_protocol.h"
 
 namespace syn {
@@ -130,9 +132,13 @@ __device__ __forceinline__ T host_load(const T *p)
 // completes the count decides the transport block and publishes ok[] and
 // done[] (every wave fences the block's payload writes at system scope
 // before its count increment).
-__device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, int slot, const float *post)
+// The workgroup's CRC reductions run on its first 512 lanes, in a spare
+// part of the dynamic LDS (`red`, 512 words: see resident_decode).
+constexpr int FOLD_LANES = 512;
+
+__device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, int slot, const float *post,
+                                     uint32_t *red)
 {
-    __shared__ uint32_t red[512];
     __shared__ unsigned s_id;
     __shared__ int s_c, s_p, s_a;
     __shared__ unsigned char *s_out;
@@ -149,10 +155,10 @@ __device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, i
     const int c = s_c, p = s_p, a = s_a;
     auto bit = [&](long long i) -> uint32_t { return post[i] < 0.f ? 1u : 0u; };
     bool cb_ok = true;
-    if (c > 1) cb_ok = group_crc24(bit, p, CRC24B_POLY, red) == received24(bit, p);
+    if (c > 1) cb_ok = group_crc24(bit, p, CRC24B_POLY, red, FOLD_LANES) == received24(bit, p);
     const long long s0 = (long long)ci * p;
     const long long len = a - s0 < 0 ? 0 : (a - s0 > p ? p : a - s0);
-    const uint32_t part = len > 0 ? group_crc24(bit, len, CRC24A_POLY, red) : 0u;
+    const uint32_t part = len > 0 ? group_crc24(bit, len, CRC24A_POLY, red, FOLD_LANES) : 0u;
     for (long long u = threadIdx.x; u < len; u += blockDim.x) s_out[s0 + u] = (unsigned char)bit(u);
     release_all_waves();  // this block's payload bits are visible before it is counted
     __syncthreads();
@@ -169,12 +175,21 @@ __device__ __noinline__ void tb_fold(const SynResidentTb &tbs, int ti, int ci, i
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict__ codes,
-                                                        const SynWork *__restrict__ ring, unsigned cap,
-                                                        SynResidentCtl *ctl, unsigned *done, ResidentQueue *q,
-                                                        long long *last_claim, const long long *__restrict__ offs,
-                                                        SynLaunch a, long long idle_ticks, long long life_ticks,
-                                                        SynResidentTb tbs)
+// Lanes of the resident workgroup: two BG1 Z=384 codewords (12 waves, three
+// per SIMD, so 168 VGPRs); the other codes' items use the first 512.
+constexpr int RESIDENT_LANES = FASTX_CPW * 384;
+// Where `red` sits in the dynamic LDS: the second codeword region's first
+// spill slots, free once a two-codeword item's decode is over, and past the
+// end of every one-codeword-kind item's layout (at most 97 KB).
+constexpr int RED_OFF = fastx_region_bytes<SYN_KIND_BG1, 384>() + fast_pad(384, 4) +
+                        spill_offset<384, kX<Graph<SYN_KIND_BG1>>.core>(0, 0);
+static_assert(RED_OFF + 4 * FOLD_LANES <= fastx_group_lds_bytes<SYN_KIND_BG1, 384>(), "red in the LDS");
+
+__global__ void __launch_bounds__(RESIDENT_LANES) __attribute__((amdgpu_waves_per_eu(3, 3)))
+resident_decode(const SynCode *__restrict__ codes, const SynWork *__restrict__ ring, unsigned cap,
+                SynResidentCtl *ctl, unsigned *done, ResidentQueue *q, long long *last_claim,
+                const long long *__restrict__ offs, SynLaunch a, long long idle_ticks, long long life_ticks,
+                SynResidentTb tbs, float *xscratch)
 {
     __shared__ unsigned s_w;
     __shared__ int s_have, s_code, s_count, s_aligned, s_any_tb;
@@ -226,9 +241,26 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
         if (s_any_tb)
             for (int j = 0; j < s_count; ++j)
                 if (s_tb[j] >= 0) io.no_hard |= 1u << j;
-        if (code->kind == SYN_KIND_BG1) {
-            if (code->z == 384) fast_group<SYN_KIND_BG1, 384, false, true, false>(code, 384, s_count, io, la);
-            else fast_group<SYN_KIND_BG1, 0, false, true>(code, code->z, s_count, io, la);
+        extern __shared__ __align__(16) unsigned char smem[];
+        const bool pair = code->kind == SYN_KIND_BG1 && code->z == 384;
+        if (pair) {
+            // up to two blocks: the two-codeword kernel, its extension
+            // columns in this workgroup's scratch (two codewords' worth)
+            constexpr int N = 68 * 384;
+            const int cwl = (int)threadIdx.x >= 384 ? 1 : 0;
+            XSlot sl;
+            sl.live = cwl < s_count;
+            sl.live_o = (1 - cwl) < s_count;
+            const int j = sl.live ? cwl : 0;
+            sl.id = s_slots[j];
+            sl.nf = s_nf[j];
+            sl.in = (const float *)s_src[j];
+            sl.xb = xscratch + ((size_t)blockIdx.x * FASTX_CPW + cwl) * N;
+            sl.post = nullptr;
+            sl.hard = ((io.no_hard >> j) & 1u) ? nullptr : la.hard + offs[s_slots[j]];
+            fastx_group<SYN_KIND_BG1, 384>(la, sl, NoHook{}, [&]() { return s_slots[cwl]; });
+        } else if (code->kind == SYN_KIND_BG1) {
+            fast_group<SYN_KIND_BG1, 0, false, true>(code, code->z, s_count, io, la);
         } else {
             if (code->z == 208) fast_group<SYN_KIND_BG2, 208, false, true, false>(code, 208, s_count, io, la);
             else fast_group<SYN_KIND_BG2, 0, false, true>(code, code->z, s_count, io, la);
@@ -237,9 +269,15 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
             __syncthreads();  // the decode is finished with the posterior in LDS
             const int pad = fast_pad(code->z, 4);
             const int n = code->n_b * code->z;
-            extern __shared__ __align__(16) unsigned char smem[];
+            uint32_t *red = (uint32_t *)(smem + RED_OFF);
             for (int j = 0; j < s_count; ++j)
-                if (s_tb[j] >= 0) tb_fold(tbs, s_tb[j], s_tbci[j], s_slots[j], (const float *)(smem + pad) + (size_t)j * n);
+                if (s_tb[j] >= 0) {
+                    // a two-codeword item's posteriors: the core columns of
+                    // region j (the payload and its CRC lie below 22Z)
+                    const float *post = pair ? (const float *)(smem + j * fastx_region_bytes<SYN_KIND_BG1, 384>() + pad)
+                                             : (const float *)(smem + pad) + (size_t)j * n;
+                    tb_fold(tbs, s_tb[j], s_tbci[j], s_slots[j], post, red);
+                }
         }
         // Every wave releases its own hard decisions and counts (system
         // scope) before the barrier; lane 0 then publishes the item's
@@ -251,5 +289,9 @@ __global__ void __launch_bounds__(512) resident_decode(const SynCode *__restrict
 }
 
 const void *resident_kernel() { return (const void *)resident_decode; }
+int resident_lanes() { return RESIDENT_LANES; }
+// LDS of a two-codeword item, and the extension scratch per workgroup (floats)
+int resident_pair_lds() { return fastx_group_lds_bytes<SYN_KIND_BG1, 384>(); }
+size_t resident_scratch_floats() { return (size_t)FASTX_CPW * 68 * 384; }
 
 }  // namespace syn

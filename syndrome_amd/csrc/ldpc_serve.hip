@@ -42,6 +42,9 @@
 namespace syn {
 const void *persist_kernel_f32_big();
 const void *resident_kernel();
+int resident_lanes();
+int resident_pair_lds();
+size_t resident_scratch_floats();
 int allow_max_dynamic_lds(const void *fn);
 }  // namespace syn
 
@@ -161,6 +164,7 @@ struct SyndromeServer : syn::ServeCore<HipResident> {
         unsigned *done_dev = nullptr;
         unsigned *d_head = nullptr;
         long long *d_last = nullptr, *d_offs = nullptr;
+        float *d_xscratch = nullptr;  // extension columns of two-codeword items, per workgroup
         size_t lds = 0;
         long long idle_ticks = 0, life_ticks = 0;
         SynTbWork *tbw_dev = nullptr;
@@ -225,9 +229,11 @@ int32_t launch_resident(SyndromeServer &s)
     const long long *offs = s.rd.d_offs;
     long long idle = s.rd.idle_ticks, life = s.rd.life_ticks;
     SynResidentTb tbs{s.rd.tbw_dev, s.rd.d_tbstate, s.rd.tb_done_dev, s.rd.tb_ok_dev, s.h_cbok_dev};
-    void *args[] = {&cp, &ring, &cap, &ctl, &dn, &queue, &last, &offs, &a, &idle, &life, &tbs};
+    float *xs = s.rd.d_xscratch;
+    void *args[] = {&cp, &ring, &cap, &ctl, &dn, &queue, &last, &offs, &a, &idle, &life, &tbs, &xs};
     DeviceScope ds(s.device);
-    if (hipLaunchKernel(syn::resident_kernel(), dim3(s.rs.grid), dim3(512), args, s.rd.lds, s.be.stream) != hipSuccess ||
+    if (hipLaunchKernel(syn::resident_kernel(), dim3(s.rs.grid), dim3(syn::resident_lanes()), args, s.rd.lds,
+                        s.be.stream) != hipSuccess ||
         hipEventRecord(s.be.ev, s.be.stream) != hipSuccess)
         return SYNDROME_DEVICE_ERROR;
     return SYNDROME_OK;
@@ -389,15 +395,26 @@ int32_t SyndromeServer::init_resident()
     rs.cws.resize(codes.size());
     rd.lds = 0;
     for (size_t c = 0; c < codes.size(); ++c) {
+        if (codes[c].kind == SYN_KIND_BG1 && codes[c].z == 384) {
+            // two blocks per item, the two-codeword kernel (ldpc_resident.hip)
+            rs.cws[c] = 2;
+            rd.lds = std::max(rd.lds, (size_t)syn::resident_pair_lds());
+            continue;
+        }
         int k = std::min(SYN_WORK_MAX, std::max(1, 512 / codes[c].z));
         while (k > 1 && group_bytes(codes[c], k) > room) --k;
         if (group_bytes(codes[c], k) > room) return SYNDROME_INVALID_PARAM;
         rs.cws[c] = k;
         rd.lds = std::max(rd.lds, group_bytes(codes[c], k));
     }
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 512, rd.lds) != hipSuccess || per_cu < 1)
+    if (rd.lds > room) return SYNDROME_INVALID_PARAM;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, syn::resident_lanes(), rd.lds) != hipSuccess ||
+        per_cu < 1)
         return SYNDROME_DEVICE_ERROR;
     rs.grid = per_cu * cus;
+    if (hipMalloc((void **)&rd.d_xscratch, sizeof(float) * syn::resident_scratch_floats() * (size_t)rs.grid) !=
+        hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
     rd.idle_ticks = (long long)(RESIDENT_IDLE_US * khz / 1000.0);
     rd.life_ticks = (long long)(RESIDENT_LIFE_S * khz * 1000.0);
     rs.cap = RING_ITEMS;
@@ -542,7 +559,7 @@ void syndrome_server_destroy(SyndromeServer *s)
         if (s->be.stream) (void)hipStreamDestroy(s->be.stream);
         for (void *p : {(void *)r.ring, (void *)r.ctl, (void *)r.done, (void *)r.tbw, (void *)r.tb_done, (void *)r.tb_ok})
             if (p) (void)hipHostFree(p);
-        for (void *p : {(void *)d.d_head, (void *)d.d_last, (void *)d.d_offs, (void *)d.d_tbstate})
+        for (void *p : {(void *)d.d_head, (void *)d.d_last, (void *)d.d_offs, (void *)d.d_tbstate, (void *)d.d_xscratch})
             if (p) (void)hipFree(p);
         for (void *p : {(void *)s->d_llr, (void *)s->d_codes})
             if (p) (void)hipFree(p);
