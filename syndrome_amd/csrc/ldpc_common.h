@@ -92,6 +92,7 @@ struct WideShape {
 #define SYN_F_PUBLIC 31u        // the flags a caller may pass; the rest are set by the launchers
 #define SYN_F_ALIGNED 32u       // element offsets and N are multiples of 4, in/post/hard 16-byte aligned
 #define SYN_F_TRACE 64u         // f32 path: posterior after every pass to `trace`
+#define SYN_F_XPAIR 128u        // resident server: BG1 Z=384 items run the two-codeword kernel
 
 // Per-launch arguments shared by every decode kernel.
 struct SynLaunch {

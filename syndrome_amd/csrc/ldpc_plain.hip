@@ -266,9 +266,14 @@ int launch_decode(int kind, bool i8, bool fast, const SynCode *d_code, int z, co
     // columns in that posterior buffer (ldpc_fastx.h), two per CU
     if (fast && !i8 && kind == SYN_KIND_BG1 && a.cws == 1 && (a.flags & SYN_F_WRITE_POST) && a.post &&
         a.iterations >= 1 && !(a.flags & SYN_F_TRACE)) {
-        if (const void *fx = fastx_kernel_f32_bg1(z)) {
-            int lanes = 0, per = 1;
-            lds = (size_t)fastx_shape_f32_bg1(z, &lanes, &per);
+        int lanes = 0, per = 1, dev = 0, lim = 0;
+        const void *fx = fastx_kernel_f32_bg1(z);
+        const size_t xl = fx ? (size_t)fastx_shape_f32_bg1(z, &lanes, &per) : 0;
+        // (a device with less LDS per workgroup keeps the all-LDS kernel)
+        if (fx && hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+            xl <= (size_t)lim) {
+            lds = xl;
             fn = fx;
             block = lanes;
             grid = (a.batch + per - 1) / per;

@@ -237,10 +237,10 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         if (group_bytes(code, k, p->i8, pk) > lds_cap) return SYNDROME_INVALID_PARAM;
         cws[c] = k;
         packed[c] = pk;
-        if (use_x_launch && x_code(code, p->i8)) {
+        int xl = 0, xc = 1;
+        const size_t xb = x_code(code, p->i8) ? (size_t)syn::fastx_shape_f32_bg1(code.z, &xl, &xc) : 0;
+        if (use_x_launch && x_code(code, p->i8) && xb + 64 <= (size_t)p->lds_limit) {
             // the extension-in-HBM launch; cws_mx: its main-launch form
-            int xl = 0, xc = 1;
-            const size_t xb = (size_t)syn::fastx_shape_f32_bg1(code.z, &xl, &xc);
             part[c] = 2;
             cws_mx[c] = k;
             cws[c] = xc;

@@ -242,7 +242,7 @@ resident_decode(const SynCode *__restrict__ codes, const SynWork *__restrict__ r
             for (int j = 0; j < s_count; ++j)
                 if (s_tb[j] >= 0) io.no_hard |= 1u << j;
         extern __shared__ __align__(16) unsigned char smem[];
-        const bool pair = code->kind == SYN_KIND_BG1 && code->z == 384;
+        const bool pair = (la.flags & SYN_F_XPAIR) && code->kind == SYN_KIND_BG1 && code->z == 384;
         if (pair) {
             // up to two blocks: the two-codeword kernel, its extension
             // columns in this workgroup's scratch (two codewords' worth)

@@ -165,6 +165,7 @@ struct SyndromeServer : syn::ServeCore<HipResident> {
         unsigned *d_head = nullptr;
         long long *d_last = nullptr, *d_offs = nullptr;
         float *d_xscratch = nullptr;  // extension columns of two-codeword items, per workgroup
+        bool xpair = false;           // BG1 Z=384 items hold two blocks (the two-codeword kernel)
         size_t lds = 0;
         long long idle_ticks = 0, life_ticks = 0;
         SynTbWork *tbw_dev = nullptr;
@@ -217,7 +218,7 @@ int32_t launch_resident(SyndromeServer &s)
     a.hard = s.h_hard_dev;
     a.iters = s.h_it_dev;
     a.iterations = s.iterations;
-    a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD;  // SYN_F_ALIGNED per item
+    a.flags = SYN_F_INIT_5G | SYN_F_WRITE_HARD | (s.rd.xpair ? SYN_F_XPAIR : 0u);  // SYN_F_ALIGNED per item
     a.beta = s.beta;
     const SynCode *cp = s.d_codes;
     const SynWork *ring = s.rd.ring_dev;
@@ -395,10 +396,11 @@ int32_t SyndromeServer::init_resident()
     rs.cws.resize(codes.size());
     rd.lds = 0;
     for (size_t c = 0; c < codes.size(); ++c) {
-        if (codes[c].kind == SYN_KIND_BG1 && codes[c].z == 384) {
+        if (codes[c].kind == SYN_KIND_BG1 && codes[c].z == 384 && (size_t)syn::resident_pair_lds() <= room) {
             // two blocks per item, the two-codeword kernel (ldpc_resident.hip)
             rs.cws[c] = 2;
             rd.lds = std::max(rd.lds, (size_t)syn::resident_pair_lds());
+            rd.xpair = true;
             continue;
         }
         int k = std::min(SYN_WORK_MAX, std::max(1, 512 / codes[c].z));
