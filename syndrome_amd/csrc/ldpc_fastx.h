@@ -354,10 +354,7 @@ __device__ __forceinline__ bool row_parity_x(const Lane &ln, const Geo<G, ZC, WM
 // of about FASTX_CHECK_LOADS edges: a scheduling barrier after each block
 // keeps the compiler from hoisting every load of the segment (up to 66)
 // ahead of the first compare, which the 168-VGPR budget cannot hold.
-#ifndef FASTX_CLD
-#define FASTX_CLD 20
-#endif
-constexpr int FASTX_CHECK_LOADS = FASTX_CLD;
+constexpr int FASTX_CHECK_LOADS = 20;
 
 template <class G>
 consteval bool check_block_end(int l)
@@ -420,22 +417,20 @@ __host__ __device__ constexpr int fastx_lds_bytes()
 }
 
 // Extension loads in flight per lane (in extension layers).
-#ifndef FASTX_PFD
-#define FASTX_PFD 3
-#endif
-constexpr int FASTX_PF = FASTX_PFD;
+// (2 and 6 measured the same or slower on BG1 Z=384; 42 extension layers
+// must be a multiple of it)
+constexpr int FASTX_PF = 3;
 // Layers of degree at least this run pass 1 with the argmin (fewer live
 // registers); lighter ones keep q for pass 2.
-#ifndef FASTX_KQB
-#define FASTX_KQB 12
-#endif
-constexpr int FASTX_KQ_BELOW = FASTX_KQB;
+// (keeping q in every layer spilled 80 VGPRs and ran 8 % slower on BG1
+// Z=384; tracking the argmin in every layer ran 2 % slower)
+constexpr int FASTX_KQ_BELOW = 12;
 
 // Codewords per workgroup.  Two workgroups of 6 waves would fit a CU by
-// registers and LDS, but the dispatcher does not place the second one (the
-// waves of a workgroup go to the SIMDs round-robin, and the busy SIMDs of
-// the first have room for one more wave, not two); one workgroup of two
-// codewords has 12 waves, three on every SIMD.
+// registers and LDS (the occupancy query says 2), but measured, the second
+// one is not placed beside the first (SQ wave-cycles per second stayed at
+// one workgroup's, and the launch ran 3.8 ms); one workgroup of two
+// codewords has 12 waves, which can only be spread three on every SIMD.
 constexpr int FASTX_CPW = 2;
 
 // Lanes of one codeword's slot: Z rounded up to whole waves (a wave holds
