@@ -1,6 +1,6 @@
 // Register-resident f32 decode with the degree-1 (extension) columns'
-// posteriors in HBM instead of LDS, for compile-time (base graph, Z) with
-// one codeword per workgroup (BG1 Z=384).
+// posteriors in HBM instead of LDS, two codewords per workgroup, for a
+// compile-time (base graph, Z): BG1 Z = 384, 352, 320, 288.
 //
 // Why: a whole BG1 Z=384 posterior is 104,448 B of LDS, so a CU holds one
 // such codeword: 6 waves on 4 SIMDs, two SIMDs carrying two waves and two
@@ -10,8 +10,8 @@
 // read and written only by lane z, in its row's layer.  Its posterior is
 // private to that lane and can live in HBM (in practice L2) with no
 // synchronisation, which leaves the 26 core columns (39,936 B) in LDS.
-// Two workgroups then share a CU, and at <= 168 VGPRs per lane their 12
-// waves put three on every SIMD.
+// Two codewords then fit one workgroup's LDS, and at <= 168 VGPRs per lane
+// its 12 waves put three on every SIMD.
 //
 // Per lane, the extension value of a layer is loaded a few extension layers
 // ahead (global loads stay in flight across the layer barriers, which wait
@@ -21,6 +21,10 @@
 // The arithmetic and its order are layer_f32's, so the decode is
 // bit-identical to the all-LDS kernel and to the reference
 // (src/qc_ldpc.rs:1072-1252).
+//
+// Users: the per-config batch kernel (fastx_f32), the work queue's launch
+// for BG1 Z=384 (persistx_f32, ldpc_plan.hip) and the resident serving
+// kernel's BG1 Z=384 items (ldpc_resident.hip).
 #pragma once
 #include "ldpc_fast.h"
 
@@ -476,9 +480,10 @@ __device__ __forceinline__ T *rfl_ptr(T *p)
 // FASTX_CPW codewords per workgroup, the lane's codeword slot t / lanes(Z),
 // lifting index z = t % lanes(Z); lanes with z >= Z, and in a wider
 // work-queue workgroup whole waves past the slots, take part in the
-// barriers and the loop's exit only.  The launch needs SYN_F_WRITE_POST (the posterior buffer
-// holds the extension columns during the decode) and at least one
-// iteration.  The codewords share the layer barriers; a converged one stops
+// barriers and the loop's exit only.  Each codeword needs an N-element
+// buffer it may write (XSlot::xb: the posterior buffer of a batch launch)
+// to hold its extension columns during the decode, and the launch at least
+// one iteration.  The codewords share the layer barriers; a converged one stops
 // (its waves skip the layers) while its partner goes on, as in
 // qc_ldpc.rs:1211 each stops at its own iteration.  `done` runs once the
 // iterations are over, before the outputs are stored (a work-queue
