@@ -84,6 +84,12 @@ struct SyndromeLdpcPlan {
     // plan is destroyed, so a HIP graph captured before the change still
     // replays valid pointers (with the launch shape it was captured with)
     std::vector<void *> retired;
+    // Streams and events of an eager run's fork: the launches after the
+    // first go on side streams that wait for the caller's stream and that
+    // it waits for in turn, so their workgroups fill CUs that another
+    // launch's drained queue leaves idle.
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     std::mutex mu;
 };
 
@@ -132,6 +138,7 @@ constexpr int SPLIT_LANES = 256;
 constexpr int HEAD_STRIDE = 32;
 constexpr unsigned RUN_RING = 16;
 constexpr int LAUNCHES = 3;
+static_assert(sizeof(SyndromeLdpcPlan::side) / sizeof(hipStream_t) == LAUNCHES - 1, "one side stream per later launch");
 constexpr int X_MIN_ITEMS_PER_CU = 4;
 
 // A code of the two-codeword extension-in-HBM kernel's launch.
@@ -396,17 +403,44 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
         L{p->fn_x, p->d_items_x, use_x ? p->n_items_x : 0, p->grid_x, p->block_x, p->lds_x}};
     // the main launch, the small codes', the extension launch (each with
     // its own counter)
-    for (int q = 0; q < LAUNCHES; ++q) {
+    const auto launch = [&](int q, hipStream_t s) -> int32_t {
         const SynItem *items = ls[q].items;
         int n_items = ls[q].n;
-        if (n_items == 0) continue;
         int *head = heads + q * HEAD_STRIDE;
-        if (syn::launch_reset(head, stream)) return SYNDROME_DEVICE_ERROR;
+        if (syn::launch_reset(head, s)) return SYNDROME_DEVICE_ERROR;
         void *args[] = {&codes, &items, &n_items, &head, &ids, &offs, &nf, &a};
-        if (hipLaunchKernel(ls[q].fn, dim3(ls[q].grid), dim3(ls[q].block), args, ls[q].lds, stream) != hipSuccess)
+        if (hipLaunchKernel(ls[q].fn, dim3(ls[q].grid), dim3(ls[q].block), args, ls[q].lds, s) != hipSuccess)
             return SYNDROME_DEVICE_ERROR;
+        return SYNDROME_OK;
+    };
+    int act[LAUNCHES], n_act = 0;
+    for (int q = 0; q < LAUNCHES; ++q)
+        if (ls[q].n) act[n_act++] = q;
+    // Each launch is a persistent grid whose workgroups leave once its queue
+    // is drained, and none waits for another, so launches may run in any
+    // overlap.  An eager run forks them: the first on the caller's stream,
+    // the others on the plan's side streams.  A run under stream capture
+    // keeps them in sequence on the caller's stream: a side stream joined
+    // into one caller's capture would capture another caller's eager run.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const bool fork = n_act > 1 && p->ev_fork && hipStreamIsCapturing(stream, &cs) == hipSuccess &&
+                      cs == hipStreamCaptureStatusNone;
+    if (!fork) {
+        for (int i = 0; i < n_act; ++i)
+            if (int32_t rc = launch(act[i], stream)) return rc;
+        return SYNDROME_OK;
     }
-    return SYNDROME_OK;
+    if (hipEventRecord(p->ev_fork, stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    for (int i = 1; i < n_act; ++i) {
+        hipStream_t s = p->side[i - 1];
+        if (hipStreamWaitEvent(s, p->ev_fork, 0) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        if (int32_t rc = launch(act[i], s)) return rc;
+        if (hipEventRecord(p->ev_join[i - 1], s) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+    }
+    int32_t rc = launch(act[0], stream);
+    for (int i = 1; i < n_act; ++i)
+        if (hipStreamWaitEvent(stream, p->ev_join[i - 1], 0) != hipSuccess) rc = SYNDROME_DEVICE_ERROR;
+    return rc;
 }
 
 }  // namespace
@@ -456,6 +490,11 @@ int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int
         if (int32_t rc = upload(p.get(), &p->d_nfill, p->nfill)) return rc;
         if (hipMalloc((void **)&p->d_head, RUN_RING * LAUNCHES * HEAD_STRIDE * sizeof(int)) != hipSuccess)
             return SYNDROME_DEVICE_ERROR;
+        for (int i = 0; i < LAUNCHES - 1; ++i)
+            if (hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&p->ev_join[i], hipEventDisableTiming) != hipSuccess)
+                return SYNDROME_DEVICE_ERROR;
+        if (hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess) return SYNDROME_DEVICE_ERROR;
         const int min_block = ((max_z + 63) / 64) * 64;
         const int b = block > 0 ? block : std::max(min_block, 512);
         if (int32_t rc = build_items(p.get(), std::max(b, min_block))) return rc;
@@ -475,6 +514,11 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *p)
                         (void *)p->d_tbs})
             if (q) (void)hipFree(q);
         for (void *q : p->retired) (void)hipFree(q);
+        for (int i = 0; i < 2; ++i) {
+            if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
+            if (p->ev_join[i]) (void)hipEventDestroy(p->ev_join[i]);
+        }
+        if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
     } catch (...) {
     }
     delete p;
