@@ -262,11 +262,17 @@ def valu_roofline(kernel, dtype, edge_slot_iters, kernel_ms, compulsory_bytes, p
 
 
 def _launch_key(r):
-    """(grid work-items, workgroup size) of a rocprofv3 CSV row: two launches
-    of one kernel can share a grid size with different workgroups."""
+    """(grid work-items, workgroup size, kernel) of a rocprofv3 CSV row: two
+    launches of one kernel can share a grid size with different workgroups,
+    and two kernels a launch shape."""
     g = r["Grid_Size_X"] if "Grid_Size_X" in r else r.get("Grid_Size", 0)
     w = r["Workgroup_Size_X"] if "Workgroup_Size_X" in r else r.get("Workgroup_Size", 0)
-    return int(g), int(w)
+    return int(g), int(w), r["Kernel_Name"].split("(")[0].replace("void ", "")
+
+
+def _kernel_match(kernel_substr, name):
+    """A kernel name matches any of the '|'-separated substrings."""
+    return any(k in name for k in kernel_substr.split("|"))
 
 
 def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_substr, cmd_trace, cmd_pmc):
@@ -288,7 +294,7 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
     tr = rows(trace_dir, "run_kernel_trace.csv")
     by_grid = defaultdict(list)
     for r in tr:
-        if kernel_substr in r["Kernel_Name"]:
+        if _kernel_match(kernel_substr, r["Kernel_Name"]):
             by_grid[_launch_key(r)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
     stats = rows(trace_dir, "run_kernel_stats.csv")
     counters = {}
@@ -296,7 +302,7 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
         ker, cpy = defaultdict(list), []
         for r in rows(d, "run_counter_collection.csv"):
             v = float(r["Counter_Value"])
-            if kernel_substr in r["Kernel_Name"]:
+            if _kernel_match(kernel_substr, r["Kernel_Name"]):
                 ker[_launch_key(r)].append(v)
             elif "copyBuffer" in r["Kernel_Name"]:
                 cpy.append((int(r["Grid_Size"]), v))
@@ -311,16 +317,16 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
     for r in stats[:6]:
         lines.append(f"| {r['Name'][:70]} | {r['Calls']} | {float(r['AverageNs']):.0f} | {r['MinNs']} | {r['MaxNs']} |")
     lines += ["", "## profiled kernel by launch shape (trace CSV)", "",
-              "| grid (work-items) | workgroup | launches | average ms |", "|---|---|---|---|"]
+              "| kernel | grid (work-items) | workgroup | launches | average ms |", "|---|---|---|---|---|"]
     for g, v in sorted(by_grid.items()):
-        lines.append(f"| {g[0]} | {g[1]} | {len(v)} | {np.mean(v):.4f} |")
+        lines.append(f"| {g[2]} | {g[0]} | {g[1]} | {len(v)} | {np.mean(v):.4f} |")
     top = max(by_grid)
     out = {"kernel": kernel_substr, "grid_work_items": top[0], "workgroup": top[1],
            "average_ms_full_batch": float(np.mean(by_grid[top])),
            "average_ms_per_step_all_grids": float(sum(np.mean(v) for v in by_grid.values()))}
     if len(by_grid) > 1:
-        lines += ["", f"One step launches every shape above once: {out['average_ms_per_step_all_grids']:.4f} ms per "
-                      f"step in all."]
+        lines += ["", f"One step launches every shape above once: {out['average_ms_per_step_all_grids']:.4f} ms of "
+                      f"kernel time per step in all (launches that run concurrently overlap within it)."]
     if calib:
         fk, fc = counters["FETCH_SIZE"]
         wk, wc = counters["WRITE_SIZE"]
@@ -337,11 +343,11 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, out_prefix, kernel_subst
         lines += ["", "## HBM traffic (--pmc)", "",
                   f"The calibration copy ({calib} B) reads FETCH_SIZE {fc:.0f} KiB raw -> scale {f_scale:.4f}, "
                   f"and WRITE_SIZE {wc:.0f} KiB raw -> scale {w_scale:.4f}.", "",
-                  "| grid (work-items) | workgroup | FETCH_SIZE KiB raw | WRITE_SIZE KiB raw | HBM MB (scaled) |",
-                  "|---|---|---|---|---|"]
+                  "| kernel | grid (work-items) | workgroup | FETCH_SIZE KiB raw | WRITE_SIZE KiB raw | HBM MB (scaled) |",
+                  "|---|---|---|---|---|---|"]
         for g in sorted(fk):
             mb = (fk[g] * f_scale + wk.get(g, 0.0) * w_scale) * 1024.0 / 1e6
-            lines.append(f"| {g[0]} | {g[1]} | {fk[g]:.0f} | {wk.get(g, 0.0):.0f} | {mb:.1f} |")
+            lines.append(f"| {g[2]} | {g[0]} | {g[1]} | {fk[g]:.0f} | {wk.get(g, 0.0):.0f} | {mb:.1f} |")
         lines += ["", f"=> {out['hbm_bytes_per_launch'] / 1e6:.1f} MB of HBM traffic per step."]
     os.makedirs(os.path.dirname(out_prefix), exist_ok=True)
     open(out_prefix + ".md", "w").write("\n".join(lines) + "\n")
@@ -529,7 +535,7 @@ def bench_uneven(torch, args):
     t8 = _timed(lambda: plan8.run(d_llr, d_post8, d_hard, d_it, args.iters, f8, quant=q), 5, torch)
     bits = float(k.sum())
     edges = np.where(bg == 0, 316, 197) * z
-    roof = valu_roofline("syn::persist_decode<false, *> (both launches of the plan)", "f32",
+    roof = valu_roofline("syn::persist_decode<false, *> and syn::persistx_f32<0, 384> (every launch of the plan)", "f32",
                          float((edges * its).sum()), t_dec * 1e3, float(n.sum()) * 9, "persist_f32_uneven_b32768")
     cpu = cpu8 = None
     if not args.no_cpu:
@@ -831,7 +837,7 @@ def summarize_sq(sq_dirs, out_md, kernel_substr, cmds, edge_slot_iters, clock_gh
     durations = defaultdict(dict)  # launch shape -> dispatch -> ms, from the counter runs' own timestamps
     for d in sq_dirs:
         rows = list(csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
-        ker = [r for r in rows if kernel_substr in r["Kernel_Name"]]
+        ker = [r for r in rows if _kernel_match(kernel_substr, r["Kernel_Name"])]
         big = max(_launch_key(r) for r in ker)
         for r in ker:
             key = _launch_key(r)

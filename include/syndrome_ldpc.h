@@ -246,8 +246,14 @@ int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int
                                   const int64_t *offsets, size_t count, float offset_beta, int32_t int8,
                                   int32_t block, SyndromeLdpcPlan **out_plan);
 void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *plan);
-/* The plan's persistent launch (two when its largest Z exceeds 256: the
- * codes of Z <= 256 get a second launch of 256-lane workgroups).  int8
+/* The plan's persistent launches: one per group of its codes (codes of
+ * Z > 256; codes of Z <= 256 in 256-lane workgroups; f32 BG2 codes of
+ * Z <= 256; f32 BG1 Z=384 blocks two per workgroup when the posterior is
+ * written back).  An eager call runs them on two streams at once: `stream`
+ * and the plan's own stream, which waits for `stream` and which `stream`
+ * then waits for, so the call is ordered on `stream` as one launch would
+ * be.  A call under stream capture issues them in
+ * sequence on `stream`.  int8
  * plans take i16 (or f32 with SYNDROME_BATCH_QUANTIZE) inputs and write i16
  * posteriors.  Each call takes the next of 16 queue counters, so up to 16
  * runs of one plan may be in flight at once (on different streams, or a
@@ -260,7 +266,8 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *plan);
 int32_t syndrome_ldpc_plan_run(SyndromeLdpcPlan *plan, const void *d_in, void *d_post, uint8_t *d_hard,
                                int32_t *d_iters, size_t iterations, float scale, int16_t app_clamp, uint32_t flags,
                                void *stream);
-/* The main launch's block, grid and LDS bytes; items of both launches. */
+/* The main launch's block, grid and LDS bytes (or the first launch that has
+ * items, when the main launch has none); items of all launches. */
 int32_t syndrome_ldpc_plan_shape(const SyndromeLdpcPlan *plan, int32_t *out_block, int32_t *out_grid,
                                  int32_t *out_items, int32_t *out_lds_bytes);
 /* Launch-shape tuner: times the plan at each workgroup size from the

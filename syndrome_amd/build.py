@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "libsyndrome_amd.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["ldpc_fast_f32_bg1.hip", "ldpc_fastx_f32_bg1.hip", "ldpc_fast_i8_bg1.hip", "ldpc_fast_f32_bg2.hip",
            "ldpc_fast_wifi.hip", "ldpc_pack_i8_bg2.hip",
-           "ldpc_persist_f32.hip", "ldpc_persist_f32_small.hip", "ldpc_persist_i8.hip", "ldpc_resident.hip", "ldpc_tb.hip", "ldpc_encode.hip", "ldpc_dematch.hip", "ldpc_plain.hip", "ldpc_wide.hip", "ldpc_plan.hip", "ldpc_serve.hip", "ldpc_capi.hip",
+           "ldpc_persist_f32.hip", "ldpc_persist_f32_small.hip", "ldpc_persist_f32_small_bg2.hip", "ldpc_persist_i8.hip", "ldpc_resident.hip", "ldpc_tb.hip", "ldpc_encode.hip", "ldpc_dematch.hip", "ldpc_plain.hip", "ldpc_wide.hip", "ldpc_plan.hip", "ldpc_serve.hip", "ldpc_capi.hip",
            "code_host.cpp"]
 ARCH = os.environ.get("SYNDROME_OFFLOAD_ARCH", "gfx950")
 

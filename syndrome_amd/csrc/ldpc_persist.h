@@ -68,12 +68,15 @@ __device__ __forceinline__ void persist_body(const SynCode *__restrict__ codes, 
 static __global__ void reset_counter(int *head) { *head = 0; }
 
 // Work items of a plan's launch that holds codes with Z > 256 (512-lane
-// workgroups) and of its launch for Z <= 256 (256-lane workgroups): the
-// f32 kernels carry compile-time-Z bodies for the largest lifting sizes of
-// each part, where the work of a batch with Z spread over the 51 sizes
-// concentrates (work goes with E*Z).
+// workgroups) and of its launches for Z <= 256: the f32 kernels carry
+// compile-time-Z bodies for the largest lifting sizes of each part, where
+// the work of a batch with Z spread over the 51 sizes concentrates (work
+// goes with E*Z).  The Z <= 256 codes of the two base graphs go to two
+// kernels (PART 1: BG1, PART 2: BG2), each with its graph's 13 bodies, in
+// translation units of their own: one kernel with all 26 takes the
+// compiler several times as long as two with 13 each.
 using ZBig = ZSet<384, 352, 320, 288>;
-using ZSmall = ZSet<256, 240, 224, 208>;
+using ZSmall = ZSet<256, 240, 224, 208, 192, 176, 160, 144, 128, 120, 112, 104, 96>;
 
 // The int8 kernel holds three waves per SIMD (168 VGPRs), the budget of
 // its pair-packed BG2 body.
@@ -84,8 +87,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(I8 ? 3
                                                        const long long *__restrict__ offs,
                                                        const int *__restrict__ nfill, SynLaunch a)
 {
-    using Z = std::conditional_t<I8, ZSet<>, std::conditional_t<PART == 0, ZBig, ZSmall>>;
-    persist_body<I8, Z, Z>(codes, items, n_items, head, ids, offs, nfill, a);
+    using Z1 = std::conditional_t<I8 || PART == 2, ZSet<>, std::conditional_t<PART == 0, ZBig, ZSmall>>;
+    using Z2 = std::conditional_t<I8 || PART == 1, ZSet<>, std::conditional_t<PART == 0, ZBig, ZSmall>>;
+    persist_body<I8, Z1, Z2>(codes, items, n_items, head, ids, offs, nfill, a);
 }
 
 }  // namespace syn

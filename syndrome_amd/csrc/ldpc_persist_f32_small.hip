@@ -1,4 +1,4 @@
-// Work-queue decode (f32) for launches whose codes all have Z <= 256.
+// Work-queue decode (f32) for launches whose codes are BG1 codes of Z <= 256.
 #include "ldpc_persist.h"
 
 namespace syn {

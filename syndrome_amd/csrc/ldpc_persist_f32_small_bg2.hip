@@ -1,0 +1,8 @@
+// Work-queue decode (f32) for launches whose codes are BG2 codes of Z <= 256.
+#include "ldpc_persist.h"
+
+namespace syn {
+
+const void *persist_kernel_f32_small_bg2() { return (const void *)persist_decode<false, 2>; }
+
+}  // namespace syn

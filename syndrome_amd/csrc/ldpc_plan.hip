@@ -19,6 +19,7 @@
 namespace syn {
 const void *persist_kernel_f32_big();
 const void *persist_kernel_f32_small();
+const void *persist_kernel_f32_small_bg2();
 const void *persist_kernel_i8();
 const void *persistx_kernel_f32_bg1(int z);
 int persistx_lanes_f32_bg1();
@@ -56,6 +57,12 @@ struct SyndromeLdpcPlan {
     // the kernel of each launch: f32 codes with Z > 256 in a launch take the
     // kernel whose compile-time-Z bodies are the large sizes
     const void *fn_main = nullptr, *fn_small = nullptr;
+    // f32 BG2 codes of Z <= 256: a launch of the kernel with their
+    // compile-time-Z bodies (the Z <= 256 launch, or the main launch when
+    // the plan has no split, keeps the BG1 codes)
+    int block_2 = 0, grid_2 = 0, n_items_2 = 0;
+    size_t lds_2 = 0;
+    const void *fn_2 = nullptr;
     // f32 BG1 codes of Z > 256: a third launch of the two-codeword kernel that
     // keeps the extension columns in the posterior buffer (ldpc_fastx.h),
     // for runs that write the posterior back; other runs decode them in the
@@ -63,13 +70,16 @@ struct SyndromeLdpcPlan {
     // list with them (the same as the main list when the plan has none)
     int block_x = 0, grid_x = 0, n_items_x = 0;
     size_t lds_x = 0;
+    // whether the extension launch pays when it runs after the others (a
+    // run under stream capture); a forked run always uses it
+    bool x_alone = false;
     int grid_mx = 0, n_items_mx = 0;
     size_t lds_mx = 0;
     const void *fn_x = nullptr, *fn_mx = nullptr;
     std::vector<int> cws_of_code;
     // device buffers
     SynCode *d_codes = nullptr;
-    SynItem *d_items = nullptr, *d_items_s = nullptr, *d_items_x = nullptr, *d_items_mx = nullptr;
+    SynItem *d_items = nullptr, *d_items_s = nullptr, *d_items_x = nullptr, *d_items_mx = nullptr, *d_items_2 = nullptr;
     int *d_ids = nullptr, *d_nfill = nullptr, *d_head = nullptr;
     long long *d_offs = nullptr;
     SynDematchCb *d_dm = nullptr;
@@ -77,19 +87,19 @@ struct SyndromeLdpcPlan {
     int n_dm_chunks = 0;
     SynTb *d_tbs = nullptr;
     int n_tbs = 0;
-    // queue counters: RUN_RING runs, three launches each, one 128-byte line
+    // queue counters: RUN_RING runs, four launches each, one 128-byte line
     // per counter; run k uses ring entry k % RUN_RING
     unsigned run_seq = 0;
     // device tables replaced by tune / set_dematch / set_tbs: kept until the
     // plan is destroyed, so a HIP graph captured before the change still
     // replays valid pointers (with the launch shape it was captured with)
     std::vector<void *> retired;
-    // Streams and events of an eager run's fork: the launches after the
-    // first go on side streams that wait for the caller's stream and that
-    // it waits for in turn, so their workgroups fill CUs that another
-    // launch's drained queue leaves idle.
-    hipStream_t side[2] = {nullptr, nullptr};
-    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+    // Stream and events of an eager run's fork: some launches go on a side
+    // stream that waits for the caller's stream and that it waits for in
+    // turn, so their workgroups fill CUs that another launch's drained
+    // queue leaves idle.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::mutex mu;
 };
 
@@ -137,8 +147,7 @@ int32_t upload(SyndromeLdpcPlan *p, T **dst, const std::vector<T> &src)
 constexpr int SPLIT_LANES = 256;
 constexpr int HEAD_STRIDE = 32;
 constexpr unsigned RUN_RING = 16;
-constexpr int LAUNCHES = 3;
-static_assert(sizeof(SyndromeLdpcPlan::side) / sizeof(hipStream_t) == LAUNCHES - 1, "one side stream per later launch");
+constexpr int LAUNCHES = 4;
 constexpr int X_MIN_ITEMS_PER_CU = 4;
 
 // A code of the two-codeword extension-in-HBM kernel's launch.
@@ -174,12 +183,13 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     for (const SynCode &code : p->codes) max_z = std::max(max_z, code.z);
     const bool split = max_z > SPLIT_LANES && block > SPLIT_LANES;
     std::vector<int> cws(nc), part(nc, 0), cws_mx(nc, 0);
-    size_t lds[3] = {0, 0, 0};
+    size_t lds[5] = {0, 0, 0, 0, 0};
     // the dynamic LDS an item may take: the device's per-workgroup limit less
     // the largest static LDS of the kernels this plan may launch (their item
     // tables)
     size_t lds_cap = (size_t)p->lds_limit, static_max = 0;
-    for (const void *fn : {syn::persist_kernel_i8(), syn::persist_kernel_f32_big(), syn::persist_kernel_f32_small()}) {
+    for (const void *fn : {syn::persist_kernel_i8(), syn::persist_kernel_f32_big(), syn::persist_kernel_f32_small(),
+                           syn::persist_kernel_f32_small_bg2()}) {
         hipFuncAttributes fa{};
         if (hipFuncGetAttributes(&fa, fn) != hipSuccess) return SYNDROME_DEVICE_ERROR;
         static_max = std::max(static_max, (size_t)fa.sharedSizeBytes);
@@ -205,11 +215,12 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     auto fits = [&](const SynCode &code, bool pk, int lanes, size_t cap) {
         return group_bytes(code, fit(code, pk, lanes, cap), p->i8, pk) <= cap;
     };
-    // The extension launch runs after the others on the stream.  When its
-    // codes are a small part of the plan (config-4-like mixes: a few items
-    // per size), its tail would leave most CUs idle behind the main launch,
-    // so it is used only where they hold at least half the plan's work
-    // (E*Z per block) or a few items per CU.
+    // A forked run (plan_launch) overlaps the extension launch with the
+    // others.  A run in sequence has it after them on the stream: when its
+    // codes are a small part of the plan (a few items per size), its tail
+    // would leave most CUs idle behind the main launch, so such a run uses
+    // it only where they hold at least half the plan's work (E*Z per block)
+    // or a few items per CU.
     size_t x_items = 0;
     double x_work = 0.0, all_work = 0.0;
     {
@@ -224,8 +235,7 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
             }
         }
     }
-    const bool use_x_launch =
-        x_items > 0 && (2.0 * x_work >= all_work || x_items >= (size_t)X_MIN_ITEMS_PER_CU * (size_t)std::max(1, p->cus));
+    p->x_alone = 2.0 * x_work >= all_work || x_items >= (size_t)X_MIN_ITEMS_PER_CU * (size_t)std::max(1, p->cus);
     std::vector<char> packed(nc, 0);
     for (size_t c = 0; c < nc; ++c) {
         const SynCode &code = p->codes[c];
@@ -246,7 +256,7 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         packed[c] = pk;
         int xl = 0, xc = 1;
         const size_t xb = x_code(code, p->i8) ? (size_t)syn::fastx_shape_f32_bg1(code.z, &xl, &xc) : 0;
-        if (use_x_launch && x_code(code, p->i8) && xb + 64 <= (size_t)p->lds_limit) {
+        if (x_code(code, p->i8) && xb + 64 <= (size_t)p->lds_limit) {
             // the extension-in-HBM launch; cws_mx: its main-launch form
             part[c] = 2;
             cws_mx[c] = k;
@@ -256,6 +266,9 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
             (void)xl;
             continue;
         }
+        // f32 BG2 codes of Z <= 256, from the Z <= 256 launch or from an
+        // unsplit main launch: the BG2 kernel's launch, of the same lanes
+        if (!p->i8 && code.kind == SYN_KIND_BG2 && code.z <= SPLIT_LANES && (part[c] == 1 || !split)) part[c] = 4;
         lds[part[c]] = std::max(lds[part[c]], group_bytes(code, k, p->i8, pk));
     }
     size_t lds_mx = lds[0];
@@ -268,8 +281,8 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         long long cost;
     };
     // items[3]: the main list with the extension-launch codes in main-launch
-    // form (its *_mx list)
-    std::vector<Item> items[4];
+    // form (its *_mx list); items[4]: the f32 BG2 codes of Z <= 256
+    std::vector<Item> items[5];
     std::vector<int> ids;
     auto add = [&](size_t c, size_t j0, int k, int q, bool pk) {
         for (size_t j = 0; j < members[c].size(); j += (size_t)k) {
@@ -287,8 +300,8 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         if (part[c] == 2) add(c, j0, cws_mx[c], 3, false);
         else if (part[c] == 0) add(c, j0, cws[c], 3, packed[c]);
     }
-    std::vector<SynItem> flat[4];
-    for (int q = 0; q < 4; ++q) {
+    std::vector<SynItem> flat[5];
+    for (int q = 0; q < 5; ++q) {
         // longest-processing-time first: the queue hands out big items early
         std::stable_sort(items[q].begin(), items[q].end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
         for (auto &x : items[q]) flat[q].push_back(x.it);
@@ -298,11 +311,12 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     if (int32_t rc = upload(p, &p->d_items_s, flat[1])) return rc;
     if (int32_t rc = upload(p, &p->d_items_x, flat[2])) return rc;
     if (int32_t rc = upload(p, &p->d_items_mx, has_x ? flat[3] : std::vector<SynItem>())) return rc;
+    if (int32_t rc = upload(p, &p->d_items_2, flat[4])) return rc;
     if (int32_t rc = upload(p, &p->d_ids, ids)) return rc;
     int main_z = 0, mx_z = 0;  // largest Z in the main launch, without and with the extension-launch codes
     for (size_t c = 0; c < nc; ++c) {
         if (!part[c]) main_z = std::max(main_z, p->codes[c].z);
-        if (part[c] != 1) mx_z = std::max(mx_z, p->codes[c].z);
+        if (part[c] == 0 || part[c] == 2) mx_z = std::max(mx_z, p->codes[c].z);
     }
     const void *fn = p->i8 ? syn::persist_kernel_i8()
                            : (main_z > SPLIT_LANES ? syn::persist_kernel_f32_big() : syn::persist_kernel_f32_small());
@@ -325,6 +339,18 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn_s, SPLIT_LANES, lds[1]) != hipSuccess || per_cu < 1)
             per_cu = 1;
         p->grid_s = std::max(1, std::min(p->n_items_s, per_cu * p->cus));
+    }
+    p->block_2 = split ? SPLIT_LANES : block;
+    p->lds_2 = lds[4];
+    p->n_items_2 = (int)flat[4].size();
+    p->grid_2 = 0;
+    p->fn_2 = syn::persist_kernel_f32_small_bg2();
+    if (p->n_items_2) {
+        if (syn::allow_max_dynamic_lds(p->fn_2)) return SYNDROME_DEVICE_ERROR;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->fn_2, p->block_2, lds[4]) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        p->grid_2 = std::max(1, std::min(p->n_items_2, per_cu * p->cus));
     }
     p->cws_of_code = cws;
     p->n_items_x = (int)flat[2].size();
@@ -388,9 +414,18 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
     // this run's counters: runs in flight at once (on different streams, or
     // a graph replay beside an eager run) never share one
     int *heads = p->d_head + (size_t)(p->run_seq++ % RUN_RING) * LAUNCHES * HEAD_STRIDE;
+    // Each launch is a persistent grid whose workgroups leave once its queue
+    // is drained, and none waits for another, so launches may run in any
+    // overlap.  An eager run forks them over the caller's stream and the
+    // plan's side stream (below).  A run under stream capture keeps them in
+    // sequence on the caller's stream: a side stream joined into one
+    // caller's capture would capture another caller's eager run.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const bool can_fork = p->ev_fork && hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
     // the extension-in-HBM launch needs the posterior buffer and an
-    // iteration; without them its codes go in the main launch
-    const bool use_x = p->n_items_x > 0 && (a.flags & SYN_F_WRITE_POST) && a.iterations >= 1;
+    // iteration; without them, or where it does not pay in sequence, its
+    // codes go in the main launch
+    const bool use_x = p->n_items_x > 0 && (a.flags & SYN_F_WRITE_POST) && a.iterations >= 1 && (can_fork || p->x_alone);
     struct L {
         const void *fn;
         const SynItem *items;
@@ -400,9 +435,10 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
         use_x ? L{p->fn_main, p->d_items, p->n_items, p->grid, p->block, p->lds}
               : L{p->fn_mx, p->n_items_x ? p->d_items_mx : p->d_items, p->n_items_mx, p->grid_mx, p->block, p->lds_mx},
         L{p->fn_small, p->d_items_s, p->n_items_s, p->grid_s, p->block_s, p->lds_s},
-        L{p->fn_x, p->d_items_x, use_x ? p->n_items_x : 0, p->grid_x, p->block_x, p->lds_x}};
-    // the main launch, the small codes', the extension launch (each with
-    // its own counter)
+        L{p->fn_x, p->d_items_x, use_x ? p->n_items_x : 0, p->grid_x, p->block_x, p->lds_x},
+        L{p->fn_2, p->d_items_2, p->n_items_2, p->grid_2, p->block_2, p->lds_2}};
+    // the main launch, the small codes', the extension launch, the small BG2
+    // codes' (each with its own counter)
     const auto launch = [&](int q, hipStream_t s) -> int32_t {
         const SynItem *items = ls[q].items;
         int n_items = ls[q].n;
@@ -413,33 +449,31 @@ int32_t plan_launch(SyndromeLdpcPlan *p, const void *in, void *post, uint8_t *ha
             return SYNDROME_DEVICE_ERROR;
         return SYNDROME_OK;
     };
-    int act[LAUNCHES], n_act = 0;
-    for (int q = 0; q < LAUNCHES; ++q)
-        if (ls[q].n) act[n_act++] = q;
-    // Each launch is a persistent grid whose workgroups leave once its queue
-    // is drained, and none waits for another, so launches may run in any
-    // overlap.  An eager run forks them: the first on the caller's stream,
-    // the others on the plan's side streams.  A run under stream capture
-    // keeps them in sequence on the caller's stream: a side stream joined
-    // into one caller's capture would capture another caller's eager run.
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    const bool fork = n_act > 1 && p->ev_fork && hipStreamIsCapturing(stream, &cs) == hipSuccess &&
-                      cs == hipStreamCaptureStatusNone;
+    int n_act = 0;
+    for (int q = 0; q < LAUNCHES; ++q) n_act += ls[q].n > 0;
+    // Two streams: the caller's and the side stream, each running its
+    // launches in order.  With the extension launch, [main, then the small
+    // BG1 codes'] runs beside [extension, then the small BG2 codes']: the
+    // two launches of whole-CU workgroups start together with the longest
+    // items, and each stream's small-code launch fills its CUs as they
+    // drain.  Without it, the main launch runs beside the small codes'
+    // launches.  (Uneven workload, f32 decode-only: 9.22 Gbit/s in
+    // sequence, 10.03 so; int8: 8.33 in sequence, 9.5 beside.)
+    const bool side_of[LAUNCHES] = {false, !use_x, true, true};
+    const bool fork = can_fork && n_act > 1;
     if (!fork) {
-        for (int i = 0; i < n_act; ++i)
-            if (int32_t rc = launch(act[i], stream)) return rc;
+        for (int q : {0, 2, 1, 3})
+            if (ls[q].n)
+                if (int32_t rc = launch(q, stream)) return rc;
         return SYNDROME_OK;
     }
-    if (hipEventRecord(p->ev_fork, stream) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-    for (int i = 1; i < n_act; ++i) {
-        hipStream_t s = p->side[i - 1];
-        if (hipStreamWaitEvent(s, p->ev_fork, 0) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-        if (int32_t rc = launch(act[i], s)) return rc;
-        if (hipEventRecord(p->ev_join[i - 1], s) != hipSuccess) return SYNDROME_DEVICE_ERROR;
-    }
-    int32_t rc = launch(act[0], stream);
-    for (int i = 1; i < n_act; ++i)
-        if (hipStreamWaitEvent(stream, p->ev_join[i - 1], 0) != hipSuccess) rc = SYNDROME_DEVICE_ERROR;
+    if (hipEventRecord(p->ev_fork, stream) != hipSuccess || hipStreamWaitEvent(p->side, p->ev_fork, 0) != hipSuccess)
+        return SYNDROME_DEVICE_ERROR;
+    int32_t rc = SYNDROME_OK;
+    for (int q : {0, 2, 1, 3})
+        if (ls[q].n && (rc = launch(q, side_of[q] ? p->side : stream))) break;
+    if (hipEventRecord(p->ev_join, p->side) != hipSuccess || hipStreamWaitEvent(stream, p->ev_join, 0) != hipSuccess)
+        rc = SYNDROME_DEVICE_ERROR;
     return rc;
 }
 
@@ -490,11 +524,10 @@ int32_t syndrome_ldpc_plan_create(const int32_t *bg, const int32_t *z, const int
         if (int32_t rc = upload(p.get(), &p->d_nfill, p->nfill)) return rc;
         if (hipMalloc((void **)&p->d_head, RUN_RING * LAUNCHES * HEAD_STRIDE * sizeof(int)) != hipSuccess)
             return SYNDROME_DEVICE_ERROR;
-        for (int i = 0; i < LAUNCHES - 1; ++i)
-            if (hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&p->ev_join[i], hipEventDisableTiming) != hipSuccess)
-                return SYNDROME_DEVICE_ERROR;
-        if (hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess) return SYNDROME_DEVICE_ERROR;
+        if (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess)
+            return SYNDROME_DEVICE_ERROR;
         const int min_block = ((max_z + 63) / 64) * 64;
         const int b = block > 0 ? block : std::max(min_block, 512);
         if (int32_t rc = build_items(p.get(), std::max(b, min_block))) return rc;
@@ -509,15 +542,13 @@ void syndrome_ldpc_plan_destroy(SyndromeLdpcPlan *p)
     try {
         DeviceScope ds(p->device);
         for (void *q : {(void *)p->d_codes, (void *)p->d_items, (void *)p->d_items_s, (void *)p->d_items_x,
-                        (void *)p->d_items_mx, (void *)p->d_ids, (void *)p->d_nfill,
+                        (void *)p->d_items_mx, (void *)p->d_items_2, (void *)p->d_ids, (void *)p->d_nfill,
                         (void *)p->d_head, (void *)p->d_offs, (void *)p->d_dm, (void *)p->d_dm_chunks,
                         (void *)p->d_tbs})
             if (q) (void)hipFree(q);
         for (void *q : p->retired) (void)hipFree(q);
-        for (int i = 0; i < 2; ++i) {
-            if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
-            if (p->ev_join[i]) (void)hipEventDestroy(p->ev_join[i]);
-        }
+        if (p->side) (void)hipStreamDestroy(p->side);
+        if (p->ev_join) (void)hipEventDestroy(p->ev_join);
         if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
     } catch (...) {
     }
@@ -543,13 +574,14 @@ int32_t syndrome_ldpc_plan_shape(const SyndromeLdpcPlan *p, int32_t *block, int3
 {
     return guard([&]() -> int32_t {
         if (!p) return SYNDROME_NULL_POINTER;
-        // the main launch's shape, or the extension launch's when the main
-        // launch has no items
+        // the main launch's shape, or, when the main launch has no items, the
+        // extension launch's or the small BG2 codes'
         const bool x = p->n_items == 0 && p->n_items_x > 0;
-        if (block) *block = x ? p->block_x : p->block;
-        if (grid) *grid = x ? p->grid_x : p->grid;
-        if (items) *items = p->n_items + p->n_items_s + p->n_items_x;  // every launch's items
-        if (lds_bytes) *lds_bytes = (int32_t)(x ? p->lds_x : p->lds);
+        const bool b2 = p->n_items == 0 && !x && p->n_items_2 > 0;
+        if (block) *block = x ? p->block_x : b2 ? p->block_2 : p->block;
+        if (grid) *grid = x ? p->grid_x : b2 ? p->grid_2 : p->grid;
+        if (items) *items = p->n_items + p->n_items_s + p->n_items_x + p->n_items_2;  // every launch's items
+        if (lds_bytes) *lds_bytes = (int32_t)(x ? p->lds_x : b2 ? p->lds_2 : p->lds);
         return SYNDROME_OK;
     });
 }
