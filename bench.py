@@ -535,7 +535,7 @@ def bench_uneven(torch, args):
     t8 = _timed(lambda: plan8.run(d_llr, d_post8, d_hard, d_it, args.iters, f8, quant=q), 5, torch)
     bits = float(k.sum())
     edges = np.where(bg == 0, 316, 197) * z
-    roof = valu_roofline("syn::persist_decode<false, *> and syn::persistx_f32<0, 384> (every launch of the plan)", "f32",
+    roof = valu_roofline("syn::persist_decode<false, *> and syn::persistx_f32<0, ...> (every launch of the plan)", "f32",
                          float((edges * its).sum()), t_dec * 1e3, float(n.sum()) * 9, "persist_f32_uneven_b32768")
     cpu = cpu8 = None
     if not args.no_cpu:

@@ -1,6 +1,6 @@
 // Register-resident f32 decode of BG1 with the extension columns'
 // posteriors in HBM (ldpc_fastx.h): two codewords per CU, for the lifting
-// sizes above 256.
+// sizes above 256, per-config and work-queue kernels.
 #include "ldpc_fastx.h"
 
 namespace syn {
@@ -16,13 +16,19 @@ const void *fastx_kernel_f32_bg1(int z)
     }
 }
 
-// The work-queue kernel: Z=384 only.  Each further size in one kernel costs
+// The work-queue kernels: Z=384 alone, and all four sizes in one kernel for
+// plans that hold the others too.  Each further size in one kernel costs
 // registers for all of them (two sizes: 87 VGPR spills, against 74 for
-// one), so the other sizes stay in the work queue's main launch.
+// one), so a plan of Z=384 blocks only keeps the one-size kernel.
 const void *persistx_kernel_f32_bg1(int z)
 {
-    if (z == 384) return (const void *)persistx_f32<SYN_KIND_BG1, 384>;
-    return nullptr;
+    switch (z) {
+    case 384: return (const void *)persistx_f32<SYN_KIND_BG1, 384>;
+    case 352:
+    case 320:
+    case 288: return (const void *)persistx_f32<SYN_KIND_BG1, 384, 352, 320, 288>;
+    default: return nullptr;
+    }
 }
 
 int persistx_lanes_f32_bg1() { return fastx_max_lanes<384>(); }

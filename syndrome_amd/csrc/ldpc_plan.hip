@@ -359,8 +359,9 @@ int32_t build_items(SyndromeLdpcPlan *p, int block)
     p->fn_x = nullptr;
     if (has_x) {
         const void *fx = nullptr;
-        for (size_t c = 0; c < nc && !fx; ++c)
-            if (part[c] == 2) fx = syn::persistx_kernel_f32_bg1(p->codes[c].z);
+        // the Z=384 kernel, or the kernel of all four sizes when another is here
+        for (size_t c = 0; c < nc; ++c)
+            if (part[c] == 2 && (!fx || p->codes[c].z != 384)) fx = syn::persistx_kernel_f32_bg1(p->codes[c].z);
         if (syn::allow_max_dynamic_lds(fx)) return SYNDROME_DEVICE_ERROR;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fx, p->block_x, lds[2]) != hipSuccess || per_cu < 1)
             per_cu = 1;

@@ -156,15 +156,20 @@ def test_fastx_one_iteration_and_cap(dev, oracle):
     assert oracle.fnv1a(hard[0]) == 0xA2448E8CF5272825  # the reference's golden checksum
 
 
-def test_plan_extension_launch_with_and_without_posterior(dev, oracle):
-    """A work-queue plan decodes its f32 BG1 Z=384 blocks in a launch of
-    the two-codeword kernel when the run writes the posterior back, and in
-    the main launch, one codeword per workgroup, when it does not: the same
-    bits both ways, and each block as the oracle decodes it alone (fillers
-    included).  Three Z=384 blocks: the last item holds one codeword."""
+@pytest.mark.parametrize("codes", [
+    [(0, 384), (1, 208), (0, 384), (0, 52), (0, 384), (1, 384)],
+    [(0, 352), (0, 320), (1, 208), (0, 288), (0, 384), (0, 352), (0, 288), (1, 384), (0, 320), (0, 96)],
+], ids=["z384", "mixed_sizes"])
+def test_plan_extension_launch_with_and_without_posterior(dev, oracle, codes):
+    """A work-queue plan decodes its f32 BG1 blocks of Z > 256 in a launch
+    of the two-codeword kernel when the run writes the posterior back (the
+    Z=384-only kernel, or the kernel of all four sizes when the plan has
+    others), and in the main launch, one codeword per workgroup, when it
+    does not: the same bits both ways, and each block as the oracle decodes
+    it alone (fillers included).  Odd counts per size leave items that hold
+    one codeword."""
     import syndrome_amd._lib as L
     from syndrome_amd import DecodePlan
-    codes = [(0, 384), (1, 208), (0, 384), (0, 52), (0, 384), (1, 384)]
     rng = np.random.default_rng(21)
     bg, z, nf, blocks = [], [], [], []
     for i, (b, zz) in enumerate(codes):
